@@ -1,0 +1,89 @@
+/*
+ * tfs_mesh.h -- host-side C ABI of libtfs_mesh.so: CRS topology from a face
+ * list, and the synthetic hex-mesh cases the solver is measured on.
+ *
+ * The topology builder is the counterpart of the reference's
+ * Matrix_Mod_Topology (Sources/Process/Matrix_Mod/Topology.f90:2-127): same
+ * layout (columns ascending within a row, diagonal position in dia, face ->
+ * entry map in pos), selectable 0- or 1-based output.
+ *
+ * The case generator builds a structured hex box (tanh clustering, node
+ * jitter, random cell numbering) and assembles the two matrix families the
+ * solver serves:
+ *   TFS_CASE_POISSON  -- the fractional-step pressure matrix,
+ *                        Sources/Process/Pressure_Matrix_Fractional.f90:26-48
+ *                        with Scoef from Sources/Process/Compute_Geometry.f90:22-49;
+ *   TFS_CASE_MOMENTUM -- fully implicit upwind convection-diffusion with an
+ *                        inertia diagonal, Sources/Process/NewUVW.f90:349-384,500-506.
+ * All random draws come from SplitMix64 (counter form), so every process and
+ * every language sees identical inputs for a given seed.
+ */
+#ifndef TFS_MESH_H
+#define TFS_MESH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- random numbers ------------------------------------------------------ */
+
+/* Uniform in [0,1) from 53 bits of SplitMix64: the i-th draw (i >= 0) of the
+ * stream keyed by (seed, tag). */
+double tfs_rng_u01(uint64_t seed, uint64_t tag, uint64_t i);
+
+/* ---- topology (Matrix_Mod_Topology) -------------------------------------- */
+
+/* Number of stored entries for faces_c (2 x n_faces, column-major, 1-based
+ * cells, c2 <= 0 marks a boundary face).  Topology.f90:33-48. */
+int64_t tfs_topology_nnz(int n_cells, int64_t n_faces, const int *faces_c);
+
+/* Fills row[n+1], col[nnz], dia[n] and (if non-NULL) pos[2*n_faces] with
+ * index base `base` (0 or 1).  pos entries of boundary faces are base-1.
+ * Returns 0, or a negative value on invalid input (bad cell number, nnz
+ * beyond the int32 range the reference's default integers hold).
+ * Topology.f90:54-121. */
+int tfs_topology_build(int n_cells, int64_t n_faces, const int *faces_c,
+                       int base, int *row, int *col, int *dia, int *pos);
+
+/* ---- synthetic cases ----------------------------------------------------- */
+
+enum { TFS_CASE_POISSON = 0, TFS_CASE_MOMENTUM = 1 };
+
+typedef struct {
+  int kind;            /* TFS_CASE_POISSON or TFS_CASE_MOMENTUM              */
+  int nx, ny, nz;      /* cells per axis                                      */
+  double beta[3];      /* tanh clustering per axis, 0 = uniform               */
+  double jitter;       /* node jitter, fraction of local spacing (0.1 = 10%)  */
+  int permute;         /* 1 = random cell numbering                           */
+  int pin_first;       /* Poisson: double a_11 to fix the null space          */
+  int rhs_zero_mean;   /* remove the mean of the random RHS                   */
+  double nu;           /* momentum: kinematic viscosity                       */
+  double cfl;          /* momentum: dt = cfl * h_min / |u|max                 */
+  int n_modes;         /* momentum: Fourier modes of the vector potential     */
+  uint64_t seed;
+} tfs_case_params;
+
+typedef struct tfs_case tfs_case;
+
+int tfs_case_create(const tfs_case_params *p, tfs_case **out);
+void tfs_case_destroy(tfs_case *c);
+/* n_cells, n_faces (interior + boundary), nnz */
+void tfs_case_sizes(const tfs_case *c, int *n_cells, int64_t *n_faces, int64_t *nnz);
+/* CRS arrays with index base `base`, RHS b[n]. */
+int tfs_case_crs(const tfs_case *c, int base, int *row, int *col, int *dia,
+                 double *val, double *b);
+/* Face list as the reference Grid_Type holds it: faces_c[2*n_faces],
+ * 1-based cells, negative c2 for boundary faces. */
+int tfs_case_faces(const tfs_case *c, int *faces_c);
+/* Time step used for the momentum inertia term (0 for Poisson). */
+double tfs_case_dt(const tfs_case *c);
+/* Largest cell aspect ratio (max edge / min edge over the cell's 12 edges). */
+double tfs_case_max_aspect(const tfs_case *c);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

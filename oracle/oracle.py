@@ -1,0 +1,186 @@
+"""ctypes access to the oracle (TEST INFRASTRUCTURE ONLY).
+
+* ``Oracle``    -- the C restatement, oracle/_build/libtfs_oracle.so.
+* ``Reference`` -- the reference's own Fortran solver path compiled with the
+  flags of its FAST build, oracle/_ref/libtfs_ref.so (only where it was built).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker or the timed CPU baseline.  Matrices are
+passed 0-based (row/col/dia as numpy int32, val float64); ``Reference`` shifts
+them to the reference's 1-based numbering.
+"""
+import ctypes
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_ORACLE = HERE / "_build" / "libtfs_oracle.so"
+LIB_REF = HERE / "_ref" / "libtfs_ref.so"
+
+METHODS = {"cg": 0, "bicg": 1, "cgs": 2}
+
+_I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class _OrMatrix(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int),
+                ("row", ctypes.c_void_p), ("col", ctypes.c_void_p),
+                ("dia", ctypes.c_void_p), ("val", ctypes.c_void_p)]
+
+
+def _c32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _c64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Oracle:
+    """The C restatement of Solvers_Mod / Matrix_Mod_Topology."""
+
+    def __init__(self, path=LIB_ORACLE):
+        if not pathlib.Path(path).exists():
+            raise FileNotFoundError(f"{path} not built (make -C oracle)")
+        lib = ctypes.CDLL(str(path))
+        mp = ctypes.POINTER(_OrMatrix)
+        lib.or_prec_form.argtypes = [mp, ctypes.c_int, _F64P]
+        lib.or_prec_solve.argtypes = [mp, _F64P, ctypes.c_int, _F64P, _F64P]
+        lib.or_residual.argtypes = [mp, _F64P, _F64P]
+        lib.or_spmv.argtypes = [mp, _F64P, _F64P]
+        for f in (lib.or_cg, lib.or_bicg, lib.or_cgs):
+            f.restype = ctypes.c_int
+            f.argtypes = [mp, _F64P, _F64P, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                          ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                          ctypes.POINTER(ctypes.c_double)]
+        lib.or_topology_nnz.restype = ctypes.c_int
+        lib.or_topology_nnz.argtypes = [ctypes.c_int, ctypes.c_int, _I32P]
+        lib.or_topology.argtypes = [ctypes.c_int, ctypes.c_int, _I32P] + [_I32P] * 5
+        self.lib = lib
+
+    def _mat(self, row, col, dia, val):
+        keep = (_c32(row), _c32(col), _c32(dia), _c64(val))
+        m = _OrMatrix(len(keep[0]) - 1, keep[0].ctypes.data, keep[1].ctypes.data,
+                      keep[2].ctypes.data, keep[3].ctypes.data)
+        return m, keep
+
+    def prec_form(self, row, col, dia, val, prec):
+        m, keep = self._mat(row, col, dia, val)
+        d = np.zeros(m.n, np.float64)
+        self.lib.or_prec_form(ctypes.byref(m), prec, d)
+        return d
+
+    def prec_solve(self, row, col, dia, val, d, prec, b):
+        m, keep = self._mat(row, col, dia, val)
+        x = np.zeros(m.n, np.float64)
+        self.lib.or_prec_solve(ctypes.byref(m), _c64(d), prec, x, _c64(b))
+        return x
+
+    def residual(self, row, col, dia, val, x, r):
+        m, keep = self._mat(row, col, dia, val)
+        r = _c64(r).copy()
+        self.lib.or_residual(ctypes.byref(m), _c64(x), r)
+        return r
+
+    def spmv(self, row, col, dia, val, x):
+        m, keep = self._mat(row, col, dia, val)
+        y = np.zeros(m.n, np.float64)
+        self.lib.or_spmv(ctypes.byref(m), _c64(x), y)
+        return y
+
+    def solve(self, method, row, col, dia, val, b, prec=2, niter=1000, tol=1e-8, x0=None,
+              ini_res=0.0):
+        """Returns dict(x, r, niter, ini_res, fin_res) -- the reference outputs."""
+        m, keep = self._mat(row, col, dia, val)
+        x = np.zeros(m.n, np.float64) if x0 is None else _c64(x0).copy()
+        r = _c64(b).copy()
+        it = ctypes.c_int(niter)
+        ir = ctypes.c_double(ini_res)
+        fr = ctypes.c_double(0.0)
+        fn = {0: self.lib.or_cg, 1: self.lib.or_bicg, 2: self.lib.or_cgs}[METHODS.get(method, method)]
+        fn(ctypes.byref(m), x, r, prec, ctypes.byref(it), tol, ctypes.byref(ir), ctypes.byref(fr))
+        return dict(x=x, r=r, niter=it.value, ini_res=ir.value, fin_res=fr.value)
+
+    def topology(self, n_cells, faces_c):
+        faces_c = _c32(faces_c)
+        nf = faces_c.size // 2
+        nnz = self.lib.or_topology_nnz(n_cells, nf, faces_c)
+        row = np.zeros(n_cells + 1, np.int32)
+        col = np.zeros(nnz, np.int32)
+        dia = np.zeros(n_cells, np.int32)
+        mir = np.zeros(nnz, np.int32)
+        pos = np.zeros(2 * nf, np.int32)
+        self.lib.or_topology(n_cells, nf, faces_c, row, col, dia, mir, pos)
+        return dict(row=row, col=col, dia=dia, mir=mir, pos=pos)
+
+
+def reference_available(path=LIB_REF):
+    return pathlib.Path(path).exists()
+
+
+class Reference:
+    """The reference solver path itself (compiled from its sources)."""
+
+    def __init__(self, path=LIB_REF):
+        if not pathlib.Path(path).exists():
+            raise FileNotFoundError(f"{path} not built (make -C oracle ref)")
+        lib = ctypes.CDLL(str(path))
+        lib.tfs_ref_set_matrix.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, _I32P, _F64P]
+        lib.tfs_ref_solve.restype = ctypes.c_double
+        lib.tfs_ref_solve.argtypes = [ctypes.c_int, ctypes.c_int, _F64P, _F64P, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_int), ctypes.c_double,
+                                      ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double)]
+        lib.tfs_ref_prec.argtypes = [ctypes.c_int, ctypes.c_int, _F64P, _F64P, _F64P]
+        lib.tfs_ref_residual.argtypes = [ctypes.c_int, _F64P, _F64P]
+        lib.tfs_ref_topology.restype = ctypes.c_int
+        lib.tfs_ref_topology.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _I32P,
+                                         ctypes.c_int] + [_I32P] * 5
+        self.lib = lib
+        self.n = 0
+
+    def set_matrix(self, row, col, dia, val):
+        """0-based CRS in; stored 1-based inside the reference Matrix_Type."""
+        row = _c32(row) + 1
+        col = _c32(col) + 1
+        dia = _c32(dia) + 1
+        self.n = len(row) - 1
+        self.lib.tfs_ref_set_matrix(self.n, int(row[-1] - 1), row, col, dia, _c64(val))
+
+    def solve(self, method, b, prec=2, niter=1000, tol=1e-8, x0=None, ini_res=0.0):
+        """Returns dict(x, r, niter, ini_res, fin_res, seconds)."""
+        x = np.zeros(self.n, np.float64) if x0 is None else _c64(x0).copy()
+        r = _c64(b).copy()
+        it = ctypes.c_int(niter)
+        ir = ctypes.c_double(ini_res)
+        fr = ctypes.c_double(0.0)
+        t = self.lib.tfs_ref_solve(METHODS.get(method, method), self.n, x, r, prec,
+                                   ctypes.byref(it), tol, ctypes.byref(ir), ctypes.byref(fr))
+        return dict(x=x, r=r, niter=it.value, ini_res=ir.value, fin_res=fr.value, seconds=t)
+
+    def prec(self, prec, b):
+        z = np.zeros(self.n, np.float64)
+        d = np.zeros(self.n, np.float64)
+        self.lib.tfs_ref_prec(self.n, prec, _c64(b), z, d)
+        return z, d
+
+    def residual(self, x, r):
+        r = _c64(r).copy()
+        self.lib.tfs_ref_residual(self.n, _c64(x), r)
+        return r
+
+    def topology(self, n_cells, faces_c, n_bnd=None):
+        faces_c = _c32(faces_c)
+        nf = faces_c.size // 2
+        if n_bnd is None:
+            n_bnd = int(max(0, -faces_c[1::2].min())) if nf else 0
+        cap = n_cells + 2 * nf
+        row = np.zeros(n_cells + 1, np.int32)
+        col = np.zeros(cap, np.int32)
+        dia = np.zeros(n_cells, np.int32)
+        mir = np.zeros(cap, np.int32)
+        pos = np.zeros(2 * nf, np.int32)
+        nnz = self.lib.tfs_ref_topology(n_cells, n_bnd, nf, faces_c, cap, row, col, dia, mir, pos)
+        return dict(row=row, col=col[:nnz], dia=dia, mir=mir[:nnz], pos=pos)

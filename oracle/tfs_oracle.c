@@ -1,0 +1,362 @@
+/*
+ * tfs_oracle.c -- CPU restatement of the T-FlowS linear-solver path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the oracle).  The GPU library never links or calls
+ * this file; tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * load it as the checker.
+ *
+ * Every function follows one reference routine loop by loop and cites it.
+ * Arrays are 0-based; the reference is 1-based Fortran.  Build with
+ * -ffp-contract=off so that a*b+c is never fused, as in the reference FAST
+ * build on x86-64 (no FMA in the baseline ISA).
+ *
+ * How each function is pinned (tests/test_oracle_*.py):
+ *   or_spmv / or_residual : dense brute force on tiny matrices, the 4x4 CRS
+ *                           example of Matrix_Mod.f90:15-34, Laplacian times a
+ *                           linear field, bitwise against the compiled reference.
+ *   or_prec_form          : closed form d_i = i/(i+1) on tridiag(-1,2,-1);
+ *                           diag(M) == diag(A) for symmetric A; bitwise against
+ *                           the compiled reference (through Cg's iterates).
+ *   or_prec_solve         : M z == r with M = (Dt+L) Dt^-1 (Dt+U) formed densely.
+ *   or_cg                 : textbook PCG (scipy cg with the same M) on small SPD
+ *                           systems; exact termination in <= n steps; bitwise
+ *                           against the compiled reference.
+ *   or_bicg               : bitwise identical to or_cg (the shadow sequence of the
+ *                           reference equals the primary one); bitwise against the
+ *                           compiled reference.
+ *   or_cgs                : dense solve on tiny nonsymmetric systems; bitwise
+ *                           against the compiled reference.
+ *   or_topology           : bitwise against the reference Matrix_Mod_Topology.
+ */
+#include "tfs_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Sources/Shared/allp_mod.f90:40 (real promoted to FP64 by -fdefault-real-8). */
+#define OR_TINY 1.e-30
+
+/* Sources/Process/Solvers_Mod/Prec_Form.f90:20-48 */
+void or_prec_form(const or_matrix *A, int prec, double *d) {
+  const int n = A->n;
+  if (prec == 1) { /* Prec_Form.f90:24-27 */
+    for (int i = 0; i < n; ++i) d[i] = A->val[A->dia[i]];
+  } else if (prec == 2) { /* Prec_Form.f90:32-40 */
+    for (int i = 0; i < n; ++i) {
+      double sum1 = A->val[A->dia[i]];
+      for (int j = A->row[i]; j < A->dia[i]; ++j) {
+        const int k = A->col[j];
+        sum1 = sum1 - d[k] * A->val[j] * A->val[j];
+      }
+      d[i] = 1.0 / sum1;
+    }
+  } else { /* Prec_Form.f90:45-48 */
+    for (int i = 0; i < n; ++i) d[i] = 1.0;
+  }
+}
+
+/* Sources/Process/Solvers_Mod/Prec_Solve.f90:31-75 */
+void or_prec_solve(const or_matrix *A, const double *d, int prec, double *x,
+                   const double *b) {
+  const int n = A->n;
+  if (prec == 1) { /* Prec_Solve.f90:35-38 */
+    for (int i = 0; i < n; ++i) x[i] = b[i] / d[i];
+  } else if (prec == 2) {
+    /* forward substitution, Prec_Solve.f90:46-53 */
+    for (int i = 0; i < n; ++i) {
+      double sum1 = b[i];
+      for (int j = A->row[i]; j < A->dia[i]; ++j)
+        sum1 = sum1 - A->val[j] * x[A->col[j]];
+      x[i] = sum1 * d[i];
+    }
+    /* scale pass, Prec_Solve.f90:55-57 */
+    for (int i = 0; i < n; ++i) x[i] = x[i] / (d[i] + OR_TINY);
+    /* backward substitution, Prec_Solve.f90:60-67 */
+    for (int i = n - 1; i >= 0; --i) {
+      double sum1 = x[i];
+      for (int j = A->dia[i] + 1; j < A->row[i + 1]; ++j)
+        sum1 = sum1 - A->val[j] * x[A->col[j]];
+      x[i] = sum1 * d[i];
+    }
+  } else { /* Prec_Solve.f90:72-75 */
+    for (int i = 0; i < n; ++i) x[i] = b[i];
+  }
+}
+
+/* Sources/Process/Solvers_Mod/Residual.f90:22-27 (buffer loop empty, n_proc=0) */
+void or_residual(const or_matrix *A, const double *x, double *r) {
+  for (int i = 0; i < A->n; ++i)
+    for (int j = A->row[i]; j < A->row[i + 1]; ++j)
+      r[i] = r[i] - A->val[j] * x[A->col[j]];
+}
+
+/* Sources/Process/Solvers_Mod/Cg.f90:128-134 */
+void or_spmv(const or_matrix *A, const double *x, double *y) {
+  for (int i = 0; i < A->n; ++i) {
+    double s = 0.0;
+    for (int j = A->row[i]; j < A->row[i + 1]; ++j) s = s + A->val[j] * x[A->col[j]];
+    y[i] = s;
+  }
+}
+
+static double or_sumsq(int n, const double *v) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s = s + v[i] * v[i];
+  return s;
+}
+
+static double or_dot(int n, const double *a, const double *b) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s = s + a[i] * b[i];
+  return s;
+}
+
+/* Sources/Process/Solvers_Mod/Cg.f90:36-181 */
+int or_cg(const or_matrix *A, double *x, double *r1, int prec, int *niter,
+          double tol, double *ini_res, double *fin_res) {
+  const int n = A->n;
+  const int cap = *niter;
+  double *d = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  double *p1 = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  double *q1 = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  double rho = 0.0, rhoold = 0.0, alfa, beta, error;
+  int iter;
+
+  or_prec_form(A, prec, d);                                  /* Cg.f90:42 */
+  const double bnrm2 = sqrt(or_sumsq(n, r1));                /* Cg.f90:48-53 */
+  if (bnrm2 < tol) { iter = 0; error = bnrm2; goto done; }   /* Cg.f90:55-58 */
+
+  or_residual(A, x, r1);                                     /* Cg.f90:63 */
+  memcpy(p1, r1, sizeof(double) * (size_t)n);                /* Cg.f90:68-70 */
+  error = sqrt(or_sumsq(n, r1));                             /* Cg.f90:75-80 */
+  *ini_res = error;                                          /* Cg.f90:85 */
+  if (error < tol) { iter = 0; goto done; }                  /* Cg.f90:87-90 */
+
+  for (iter = 1; iter <= cap; ++iter) {                      /* Cg.f90:97 */
+    or_prec_solve(A, d, prec, q1, r1);                       /* Cg.f90:103 */
+    rho = or_dot(n, r1, q1);                                 /* Cg.f90:108-112 */
+    if (iter == 1) {
+      memcpy(p1, q1, sizeof(double) * (size_t)n);
+    } else {
+      beta = rho / rhoold;                                   /* Cg.f90:119-122 */
+      for (int i = 0; i < n; ++i) p1[i] = q1[i] + beta * p1[i];
+    }
+    or_spmv(A, p1, q1);                                      /* Cg.f90:128-134 */
+    alfa = or_dot(n, p1, q1);                                /* Cg.f90:148-153 */
+    alfa = rho / alfa;
+    for (int i = 0; i < n; ++i) {                            /* Cg.f90:159-162 */
+      x[i] = x[i] + alfa * p1[i];
+      r1[i] = r1[i] - alfa * q1[i];
+    }
+    error = sqrt(or_sumsq(n, r1));                           /* Cg.f90:167-172 */
+    if (error < tol) goto done;                              /* Cg.f90:174 */
+    rhoold = rho;                                            /* Cg.f90:176 */
+  }
+  /* Fortran DO leaves iter == cap+1 on normal completion (Cg.f90:178,181). */
+done:
+  *fin_res = error;
+  *niter = iter;
+  free(d);
+  free(p1);
+  free(q1);
+  return 0;
+}
+
+/* Sources/Process/Solvers_Mod/Bicg.f90:39-194 */
+int or_bicg(const or_matrix *A, double *x, double *r1, int prec, int *niter,
+            double tol, double *ini_res, double *fin_res) {
+  const int n = A->n;
+  const int cap = *niter;
+  const size_t nb = sizeof(double) * (size_t)(n > 0 ? n : 1);
+  double *d = (double *)malloc(nb);
+  double *p1 = (double *)malloc(nb), *p2 = (double *)malloc(nb);
+  double *q1 = (double *)malloc(nb), *q2 = (double *)malloc(nb);
+  double *r2 = (double *)malloc(nb);
+  double rho = 0.0, rhoold = 0.0, alfa, beta, error;
+  int iter;
+
+  or_prec_form(A, prec, d);                                  /* Bicg.f90:45 */
+  const double bnrm2 = sqrt(or_sumsq(n, r1));                /* Bicg.f90:51-56 */
+  if (bnrm2 < tol) { iter = 0; error = bnrm2; goto done; }   /* Bicg.f90:58-61 */
+
+  or_residual(A, x, r1);                                     /* Bicg.f90:66 */
+  error = sqrt(or_sumsq(n, r1));                             /* Bicg.f90:71-76 */
+  *ini_res = error;                                          /* Bicg.f90:81 */
+  if (error < tol) { iter = 0; goto done; }                  /* Bicg.f90:83-86 */
+  memcpy(r2, r1, nb);                                        /* Bicg.f90:91-93 */
+
+  for (iter = 1; iter <= cap; ++iter) {                      /* Bicg.f90:100 */
+    or_prec_solve(A, d, prec, q1, r1);                       /* Bicg.f90:107 */
+    or_prec_solve(A, d, prec, q2, r2);                       /* Bicg.f90:108 */
+    rho = or_dot(n, q1, r2);                                 /* Bicg.f90:113-117 */
+    if (iter == 1) {                                         /* Bicg.f90:119-130 */
+      memcpy(p1, q1, nb);
+      memcpy(p2, q2, nb);
+    } else {
+      beta = rho / rhoold;
+      for (int i = 0; i < n; ++i) {
+        p1[i] = q1[i] + beta * p1[i];
+        p2[i] = q2[i] + beta * p2[i];
+      }
+    }
+    for (int i = 0; i < n; ++i) {                            /* Bicg.f90:136-144 */
+      double s1 = 0.0, s2 = 0.0;
+      for (int j = A->row[i]; j < A->row[i + 1]; ++j) {
+        s1 = s1 + A->val[j] * p1[A->col[j]];
+        s2 = s2 + A->val[j] * p2[A->col[j]];
+      }
+      q1[i] = s1;
+      q2[i] = s2;
+    }
+    alfa = or_dot(n, p2, q1);                                /* Bicg.f90:160-165 */
+    alfa = rho / alfa;
+    for (int i = 0; i < n; ++i) {                            /* Bicg.f90:171-175 */
+      x[i] = x[i] + alfa * p1[i];
+      r1[i] = r1[i] - alfa * q1[i];
+      r2[i] = r2[i] - alfa * q2[i];
+    }
+    error = sqrt(or_sumsq(n, r1));                           /* Bicg.f90:180-185 */
+    if (error < tol) goto done;                              /* Bicg.f90:187 */
+    rhoold = rho;                                            /* Bicg.f90:189 */
+  }
+done:
+  *fin_res = error;
+  *niter = iter;
+  free(d);
+  free(p1); free(p2); free(q1); free(q2); free(r2);
+  return 0;
+}
+
+/* Sources/Process/Solvers_Mod/Cgs.f90:43-230 */
+int or_cgs(const or_matrix *A, double *x, double *r1, int prec, int *niter,
+           double tol, double *ini_res, double *fin_res) {
+  const int n = A->n;
+  const int cap = *niter;
+  const size_t nb = sizeof(double) * (size_t)(n > 0 ? n : 1);
+  double *d = (double *)malloc(nb);
+  double *p1 = (double *)malloc(nb), *p2 = (double *)malloc(nb);
+  double *q1 = (double *)malloc(nb), *q2 = (double *)malloc(nb);
+  double *r2 = (double *)malloc(nb), *u1 = (double *)malloc(nb);
+  double *u2 = (double *)malloc(nb), *v2 = (double *)malloc(nb);
+  double *u1q1 = (double *)malloc(nb);
+  double rho = 0.0, rhoold = 0.0, alfa, beta, error;
+  int iter;
+
+  or_prec_form(A, prec, d);                                  /* Cgs.f90:49 */
+  const double bnrm2 = sqrt(or_sumsq(n, r1));                /* Cgs.f90:55-60 */
+  if (bnrm2 < tol) { iter = 0; error = bnrm2; goto done; }   /* Cgs.f90:62-65 */
+
+  or_residual(A, x, r1);                                     /* Cgs.f90:70 */
+  memcpy(r2, r1, nb);                                        /* Cgs.f90:75-77 */
+  error = sqrt(or_sumsq(n, r1));                             /* Cgs.f90:82-87 */
+  *ini_res = error;                                          /* Cgs.f90:92 */
+  if (error < tol) { iter = 0; goto done; }                  /* Cgs.f90:94-97 */
+
+  for (iter = 1; iter <= cap; ++iter) {                      /* Cgs.f90:104 */
+    rho = or_dot(n, r1, r2);                                 /* Cgs.f90:109-113 */
+    if (iter == 1) {                                         /* Cgs.f90:115-126 */
+      for (int i = 0; i < n; ++i) {
+        u1[i] = r1[i];
+        u2[i] = u1[i];
+      }
+    } else {
+      beta = rho / rhoold;
+      for (int i = 0; i < n; ++i) {
+        u1[i] = r1[i] + beta * q1[i];
+        u2[i] = u1[i] + beta * (q1[i] + beta * u2[i]);
+      }
+    }
+    or_prec_solve(A, d, prec, p2, u2);                       /* Cgs.f90:131 */
+    /* v2 = A p2, Cgs.f90:136-143.  The `alfa` accumulated inside that loop is
+     * overwritten at Cgs.f90:157 before any use, so it is not restated. */
+    or_spmv(A, p2, v2);
+    alfa = or_dot(n, r2, v2);                                /* Cgs.f90:157-162 */
+    alfa = rho / alfa;
+    for (int i = 0; i < n; ++i) q1[i] = u1[i] - alfa * v2[i]; /* Cgs.f90:167-169 */
+    for (int i = 0; i < n; ++i) u1q1[i] = u1[i] + q1[i];      /* Cgs.f90:174-176 */
+    or_prec_solve(A, d, prec, p1, u1q1);                     /* Cgs.f90:177 */
+    for (int i = 0; i < n; ++i) x[i] = x[i] + alfa * p1[i];  /* Cgs.f90:182-184 */
+    or_spmv(A, p1, q2);                                      /* Cgs.f90:189-195 */
+    for (int i = 0; i < n; ++i) r1[i] = r1[i] - alfa * q2[i]; /* Cgs.f90:209-211 */
+    error = sqrt(or_sumsq(n, r1));                           /* Cgs.f90:216-221 */
+    if (error < tol) goto done;                              /* Cgs.f90:223 */
+    rhoold = rho;                                            /* Cgs.f90:225 */
+  }
+done:
+  *fin_res = error;
+  *niter = iter;
+  free(d);
+  free(p1); free(p2); free(q1); free(q2); free(r2);
+  free(u1); free(u2); free(v2); free(u1q1);
+  return 0;
+}
+
+/* Sources/Process/Matrix_Mod/Topology.f90:33-47: nnz = sum of stencil widths. */
+int or_topology_nnz(int n_cells, int n_faces, const int *faces_c) {
+  long n = n_cells;
+  for (int s = 0; s < n_faces; ++s)
+    if (faces_c[2 * s + 1] > 0) n += 2;
+  return (int)n;
+}
+
+static int or_cmp_int(const void *a, const void *b) {
+  const int x = *(const int *)a, y = *(const int *)b;
+  return (x > y) - (x < y);
+}
+
+/* Sources/Process/Matrix_Mod/Topology.f90:26-121 (1-based outputs). */
+void or_topology(int n_cells, int n_faces, const int *faces_c, int *row,
+                 int *col, int *dia, int *mir, int *pos) {
+  int *stencw = (int *)malloc(sizeof(int) * (size_t)(n_cells + 1));
+  const int nnz = or_topology_nnz(n_cells, n_faces, faces_c);
+  for (int c = 1; c <= n_cells; ++c) stencw[c] = 1;
+  for (int s = 0; s < n_faces; ++s) {                        /* :34-41 */
+    const int c1 = faces_c[2 * s], c2 = faces_c[2 * s + 1];
+    if (c2 > 0) { stencw[c1]++; stencw[c2]++; }
+  }
+  for (int k = 0; k < nnz; ++k) { col[k] = 0; mir[k] = 0; }
+  for (int s = 0; s < 2 * n_faces; ++s) pos[s] = 0;
+  row[0] = 1;                                                /* :54-59 */
+  for (int c = 1; c <= n_cells; ++c) {
+    row[c] = row[c - 1] + stencw[c];
+    col[row[c - 1] - 1] = c;
+    stencw[c] = 1;
+  }
+  for (int s = 0; s < n_faces; ++s) {                        /* :62-71 */
+    const int c1 = faces_c[2 * s], c2 = faces_c[2 * s + 1];
+    if (c2 > 0) {
+      col[row[c1 - 1] + stencw[c1] - 1] = c2;
+      col[row[c2 - 1] + stencw[c2] - 1] = c1;
+      stencw[c1]++;
+      stencw[c2]++;
+    }
+  }
+  for (int c = 1; c <= n_cells; ++c) {                       /* :75-82 */
+    qsort(&col[row[c - 1] - 1], (size_t)stencw[c], sizeof(int), or_cmp_int);
+    for (int p = row[c - 1]; p <= row[c] - 1; ++p)
+      if (col[p - 1] == c) dia[c - 1] = p;
+  }
+  for (int c1 = 1; c1 <= n_cells; ++c1) {                    /* :85-103 */
+    for (int p1 = row[c1 - 1]; p1 <= row[c1] - 1; ++p1) {
+      const int c2 = col[p1 - 1];
+      for (int p2 = row[c2 - 1]; p2 <= row[c2] - 1; ++p2) {
+        if (col[p2 - 1] == c1) {
+          mir[p1 - 1] = p2;
+          mir[p2 - 1] = p1;
+          break;
+        }
+      }
+    }
+  }
+  for (int s = 0; s < n_faces; ++s) {                        /* :106-121 */
+    const int c1 = faces_c[2 * s], c2 = faces_c[2 * s + 1];
+    if (c2 > 0) {
+      for (int p = row[c1 - 1]; p <= row[c1] - 1; ++p)
+        if (col[p - 1] == c2) pos[2 * s] = p;
+      for (int p = row[c2 - 1]; p <= row[c2] - 1; ++p)
+        if (col[p - 1] == c1) pos[2 * s + 1] = p;
+    }
+  }
+  free(stencw);
+}

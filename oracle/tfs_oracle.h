@@ -1,0 +1,71 @@
+/*
+ * tfs_oracle.h -- CPU restatement of the T-FlowS linear-solver path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the oracle: the GPU product is checked
+ * against it.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; nothing in the product links or calls it.
+ *
+ * Indexing: all arrays are 0-based here.  row[n+1] holds row starts
+ * (row[n] == nnz), col[nnz] sorted ascending within each row, dia[n] the
+ * position of the diagonal entry, val[nnz] the FP64 coefficients.  This is the
+ * reference Matrix_Type (Sources/Process/Matrix_Mod.f90:36-49) shifted by one.
+ *
+ * Arithmetic follows the reference loop by loop (left-to-right sums in row
+ * order, no fused multiply-add), so on x86-64 the results are bitwise equal to
+ * the reference compiled with default floating-point semantics.
+ */
+#ifndef TFS_ORACLE_H
+#define TFS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int n;
+  const int *row;
+  const int *col;
+  const int *dia;
+  const double *val;
+} or_matrix;
+
+/* Sources/Process/Solvers_Mod/Prec_Form.f90:2-51 -> d[n] (the used diagonal of D). */
+void or_prec_form(const or_matrix *A, int prec, double *d);
+
+/* Sources/Process/Solvers_Mod/Prec_Solve.f90:2-78: x := M^-1 b. */
+void or_prec_solve(const or_matrix *A, const double *d, int prec, double *x,
+                   const double *b);
+
+/* Sources/Process/Solvers_Mod/Residual.f90:22-27: r := r - A x (in place). */
+void or_residual(const or_matrix *A, const double *x, double *r);
+
+/* CRS product y := A x, summed as in Sources/Process/Solvers_Mod/Cg.f90:128-134. */
+void or_spmv(const or_matrix *A, const double *x, double *y);
+
+/* Krylov solvers with the reference signature (Cg.f90:2-4, Bicg.f90:2-4,
+ * Cgs.f90:2-4).  x in/out, r1 in = b / out = final residual, niter in = cap /
+ * out = count.  ini_res is only written where the reference writes it.
+ * On the ||b|| < tol early exit the reference returns an uninitialised
+ * `error` as fin_res; this restatement returns ||b|| there.
+ * Returns 0. */
+int or_cg(const or_matrix *A, double *x, double *r1, int prec, int *niter,
+          double tol, double *ini_res, double *fin_res);
+int or_bicg(const or_matrix *A, double *x, double *r1, int prec, int *niter,
+            double tol, double *ini_res, double *fin_res);
+int or_cgs(const or_matrix *A, double *x, double *r1, int prec, int *niter,
+           double tol, double *ini_res, double *fin_res);
+
+/* Sources/Process/Matrix_Mod/Topology.f90:33-121.  faces_c is (2, n_faces)
+ * column-major with 1-based cell numbers (c2 <= 0 marks a boundary face), as
+ * the reference Grid_Type holds it.  Outputs are 1-based as in the reference:
+ * row[n+1], col[nnz], dia[n], mir[nnz], pos[2*n_faces] (0 where unset).
+ * Returns nnz (without the reference's one spare entry). */
+int or_topology_nnz(int n_cells, int n_faces, const int *faces_c);
+void or_topology(int n_cells, int n_faces, const int *faces_c, int *row,
+                 int *col, int *dia, int *mir, int *pos);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,161 @@
+"""Synthetic hex-mesh cases and the CRS topology builder (libtfs_mesh.so).
+
+``make_case`` returns the reference's matrix layout (row/col/dia/val, 0-based
+here; ``base=1`` gives the Fortran numbering) and the right-hand side.  The
+matrices follow the reference's assembly formulas:
+
+* Poisson (fractional-step pressure): off-diagonal ``-Scoef`` per interior face,
+  diagonal the sum (Sources/Process/Pressure_Matrix_Fractional.f90:26-48,
+  Sources/Process/Compute_Geometry.f90:22-49); Neumann walls; optionally cell 1
+  pinned by doubling ``a_11``.
+* Momentum: fully implicit diffusion with upwind convection
+  ``A12 = A0 - min(F,0)``, ``A21 = A0 + max(F,0)`` and an inertia diagonal
+  ``V/dt`` (Sources/Process/NewUVW.f90:349-384,500-506), with face fluxes from a
+  seeded divergence-free field.
+"""
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+
+POISSON = 0
+MOMENTUM = 1
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("beta", ctypes.c_double * 3),
+        ("jitter", ctypes.c_double),
+        ("permute", ctypes.c_int),
+        ("pin_first", ctypes.c_int),
+        ("rhs_zero_mean", ctypes.c_int),
+        ("nu", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+        ("n_modes", ctypes.c_int),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+_I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def _lib():
+    lib = _native.mesh_lib()
+    if not getattr(lib, "_tfs_typed", False):
+        lib.tfs_rng_u01.restype = ctypes.c_double
+        lib.tfs_rng_u01.argtypes = [ctypes.c_uint64] * 3
+        lib.tfs_topology_nnz.restype = ctypes.c_int64
+        lib.tfs_topology_nnz.argtypes = [ctypes.c_int, ctypes.c_int64, _I32P]
+        lib.tfs_topology_build.restype = ctypes.c_int
+        lib.tfs_topology_build.argtypes = [ctypes.c_int, ctypes.c_int64, _I32P, ctypes.c_int,
+                                           _I32P, _I32P, _I32P, ctypes.c_void_p]
+        lib.tfs_case_create.restype = ctypes.c_int
+        lib.tfs_case_create.argtypes = [ctypes.POINTER(_Params), ctypes.POINTER(ctypes.c_void_p)]
+        lib.tfs_case_destroy.argtypes = [ctypes.c_void_p]
+        lib.tfs_case_sizes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(ctypes.c_int64)]
+        lib.tfs_case_crs.restype = ctypes.c_int
+        lib.tfs_case_crs.argtypes = [ctypes.c_void_p, ctypes.c_int, _I32P, _I32P, _I32P,
+                                     _F64P, _F64P]
+        lib.tfs_case_faces.restype = ctypes.c_int
+        lib.tfs_case_faces.argtypes = [ctypes.c_void_p, _I32P]
+        lib.tfs_case_dt.restype = ctypes.c_double
+        lib.tfs_case_dt.argtypes = [ctypes.c_void_p]
+        lib.tfs_case_max_aspect.restype = ctypes.c_double
+        lib.tfs_case_max_aspect.argtypes = [ctypes.c_void_p]
+        lib._tfs_typed = True
+    return lib
+
+
+def rng_u01(seed, tag, i):
+    """i-th uniform draw in [0,1) of the SplitMix64 stream keyed by (seed, tag)."""
+    return _lib().tfs_rng_u01(seed, tag, i)
+
+
+@dataclass
+class Case:
+    """One linear system in the reference CRS layout (index base ``base``)."""
+    n: int
+    row: np.ndarray
+    col: np.ndarray
+    dia: np.ndarray
+    val: np.ndarray
+    b: np.ndarray
+    base: int = 0
+    dt: float = 0.0
+    max_aspect: float = 1.0
+    faces: np.ndarray = None
+    params: dict = field(default_factory=dict)
+
+    @property
+    def nnz(self):
+        return int(self.row[-1] - self.base)
+
+
+def make_case(kind=POISSON, nx=8, ny=None, nz=None, beta=(0.0, 0.0, 0.0), jitter=0.0,
+              permute=False, pin_first=True, rhs_zero_mean=True, nu=1e-3, cfl=1.0,
+              n_modes=8, seed=20261202, base=0, with_faces=False):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    if np.isscalar(beta):
+        beta = (float(beta),) * 3
+    p = _Params(kind=kind, nx=nx, ny=ny, nz=nz, beta=(ctypes.c_double * 3)(*beta),
+                jitter=jitter, permute=int(permute), pin_first=int(pin_first),
+                rhs_zero_mean=int(rhs_zero_mean), nu=nu, cfl=cfl, n_modes=n_modes,
+                seed=seed)
+    lib = _lib()
+    h = ctypes.c_void_p()
+    rc = lib.tfs_case_create(ctypes.byref(p), ctypes.byref(h))
+    if rc != 0:
+        raise ValueError(f"tfs_case_create failed ({rc})")
+    try:
+        n = ctypes.c_int()
+        nf = ctypes.c_int64()
+        nnz = ctypes.c_int64()
+        lib.tfs_case_sizes(h, ctypes.byref(n), ctypes.byref(nf), ctypes.byref(nnz))
+        n, nf, nnz = n.value, nf.value, nnz.value
+        row = np.empty(n + 1, np.int32)
+        col = np.empty(nnz, np.int32)
+        dia = np.empty(n, np.int32)
+        val = np.empty(nnz, np.float64)
+        b = np.empty(n, np.float64)
+        rc = lib.tfs_case_crs(h, base, row, col, dia, val, b)
+        if rc != 0:
+            raise ValueError(f"tfs_case_crs failed ({rc})")
+        faces = None
+        if with_faces:
+            faces = np.empty(2 * nf, np.int32)
+            lib.tfs_case_faces(h, faces)
+        params = dict(kind=kind, nx=nx, ny=ny, nz=nz, beta=tuple(beta), jitter=jitter,
+                      permute=bool(permute), pin_first=bool(pin_first),
+                      rhs_zero_mean=bool(rhs_zero_mean), nu=nu, cfl=cfl, n_modes=n_modes,
+                      seed=seed)
+        return Case(n=n, row=row, col=col, dia=dia, val=val, b=b, base=base,
+                    dt=lib.tfs_case_dt(h), max_aspect=lib.tfs_case_max_aspect(h),
+                    faces=faces, params=params)
+    finally:
+        lib.tfs_case_destroy(h)
+
+
+def topology(n_cells, faces_c, base=0, with_pos=False):
+    """CRS pattern from a face list (faces_c: flat (2, n_faces) column-major,
+    1-based cells, c2 <= 0 boundary).  Matrix_Mod_Topology counterpart."""
+    lib = _lib()
+    faces_c = np.ascontiguousarray(faces_c, dtype=np.int32)
+    nf = faces_c.size // 2
+    nnz = lib.tfs_topology_nnz(n_cells, nf, faces_c)
+    row = np.empty(n_cells + 1, np.int32)
+    col = np.empty(nnz, np.int32)
+    dia = np.empty(n_cells, np.int32)
+    pos = np.empty(2 * nf, np.int32) if with_pos else None
+    rc = lib.tfs_topology_build(n_cells, nf, faces_c, base, row, col, dia,
+                                pos.ctypes.data if with_pos else None)
+    if rc != 0:
+        raise ValueError(f"tfs_topology_build failed ({rc})")
+    return (row, col, dia, pos) if with_pos else (row, col, dia)
