@@ -1,0 +1,121 @@
+/*
+ * tfs_solver.h -- C ABI of libtfs_solver.so, the MI355X (gfx950) drop-in for
+ * T-FlowS's preconditioned Krylov solvers.
+ *
+ * Reference interface replaced (all paths relative to the T-FlowS tree):
+ *   subroutine Cg  (A, x, r1, prec, niter, tol, ini_res, fin_res)
+ *                                  Sources/Process/Solvers_Mod/Cg.f90:2-4,24-29
+ *   subroutine Bicg(same)          Sources/Process/Solvers_Mod/Bicg.f90:2-4
+ *   subroutine Cgs (same)          Sources/Process/Solvers_Mod/Cgs.f90:2-4
+ *   type(Matrix_Type) A            Sources/Process/Matrix_Mod.f90:36-49
+ *   Prec_Form / Prec_Solve         Sources/Process/Solvers_Mod/Prec_Form.f90:2-51,
+ *                                  Sources/Process/Solvers_Mod/Prec_Solve.f90:2-78
+ *   Residual                       Sources/Process/Solvers_Mod/Residual.f90:2-40
+ *
+ * Semantics kept from the reference:
+ *   - tol is ABSOLUTE on ||r||_2 (Cg.f90:174);
+ *   - niter: in = cap; out = k on convergence at iteration k, 0 on an early
+ *     exit, cap+1 without convergence (Cg.f90:55-58,87-90,97,181);
+ *   - x is updated in place from the caller's initial guess; r1 comes in as b
+ *     and goes out as the final recurrence residual (Cg.f90:63,161);
+ *   - ini_res is written only after the residual is formed (Cg.f90:85);
+ *     on the ||b|| < tol exit fin_res = ||b|| (the reference leaves an
+ *     uninitialised value there);
+ *   - prec 0 = none, 1 = diagonal, 2 = the reference's diagonal incomplete
+ *     Cholesky (Prec_Form.f90:32-40 with Prec_Solve.f90:46-67), applied
+ *     exactly (same factor, same row-internal order) by level-scheduled sweeps;
+ *   - Bicg equals Cg: the reference's shadow sequence (Bicg.f90:91-175) is
+ *     bitwise equal to the primary one, so it is computed once.
+ * Differences: dot products are reduced in a fixed tree instead of left to
+ * right, so iterates agree with the reference to rounding, not bitwise.
+ *
+ * All entry points are synchronous (they return when the device work is
+ * done) and are not reentrant for the same matrix handle.  Indices cross the
+ * ABI as int32 (the reference's default integer); `base` selects 0- or
+ * 1-based numbering.  Device pointers are HIP device addresses on the device
+ * the library was initialised on.
+ */
+#ifndef TFS_SOLVER_H
+#define TFS_SOLVER_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ---------------------------------------------------------------- */
+enum {
+  TFS_OK = 0,
+  TFS_NOT_CONVERGED = 1,    /* niter came back as cap+1 (the reference's only signal) */
+  TFS_ERR_INVALID = -1,     /* bad argument or inconsistent CRS structure            */
+  TFS_ERR_NOMEM = -2,       /* device allocation failed                              */
+  TFS_ERR_BREAKDOWN = -3,   /* rho, (p,q) or ||r|| became 0, Inf or NaN              */
+  TFS_ERR_DEVICE = -4,      /* HIP runtime error                                      */
+  TFS_ERR_NO_DEVICE = -5    /* no GPU visible                                         */
+};
+enum { TFS_CG = 0, TFS_BICG = 1, TFS_CGS = 2 };
+
+const char *tfs_status_string(int status);
+/* Library and device set-up; called implicitly by tfs_matrix_create. */
+int tfs_init(int device);
+
+/* ---- matrix: the Matrix_Type counterpart ------------------------------------
+ * Structure (row/col/dia) is uploaded once, as it is fixed after
+ * Matrix_Mod_Topology (Sources/Process/Matrix_Mod/Topology.f90:2-127); the
+ * level schedules of the triangular sweeps are built from it here.  Values
+ * change before every solve and are uploaded with tfs_matrix_set_values,
+ * which invalidates the cached preconditioner factor. */
+typedef struct tfs_matrix tfs_matrix;
+
+int tfs_matrix_create(int n, long long nnz, const int *row, const int *col,
+                      const int *dia, int base, tfs_matrix **out);
+int tfs_matrix_destroy(tfs_matrix *A);
+/* val[nnz] from host memory (the reference's A % val). */
+int tfs_matrix_set_values(tfs_matrix *A, const double *val);
+/* val[nnz] already on the device (copied device to device). */
+int tfs_matrix_set_values_device(tfs_matrix *A, const double *d_val);
+/* The caller rewrote the matrix's own device values in place
+ * (tfs_matrix_values_device); the cached factor is dropped. */
+int tfs_matrix_values_changed(tfs_matrix *A);
+/* Device address of the matrix's FP64 values (nnz entries). */
+double *tfs_matrix_values_device(tfs_matrix *A);
+
+typedef struct {
+  int n;
+  long long nnz;
+  int levels_lower;     /* depth of the forward-sweep DAG                    */
+  int levels_upper;     /* depth of the backward-sweep DAG                   */
+  int row_blocks;       /* SpMV row blocks (wavefront work units)            */
+  long long device_bytes;
+} tfs_matrix_info;
+int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info);
+
+/* ---- the solvers (Cg / Bicg / Cgs) -------------------------------------------
+ * Host-memory variant: x1 = c_loc(x(1)) of x(-NB:N), r1 = b(1:N). */
+int tfs_solve(tfs_matrix *A, int method, int prec, double *x1, double *r1,
+              int *niter, double tol, double *ini_res, double *fin_res);
+/* Device-memory variant: x and r1 are device arrays of n doubles. */
+int tfs_solve_device(tfs_matrix *A, int method, int prec, double *d_x, double *d_r1,
+                     int *niter, double tol, double *ini_res, double *fin_res);
+
+/* ---- building blocks on device vectors ---------------------------------------- */
+/* y := A x  (row sums in the reference order, Cg.f90:128-134). */
+int tfs_spmv_device(tfs_matrix *A, const double *d_x, double *d_y);
+/* r := r - A x  (Residual.f90:22-27, term by term in row order). */
+int tfs_residual_device(tfs_matrix *A, const double *d_x, double *d_r);
+/* Prec_Form: builds (or reuses) the factor for prec. */
+int tfs_prec_form(tfs_matrix *A, int prec);
+/* Prec_Solve: x := M^-1 b with the current factor (tfs_prec_form first). */
+int tfs_prec_solve_device(tfs_matrix *A, int prec, const double *d_b, double *d_x);
+/* Copies the n used diagonal entries of the preconditioner D to d_out. */
+int tfs_prec_diag_device(tfs_matrix *A, double *d_out);
+
+/* ---- measurement ----------------------------------------------------------------
+ * Runs `reps` back-to-back SpMV launches y := A x on the library's stream and
+ * returns the mean launch duration in ms from HIP events on that stream. */
+int tfs_spmv_time(tfs_matrix *A, const double *d_x, double *d_y, int reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,256 @@
+"""Python mirror of T-FlowS's Solvers_Mod on the GPU (libtfs_solver.so).
+
+Same names, argument meaning and error behaviour as the reference subroutines
+(Sources/Process/Solvers_Mod/{Cg,Bicg,Cgs}.f90):
+
+    niter, ini_res, fin_res = Cg(A, x, r1, prec, niter, tol, ini_res=0.0)
+
+* ``A``      -- ``Matrix`` (the Matrix_Type counterpart, structure resident on
+                the device, values uploaded with ``set_values``);
+* ``x``      -- initial guess in, solution out (in place);
+* ``r1``     -- right-hand side in, final recurrence residual out (in place);
+* ``prec``   -- 0 none, 1 diagonal, 2 the reference's incomplete Cholesky;
+* ``niter``  -- iteration cap in; the returned count is k on convergence at
+                iteration k, 0 on an early exit, cap+1 without convergence;
+* ``tol``    -- ABSOLUTE tolerance on ||r||_2 (Cg.f90:174).
+
+``x`` and ``r1`` may be numpy float64 arrays (host path, ``tfs_solve``) or
+torch float64 tensors on the GPU (device path, ``tfs_solve_device``).  As in
+the reference there is no exception for non-convergence; the last status is
+kept in ``A.status``.  Invalid arguments and device failures raise.
+There is no CPU fallback: without the HIP library this module raises.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+TFS_OK = 0
+TFS_NOT_CONVERGED = 1
+TFS_ERR_INVALID = -1
+TFS_ERR_NOMEM = -2
+TFS_ERR_BREAKDOWN = -3
+TFS_ERR_DEVICE = -4
+TFS_ERR_NO_DEVICE = -5
+
+CG, BICG, CGS = 0, 1, 2
+_METHODS = {"cg": CG, "bicg": BICG, "cgs": CGS}
+
+_I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_VP = ctypes.c_void_p
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+
+
+class SolverError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        super().__init__(f"{what}: {status_string(status)} ({status})")
+
+
+class _Info(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("nnz", ctypes.c_longlong),
+                ("levels_lower", ctypes.c_int), ("levels_upper", ctypes.c_int),
+                ("row_blocks", ctypes.c_int), ("device_bytes", ctypes.c_longlong)]
+
+
+def _lib():
+    # torch first: its HIP runtime then serves this library too (same soname),
+    # so device pointers from torch tensors are valid here.
+    import torch  # noqa: F401
+    lib = _native.solver_lib()
+    if getattr(lib, "_tfs_typed", False):
+        return lib
+    lib.tfs_status_string.restype = ctypes.c_char_p
+    lib.tfs_status_string.argtypes = [ctypes.c_int]
+    lib.tfs_init.argtypes = [ctypes.c_int]
+    lib.tfs_matrix_create.argtypes = [ctypes.c_int, ctypes.c_longlong, _I32P, _I32P, _I32P,
+                                      ctypes.c_int, ctypes.POINTER(_VP)]
+    lib.tfs_matrix_destroy.argtypes = [_VP]
+    lib.tfs_matrix_set_values.argtypes = [_VP, _F64P]
+    lib.tfs_matrix_set_values_device.argtypes = [_VP, _VP]
+    lib.tfs_matrix_values_changed.argtypes = [_VP]
+    lib.tfs_matrix_values_device.restype = _VP
+    lib.tfs_matrix_values_device.argtypes = [_VP]
+    lib.tfs_matrix_get_info.argtypes = [_VP, ctypes.POINTER(_Info)]
+    lib.tfs_solve.argtypes = [_VP, ctypes.c_int, ctypes.c_int, _F64P, _F64P, _IP,
+                              ctypes.c_double, _DP, _DP]
+    lib.tfs_solve_device.argtypes = [_VP, ctypes.c_int, ctypes.c_int, _VP, _VP, _IP,
+                                     ctypes.c_double, _DP, _DP]
+    lib.tfs_spmv_device.argtypes = [_VP, _VP, _VP]
+    lib.tfs_residual_device.argtypes = [_VP, _VP, _VP]
+    lib.tfs_prec_form.argtypes = [_VP, ctypes.c_int]
+    lib.tfs_prec_solve_device.argtypes = [_VP, ctypes.c_int, _VP, _VP]
+    lib.tfs_prec_diag_device.argtypes = [_VP, _VP]
+    lib.tfs_spmv_time.argtypes = [_VP, _VP, _VP, ctypes.c_int, _DP]
+    lib._tfs_typed = True
+    return lib
+
+
+def status_string(status):
+    return _lib().tfs_status_string(status).decode()
+
+
+def _check(rc, what):
+    if rc < 0 and rc != TFS_ERR_BREAKDOWN:
+        raise SolverError(rc, what)
+    return rc
+
+
+def _is_device_tensor(a):
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        return False
+    return isinstance(a, torch.Tensor) and a.is_cuda
+
+
+def _dptr(t, n, what):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64
+            and t.is_contiguous() and t.numel() >= n):
+        raise ValueError(f"{what}: expected a contiguous float64 GPU tensor of >= {n} entries")
+    return t.data_ptr()
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+class Matrix:
+    """Matrix_Type counterpart (Sources/Process/Matrix_Mod.f90:36-49) on the GPU."""
+
+    def __init__(self, row, col, dia, val=None, base=0):
+        lib = _lib()
+        row = np.ascontiguousarray(row, dtype=np.int32)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        dia = np.ascontiguousarray(dia, dtype=np.int32)
+        n = len(row) - 1
+        nnz = int(row[-1]) - base if n >= 0 else 0
+        h = _VP()
+        _check(lib.tfs_matrix_create(n, nnz, row, col, dia, base, ctypes.byref(h)),
+               "tfs_matrix_create")
+        self._h = h
+        self._lib = lib
+        self.n = n
+        self.nnz = nnz
+        self.status = TFS_OK
+        if val is not None:
+            self.set_values(val)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.tfs_matrix_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        i = _Info()
+        _check(self._lib.tfs_matrix_get_info(self._h, ctypes.byref(i)), "tfs_matrix_get_info")
+        return dict(n=i.n, nnz=i.nnz, levels_lower=i.levels_lower,
+                    levels_upper=i.levels_upper, row_blocks=i.row_blocks,
+                    device_bytes=i.device_bytes)
+
+    def set_values(self, val):
+        if _is_device_tensor(val):
+            _sync()
+            rc = self._lib.tfs_matrix_set_values_device(self._h, _dptr(val, self.nnz, "val"))
+        else:
+            val = np.ascontiguousarray(val, dtype=np.float64)
+            if val.size < self.nnz:
+                raise ValueError("val shorter than nnz")
+            rc = self._lib.tfs_matrix_set_values(self._h, val)
+        _check(rc, "tfs_matrix_set_values")
+
+    def values_changed(self):
+        _check(self._lib.tfs_matrix_values_changed(self._h), "tfs_matrix_values_changed")
+
+    def values_device_ptr(self):
+        return self._lib.tfs_matrix_values_device(self._h)
+
+    # ---- building blocks (torch GPU tensors) ----
+    def spmv(self, x, y):
+        _sync()
+        _check(self._lib.tfs_spmv_device(self._h, _dptr(x, self.n, "x"), _dptr(y, self.n, "y")),
+               "tfs_spmv_device")
+        return y
+
+    def residual(self, x, r):
+        _sync()
+        _check(self._lib.tfs_residual_device(self._h, _dptr(x, self.n, "x"),
+                                             _dptr(r, self.n, "r")), "tfs_residual_device")
+        return r
+
+    def prec_form(self, prec):
+        _check(self._lib.tfs_prec_form(self._h, prec), "tfs_prec_form")
+
+    def prec_solve(self, prec, b, x):
+        _sync()
+        _check(self._lib.tfs_prec_solve_device(self._h, prec, _dptr(b, self.n, "b"),
+                                               _dptr(x, self.n, "x")), "tfs_prec_solve_device")
+        return x
+
+    def prec_diag(self, out):
+        _sync()
+        _check(self._lib.tfs_prec_diag_device(self._h, _dptr(out, self.n, "out")),
+               "tfs_prec_diag_device")
+        return out
+
+    def spmv_time(self, x, y, reps=20):
+        _sync()
+        ms = ctypes.c_double()
+        _check(self._lib.tfs_spmv_time(self._h, _dptr(x, self.n, "x"), _dptr(y, self.n, "y"),
+                                       reps, ctypes.byref(ms)), "tfs_spmv_time")
+        return ms.value
+
+
+def _solve(method, A, x, r1, prec, niter, tol, ini_res=0.0):
+    lib = A._lib
+    it = ctypes.c_int(int(niter))
+    ir = ctypes.c_double(float(ini_res))
+    fr = ctypes.c_double(0.0)
+    if _is_device_tensor(x) or _is_device_tensor(r1):
+        _sync()
+        rc = lib.tfs_solve_device(A.handle, method, prec, _dptr(x, A.n, "x"),
+                                  _dptr(r1, A.n, "r1"), ctypes.byref(it), float(tol),
+                                  ctypes.byref(ir), ctypes.byref(fr))
+    else:
+        for a, nm in ((x, "x"), (r1, "r1")):
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64
+                    and a.flags.c_contiguous and a.size >= A.n):
+                raise ValueError(f"{nm}: expected a contiguous float64 numpy array")
+        rc = lib.tfs_solve(A.handle, method, prec, x, r1, ctypes.byref(it), float(tol),
+                           ctypes.byref(ir), ctypes.byref(fr))
+    A.status = _check(rc, ("Cg", "Bicg", "Cgs")[method])
+    return it.value, ir.value, fr.value
+
+
+def Cg(A, x, r1, prec, niter, tol, ini_res=0.0):
+    """Preconditioned CG, Sources/Process/Solvers_Mod/Cg.f90:2-183."""
+    return _solve(CG, A, x, r1, prec, niter, tol, ini_res)
+
+
+def Bicg(A, x, r1, prec, niter, tol, ini_res=0.0):
+    """'BiCG' of Sources/Process/Solvers_Mod/Bicg.f90:2-196 (equal to Cg bit for bit)."""
+    return _solve(BICG, A, x, r1, prec, niter, tol, ini_res)
+
+
+def Cgs(A, x, r1, prec, niter, tol, ini_res=0.0):
+    """Preconditioned CGS, Sources/Process/Solvers_Mod/Cgs.f90:2-232."""
+    return _solve(CGS, A, x, r1, prec, niter, tol, ini_res)
+
+
+def solve(method, A, x, r1, prec, niter, tol, ini_res=0.0):
+    return _solve(_METHODS.get(method, method), A, x, r1, prec, niter, tol, ini_res)

@@ -1,0 +1,230 @@
+"""GPU solver (libtfs_solver.so on gfx950) against the oracle.
+
+Bar (floating point; reductions are tree-ordered on the GPU):
+  * SpMV, residual, factor and preconditioner sweeps: bitwise equal to the
+    oracle (same operations in the same row-internal order);
+  * solves: iterations within +-5% (at least +-1), final residual below tol,
+    solution relative L2 difference <= 1e-8 at tol 1e-10 * ||b||;
+  * Bicg equals Cg, host and device paths agree, reruns are bitwise equal.
+"""
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from tflows_amd import cases, mesh
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+X_TOL = 1e-8       # solution rel-L2 at tol 1e-10 relative
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    if not t.cuda.is_available():
+        pytest.fail("no GPU visible")
+    return t
+
+
+@pytest.fixture(scope="module")
+def S():
+    from tflows_amd import solvers
+    return solvers
+
+
+CASES = {
+    "poisson12": dict(kind=mesh.POISSON, nx=12, beta=1.5),
+    "poisson_perm": dict(kind=mesh.POISSON, nx=10, ny=8, nz=6, beta=(0.0, 2.0, 3.0),
+                         permute=True),
+    "momentum10": dict(kind=mesh.MOMENTUM, nx=10, beta=2.0, jitter=0.1, permute=True,
+                       pin_first=False, rhs_zero_mean=False),
+    "config1": cases.config_kwargs(1),
+    "config2_small": cases.config_kwargs(2, scale=0.25),
+}
+
+_case_cache = {}
+
+
+def get_case(name):
+    if name not in _case_cache:
+        _case_cache[name] = mesh.make_case(**CASES[name])
+    return _case_cache[name]
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def matrix(S, c):
+    return S.Matrix(c.row, c.col, c.dia, c.val)
+
+
+def iters_close(a, b):
+    return abs(a - b) <= max(1, int(0.05 * b))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_spmv_and_residual_bitwise(torch, S, oracle, name):
+    c = get_case(name)
+    print(f"case {name} n={c.n}", flush=True)
+    A = matrix(S, c)
+    x = np.sin(np.arange(c.n) * 0.37) + 0.1
+    y = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    A.spmv(dev(torch, x), y)
+    assert np.array_equal(host(y), oracle.spmv(c.row, c.col, c.dia, c.val, x))
+    r = dev(torch, c.b)
+    A.residual(dev(torch, x), r)
+    assert np.array_equal(host(r), oracle.residual(c.row, c.col, c.dia, c.val, x, c.b))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_factor_and_sweeps_bitwise(torch, S, oracle, name, prec):
+    c = get_case(name)
+    A = matrix(S, c)
+    A.prec_form(prec)
+    d = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    A.prec_diag(d)
+    d_or = oracle.prec_form(c.row, c.col, c.dia, c.val, prec)
+    assert np.array_equal(host(d), d_or)
+    z = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    A.prec_solve(prec, dev(torch, c.b), z)
+    assert np.array_equal(host(z), oracle.prec_solve(c.row, c.col, c.dia, c.val, d_or, prec, c.b))
+
+
+@pytest.mark.parametrize("name", ["poisson12", "poisson_perm", "momentum10", "config2_small"])
+@pytest.mark.parametrize("method", ["cg", "bicg", "cgs"])
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_solve_parity(torch, S, oracle, name, method, prec):
+    c = get_case(name)
+    print(f"case {name} {method} prec={prec}", flush=True)
+    tol = 1e-10 * float(np.linalg.norm(c.b))
+    cap = 3000
+    ref = oracle.solve(method, c.row, c.col, c.dia, c.val, c.b, prec=prec, niter=cap, tol=tol)
+    A = matrix(S, c)
+    x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    r = dev(torch, c.b)
+    niter, ini, fin = S.solve(method, A, x, r, prec, cap, tol)
+    assert iters_close(niter, ref["niter"]), (niter, ref["niter"])
+    assert abs(ini - ref["ini_res"]) <= 1e-12 * ref["ini_res"]
+    if ref["niter"] <= cap:  # the reference converged
+        assert fin < tol and A.status == S.TFS_OK
+        assert rel_l2(host(x), ref["x"]) <= X_TOL
+    else:  # the reference stagnated: match the stagnation
+        assert niter == cap + 1 and A.status == S.TFS_NOT_CONVERGED
+
+
+def test_config1_against_reference_golden(torch, S):
+    """32^3 Poisson, Cg prec 2 to 1e-8: the reference's own outputs."""
+    z = np.load(GOLDEN / "config1_cg_p2.npz", allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    c = get_case("config1")
+    A = matrix(S, c)
+    x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    r = dev(torch, c.b)
+    niter, ini, fin = S.Cg(A, x, r, 2, meta["cap"], meta["tol"])
+    assert iters_close(niter, meta["niter"]), (niter, meta["niter"])
+    assert fin < meta["tol"]
+    assert abs(ini - meta["ini_res"]) <= 1e-12 * meta["ini_res"]
+    # at tol 1e-8 the two iterates differ by rounding amplified by the
+    # conditioning; both are within the solve tolerance of the true solution
+    assert rel_l2(host(x), z["x"]) <= 1e-6
+
+
+def test_bicg_equals_cg_and_reruns_are_bitwise(torch, S):
+    c = get_case("momentum10")
+    tol = 1e-10 * float(np.linalg.norm(c.b))
+    A = matrix(S, c)
+    out = []
+    for method in ("cg", "bicg", "cg"):
+        x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+        r = dev(torch, c.b)
+        res = S.solve(method, A, x, r, 2, 500, tol)
+        out.append((res, host(x), host(r)))
+    for o in out[1:]:
+        assert o[0] == out[0][0]
+        assert np.array_equal(o[1], out[0][1]) and np.array_equal(o[2], out[0][2])
+
+
+def test_host_and_device_paths_agree(torch, S):
+    c = get_case("poisson_perm")
+    tol = 1e-10 * float(np.linalg.norm(c.b))
+    A = matrix(S, c)
+    xh = np.zeros(c.n)
+    rh = c.b.copy()
+    a = S.Cgs(A, xh, rh, 2, 500, tol)
+    xd = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    rd = dev(torch, c.b)
+    b = S.Cgs(A, xd, rd, 2, 500, tol)
+    assert a == b
+    assert np.array_equal(xh, host(xd)) and np.array_equal(rh, host(rd))
+
+
+@pytest.mark.parametrize("method", ["cg", "bicg", "cgs"])
+def test_early_exits_and_cap(torch, S, method):
+    c = get_case("poisson12")
+    A = matrix(S, c)
+    # ||b|| < tol: niter 0, x and ini_res untouched
+    x = torch.full((c.n,), 3.0, dtype=torch.float64, device="cuda")
+    r = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    niter, ini, fin = S.solve(method, A, x, r, 2, 10, 1e-8, ini_res=-7.0)
+    assert niter == 0 and ini == -7.0 and torch.all(x == 3.0)
+    # zero-trip loop and a short cap
+    x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    niter, ini, fin = S.solve(method, A, x, dev(torch, c.b), 2, 0, 1e-14)
+    assert niter == 1 and fin == ini and A.status == S.TFS_NOT_CONVERGED
+    x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    niter, ini, fin = S.solve(method, A, x, dev(torch, c.b), 2, 3, 1e-14)
+    assert niter == 4 and A.status == S.TFS_NOT_CONVERGED
+
+
+def test_breakdown_is_reported(torch, S):
+    """A zero diagonal makes the Jacobi preconditioner divide by zero: the
+    reference runs on with Inf/NaN to cap+1; the drop-in stops and says so."""
+    c = mesh.make_case(kind=mesh.POISSON, nx=4)
+    val = c.val.copy()
+    val[c.dia[5]] = 0.0
+    A = S.Matrix(c.row, c.col, c.dia, val)
+    x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    niter, ini, fin = S.Cg(A, x, dev(torch, c.b), 1, 50, 1e-12)
+    assert niter == 51 and not np.isfinite(fin) and A.status == S.TFS_ERR_BREAKDOWN
+
+
+def test_values_update_invalidates_factor(torch, S, oracle):
+    c = get_case("momentum10")
+    A = matrix(S, c)
+    A.prec_form(2)
+    val2 = c.val * 1.5
+    A.set_values(dev(torch, val2))
+    A.prec_form(2)
+    d = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    A.prec_diag(d)
+    assert np.array_equal(host(d), oracle.prec_form(c.row, c.col, c.dia, val2, 2))
+
+
+def test_config2_full_size(torch, S, oracle):
+    """128^3 momentum (permuted, jittered, stretched), Bicg prec 2 to 1e-10:
+    GPU against the oracle at full size, plus the true residual b - A x."""
+    c = mesh.make_case(**cases.config_kwargs(2))
+    print(f"config2 n={c.n} nnz={c.nnz}", flush=True)
+    tol = 1e-10 * float(np.linalg.norm(c.b))
+    ref = oracle.solve("bicg", c.row, c.col, c.dia, c.val, c.b, prec=2, niter=10000, tol=tol)
+    A = matrix(S, c)
+    info = A.info()
+    print(f"levels lower={info['levels_lower']} upper={info['levels_upper']}", flush=True)
+    x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    r = dev(torch, c.b)
+    niter, ini, fin = S.Bicg(A, x, r, 2, 10000, tol)
+    assert iters_close(niter, ref["niter"]) and fin < tol
+    assert rel_l2(host(x), ref["x"]) <= X_TOL
+    true_r = dev(torch, c.b)
+    A.residual(x, true_r)
+    assert float(torch.linalg.norm(true_r)) < 10 * tol

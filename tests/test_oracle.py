@@ -1,0 +1,339 @@
+"""Pins the oracle (oracle/tfs_oracle.c) before anything is checked against it.
+
+Each oracle function is checked by something other than its own formula:
+  * the reference's own solver path compiled from its sources (bitwise);
+  * golden fixtures written by that compiled reference (bitwise);
+  * closed forms, invariants and dense brute force on tiny systems, chosen so
+    that a plausible slip (a dropped square, a wrong index, a transposed
+    operand, a sign) fails at least one of them -- the planted-mistake tests
+    at the end show that for the factor.
+"""
+import hashlib
+import json
+import pathlib
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import rel_l2
+from tflows_amd import cases, mesh
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+
+
+def dense(c):
+    A = np.zeros((c.n, c.n))
+    for i in range(c.n):
+        A[i, c.col[c.row[i]:c.row[i + 1]]] = c.val[c.row[i]:c.row[i + 1]]
+    return A
+
+
+def crs_from_dense(A):
+    n = A.shape[0]
+    row, col, val, dia = [0], [], [], []
+    for i in range(n):
+        for j in range(n):
+            if A[i, j] != 0 or i == j:
+                if i == j:
+                    dia.append(len(col))
+                col.append(j)
+                val.append(A[i, j])
+        row.append(len(col))
+    return (np.array(row, np.int32), np.array(col, np.int32), np.array(dia, np.int32),
+            np.array(val, np.float64))
+
+
+SMALL = [
+    dict(kind=mesh.POISSON, nx=6, beta=1.0),
+    dict(kind=mesh.POISSON, nx=5, ny=4, nz=3, beta=(0.0, 2.0, 1.0), permute=True),
+    dict(kind=mesh.MOMENTUM, nx=5, beta=2.0, jitter=0.1, permute=True, pin_first=False,
+         rhs_zero_mean=False),
+]
+
+
+# ---------------------------------------------------------------- known answers
+def test_matrix_mod_example_layout(oracle):
+    """The 4x4 CRS example of Sources/Process/Matrix_Mod.f90:15-34.
+
+    The comment lists dia = [1 5 9 12], but with its own col array the
+    diagonal of row 3 (a_33 = 99, col 3) is at position 8 (col(9) = 4), so the
+    layout's definition -- col(dia(i)) == i -- gives [1 5 8 12]."""
+    Ad = np.array([[10, 0, 4, 5], [2, 12, -1, 0], [0, 1, 99, 7], [-3, 11, 0, 53]], float)
+    val = np.array([10, 4, 5, 2, 12, -1, 1, 99, 7, -3, 11, 53], float)
+    col = np.array([1, 3, 4, 1, 2, 3, 2, 3, 4, 1, 2, 4]) - 1
+    row = np.array([1, 4, 7, 10, 13]) - 1          # row(5) = nnz+1 implied
+    dia = np.array([1, 5, 8, 12]) - 1
+    assert all(col[dia[i]] == i for i in range(4))
+    assert col[9 - 1] != 3 - 1  # the documented dia(3) = 9 points at column 4
+    r2, c2, d2, v2 = crs_from_dense(Ad)
+    assert np.array_equal(r2, row) and np.array_equal(c2, col) and np.array_equal(d2, dia)
+    assert np.array_equal(v2, val)
+    x = np.array([1.0, -2.0, 3.0, 0.5])
+    assert np.array_equal(oracle.spmv(row, col, dia, val, x), Ad @ x)
+    assert np.array_equal(oracle.residual(row, col, dia, val, x, Ad @ x), np.zeros(4))
+
+
+def test_prec_form_closed_form_1d_laplacian(oracle):
+    """tridiag(-1,2,-1): d_1 = 1/2, d_i = 1/(2 - d_{i-1}), i.e. d_i = i/(i+1)."""
+    n = 60
+    A = 2 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+    row, col, dia, val = crs_from_dense(A)
+    d = oracle.prec_form(row, col, dia, val, 2)
+    i = np.arange(1, n + 1)
+    np.testing.assert_allclose(d, i / (i + 1.0), rtol=1e-14)
+    np.testing.assert_array_equal(oracle.prec_form(row, col, dia, val, 1), np.full(n, 2.0))
+    np.testing.assert_array_equal(oracle.prec_form(row, col, dia, val, 0), np.ones(n))
+
+
+def precond_matrix(c, d):
+    """M = (Dt + L) Dt^-1 (Dt + U), Dt = diag(1/d): what Prec_Solve applies."""
+    A = dense(c)
+    L, U = np.tril(A, -1), np.triu(A, 1)
+    Dt = np.diag(1.0 / d)
+    return (Dt + L) @ np.diag(d) @ (Dt + U)
+
+
+@pytest.mark.parametrize("kw", SMALL[:2])
+def test_dic_agrees_with_A_on_its_pattern(oracle, kw):
+    """Defining property of the diagonal IC factor on symmetric A: M equals A
+    on the diagonal and on A's off-diagonal pattern."""
+    c = mesh.make_case(**kw)
+    A = dense(c)
+    d = oracle.prec_form(c.row, c.col, c.dia, c.val, 2)
+    M = precond_matrix(c, d)
+    pattern = A != 0
+    np.testing.assert_allclose(M[pattern], A[pattern], rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("kw", SMALL)
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_prec_solve_inverts_M(oracle, kw, prec):
+    c = mesh.make_case(**kw)
+    A = dense(c)
+    d = oracle.prec_form(c.row, c.col, c.dia, c.val, prec)
+    z = oracle.prec_solve(c.row, c.col, c.dia, c.val, d, prec, c.b)
+    if prec == 0:
+        M = np.eye(c.n)
+    elif prec == 1:
+        M = np.diag(np.diag(A))
+    else:
+        M = precond_matrix(c, d)
+    assert np.linalg.norm(M @ z - c.b) <= 1e-12 * np.linalg.norm(c.b)
+
+
+@pytest.mark.parametrize("kw", SMALL)
+def test_residual_and_spmv_bruteforce(oracle, kw):
+    c = mesh.make_case(**kw)
+    A = dense(c)
+    x = np.sin(np.arange(c.n))
+    np.testing.assert_allclose(oracle.spmv(c.row, c.col, c.dia, c.val, x), A @ x,
+                               rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(oracle.residual(c.row, c.col, c.dia, c.val, x, c.b), c.b - A @ x,
+                               rtol=1e-13, atol=1e-13)
+
+
+def test_laplacian_annihilates_linear_field(oracle):
+    """Uniform FV Laplacian (no pin): A . (linear field) vanishes on interior cells."""
+    n = 7
+    c = mesh.make_case(kind=mesh.POISSON, nx=n, pin_first=False)
+    h = 1.0 / n
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    phi = (1.0 * (i + 0.5) * h + 2.0 * (j + 0.5) * h - 3.0 * (k + 0.5) * h).ravel()
+    y = oracle.spmv(c.row, c.col, c.dia, c.val, phi).reshape(n, n, n)
+    assert np.abs(y[1:-1, 1:-1, 1:-1]).max() < 1e-13
+    assert np.abs(y[0]).max() > 1e-3  # the Neumann wall rows do not vanish
+
+
+# ---------------------------------------------------------------- solvers
+def test_cg_exact_termination(oracle):
+    rng = np.random.default_rng(1)
+    n = 8
+    B = rng.standard_normal((n, n))
+    A = B @ B.T + n * np.eye(n)
+    row, col, dia, val = crs_from_dense(A)
+    b = rng.standard_normal(n)
+    for prec in (0, 1, 2):
+        r = oracle.solve("cg", row, col, dia, val, b, prec=prec, niter=50,
+                         tol=1e-12 * np.linalg.norm(b))
+        assert r["niter"] <= n + 2
+        np.testing.assert_allclose(r["x"], np.linalg.solve(A, b), rtol=1e-9)
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_cg_matches_scipy_pcg(oracle, prec):
+    """scipy's PCG with the same M needs the same iterations (+-1)."""
+    c = mesh.make_case(kind=mesh.POISSON, nx=8, beta=1.5)
+    A = sp.csr_matrix((c.val, c.col, c.row), shape=(c.n, c.n))
+    d = oracle.prec_form(c.row, c.col, c.dia, c.val, prec)
+    Minv = spla.LinearOperator((c.n, c.n), matvec=lambda v: oracle.prec_solve(
+        c.row, c.col, c.dia, c.val, d, prec, np.ascontiguousarray(v, np.float64)))
+    tol = 1e-10 * np.linalg.norm(c.b)
+    it = [0]
+
+    def cb(xk):
+        it[0] += 1
+    x_sp, info = spla.cg(A, c.b, rtol=0.0, atol=tol, maxiter=5000, M=Minv, callback=cb)
+    assert info == 0
+    r = oracle.solve("cg", c.row, c.col, c.dia, c.val, c.b, prec=prec, niter=5000, tol=tol)
+    assert abs(r["niter"] - it[0]) <= 1
+    assert rel_l2(r["x"], x_sp) < 1e-8
+
+
+@pytest.mark.parametrize("kw", SMALL)
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_bicg_equals_cg_bitwise(oracle, kw, prec):
+    c = mesh.make_case(**kw)
+    tol = 1e-10 * np.linalg.norm(c.b)
+    a = oracle.solve("cg", c.row, c.col, c.dia, c.val, c.b, prec=prec, niter=500, tol=tol)
+    b = oracle.solve("bicg", c.row, c.col, c.dia, c.val, c.b, prec=prec, niter=500, tol=tol)
+    for k in ("x", "r"):
+        assert np.array_equal(a[k], b[k])
+    assert (a["niter"], a["fin_res"], a["ini_res"]) == (b["niter"], b["fin_res"], b["ini_res"])
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_cgs_dense_solve_nonsymmetric(oracle, prec):
+    c = mesh.make_case(kind=mesh.MOMENTUM, nx=4, beta=2.0, jitter=0.1, permute=True,
+                       pin_first=False, rhs_zero_mean=False)
+    A = dense(c)
+    assert np.abs(A - A.T).max() > 1e-6  # genuinely nonsymmetric
+    r = oracle.solve("cgs", c.row, c.col, c.dia, c.val, c.b, prec=prec, niter=500,
+                     tol=1e-12 * np.linalg.norm(c.b))
+    assert r["niter"] <= 200
+    assert rel_l2(r["x"], np.linalg.solve(A, c.b)) < 1e-9
+
+
+@pytest.mark.parametrize("method", ["cg", "bicg", "cgs"])
+def test_early_exits_and_cap(oracle, method):
+    c = mesh.make_case(kind=mesh.POISSON, nx=5)
+    args = (c.row, c.col, c.dia, c.val)
+    # ||b|| < tol: niter 0, x untouched, ini_res untouched (Cg.f90:55-58)
+    x0 = np.full(c.n, 3.0)
+    r = oracle.solve(method, *args, np.zeros(c.n), prec=2, niter=10, tol=1e-8, x0=x0,
+                     ini_res=-7.0)
+    assert r["niter"] == 0 and np.array_equal(r["x"], x0) and r["ini_res"] == -7.0
+    # x0 already the solution: ini_res < tol, niter 0 (Cg.f90:87-90)
+    xs = np.linalg.solve(dense(c), c.b)
+    bb = dense(c) @ xs
+    r = oracle.solve(method, *args, bb, prec=2, niter=10, tol=1e-6, x0=xs)
+    assert r["niter"] == 0 and r["ini_res"] < 1e-6
+    # zero-trip loop: niter comes back as 1; a short cap: cap + 1
+    r = oracle.solve(method, *args, c.b, prec=2, niter=0, tol=1e-14)
+    assert r["niter"] == 1 and r["fin_res"] == r["ini_res"]
+    r = oracle.solve(method, *args, c.b, prec=2, niter=3, tol=1e-14)
+    assert r["niter"] == 4
+
+
+# ---------------------------------------------------------------- the reference
+@pytest.mark.parametrize("kw", SMALL)
+def test_oracle_bitwise_vs_compiled_reference(oracle, reference, kw):
+    c = mesh.make_case(**kw)
+    reference.set_matrix(c.row, c.col, c.dia, c.val)
+    tol = 1e-10 * np.linalg.norm(c.b)
+    for prec in (0, 1, 2):
+        z_ref, d_ref = reference.prec(prec, c.b)
+        d = oracle.prec_form(c.row, c.col, c.dia, c.val, prec)
+        assert np.array_equal(d, d_ref)
+        assert np.array_equal(oracle.prec_solve(c.row, c.col, c.dia, c.val, d, prec, c.b), z_ref)
+        for meth in ("cg", "bicg", "cgs"):
+            a = oracle.solve(meth, c.row, c.col, c.dia, c.val, c.b, prec=prec, niter=700,
+                             tol=tol)
+            r = reference.solve(meth, c.b, prec=prec, niter=700, tol=tol)
+            assert np.array_equal(a["x"], r["x"]) and np.array_equal(a["r"], r["r"])
+            assert (a["niter"], a["ini_res"], a["fin_res"]) == (r["niter"], r["ini_res"],
+                                                                 r["fin_res"])
+    x = np.cos(np.arange(c.n))
+    assert np.array_equal(oracle.residual(c.row, c.col, c.dia, c.val, x, c.b),
+                          reference.residual(x, c.b))
+
+
+def test_topology_bitwise_vs_compiled_reference(oracle, reference):
+    c = mesh.make_case(kind=mesh.MOMENTUM, nx=7, ny=5, nz=6, beta=1.0, jitter=0.1,
+                       permute=True, with_faces=True)
+    t_ref = reference.topology(c.n, c.faces)
+    t_or = oracle.topology(c.n, c.faces)
+    for k in ("row", "col", "dia", "mir", "pos"):
+        assert np.array_equal(t_ref[k], t_or[k]), k
+
+
+# ---------------------------------------------------------------- golden fixtures
+def _golden():
+    return sorted(GOLDEN.glob("*.npz"))
+
+
+def _load(p):
+    z = np.load(p, allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    return meta, {k: z[k] for k in z.files if k != "meta"}
+
+
+def _digest(c):
+    h = hashlib.sha256()
+    for a in (c.row, c.col, c.dia, c.val, c.b):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+@pytest.mark.parametrize("path", _golden(), ids=lambda p: p.stem)
+def test_oracle_bitwise_vs_golden(oracle, path):
+    meta, arr = _load(path)
+    kw = dict(meta["case"])
+    if "beta" in kw and isinstance(kw["beta"], list):
+        kw["beta"] = tuple(kw["beta"])
+    if path.stem.startswith("topology"):
+        c = mesh.make_case(with_faces=True, **kw)
+        t = oracle.topology(c.n, c.faces)
+        for k in ("row", "col", "dia", "mir", "pos"):
+            assert np.array_equal(t[k], arr[k]), k
+        return
+    c = mesh.make_case(**kw)
+    assert _digest(c) == meta["sha256"], "case generator changed since the fixture was made"
+    if path.stem.endswith("_residual"):
+        assert np.array_equal(oracle.residual(c.row, c.col, c.dia, c.val, c.b, c.b), arr["r"])
+        return
+    if "_prec" in path.stem and "method" not in meta:
+        d = oracle.prec_form(c.row, c.col, c.dia, c.val, meta["prec"])
+        assert np.array_equal(d, arr["d"])
+        assert np.array_equal(oracle.prec_solve(c.row, c.col, c.dia, c.val, d, meta["prec"], c.b),
+                              arr["z"])
+        return
+    r = oracle.solve(meta["method"], c.row, c.col, c.dia, c.val, c.b, prec=meta["prec"],
+                     niter=meta["cap"], tol=meta["tol"])
+    assert r["niter"] == meta["niter"]
+    assert r["ini_res"] == meta["ini_res"] and r["fin_res"] == meta["fin_res"]
+    assert np.array_equal(r["x"], arr["x"]) and np.array_equal(r["r"], arr["r"])
+
+
+def test_config1_golden_meets_tolerance():
+    meta, arr = _load(GOLDEN / "config1_cg_p2.npz")
+    assert meta["niter"] < meta["cap"] and meta["fin_res"] < meta["tol"]
+    assert meta["case"] == cases.config_kwargs(1) or meta["case"]["nx"] == 32
+
+
+# ---------------------------------------------------------------- planted mistakes
+def _prec_form_py(c, variant):
+    """Python copy of the factor recurrence with a switchable slip."""
+    d = np.zeros(c.n)
+    for i in range(c.n):
+        s = c.val[c.dia[i]]
+        for j in range(c.row[i], c.dia[i]):
+            k = c.col[j]
+            if variant == "ok":
+                s -= d[k] * c.val[j] * c.val[j]
+            elif variant == "no_square":
+                s -= d[k] * c.val[j]
+            elif variant == "own_d":
+                s -= d[i] * c.val[j] * c.val[j]
+        d[i] = 1.0 / s
+    return d
+
+
+@pytest.mark.parametrize("variant,caught", [("ok", False), ("no_square", True), ("own_d", True)])
+def test_planted_factor_mistakes_are_caught(variant, caught):
+    c = mesh.make_case(kind=mesh.POISSON, nx=5, beta=1.0)
+    A = dense(c)
+    M = precond_matrix(c, _prec_form_py(c, variant))
+    ok = np.allclose(np.diag(M), np.diag(A), rtol=1e-12)
+    assert ok != caught
