@@ -114,6 +114,19 @@ int tfs_prec_diag_device(tfs_matrix *A, double *d_out);
  * returns the mean launch duration in ms from HIP events on that stream. */
 int tfs_spmv_time(tfs_matrix *A, const double *d_x, double *d_y, int reps, double *ms);
 
+/* Kernel classes for event timing during real solves. */
+enum {
+  TFS_KC_SPMV = 0,    /* k_spmv in every mode                                  */
+  TFS_KC_SWEEP = 1,   /* forward / backward preconditioner sweeps (per level) */
+  TFS_KC_FACTOR = 2,  /* Prec_Form level kernels                               */
+  TFS_KC_VECTOR = 3   /* reserved: vector update / reduction kernels           */
+};
+/* From now on every launch of `kclass` on the library stream is bracketed by
+ * a HIP event pair (up to 4096 pairs between stream synchronisations). */
+int tfs_timing_start(int kclass);
+/* Stops recording; total bracketed milliseconds and number of launches. */
+int tfs_timing_stop(double *ms, long long *launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -373,6 +373,41 @@ struct Lib {
 };
 Lib g_lib;
 
+// Optional HIP-event timing of one kernel class on the library stream: an
+// event pair brackets every launch of that class; pairs are harvested at the
+// next stream synchronisation.
+struct Timing {
+  bool on = false;
+  int kclass = -1;
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  double ms = 0.0;
+  long long launches = 0;
+};
+Timing g_tm;
+
+inline bool tm_active(int kc) { return g_tm.on && g_tm.kclass == kc && g_tm.used + 2 <= g_tm.ev.size(); }
+inline void tm_begin(int kc) {
+  if (tm_active(kc)) (void)hipEventRecord(g_tm.ev[g_tm.used], g_lib.stream);
+}
+inline void tm_end(int kc) {
+  if (tm_active(kc)) {
+    (void)hipEventRecord(g_tm.ev[g_tm.used + 1], g_lib.stream);
+    g_tm.used += 2;
+  }
+}
+// Call with the stream idle.
+void tm_harvest() {
+  for (size_t k = 0; k + 1 < g_tm.used; k += 2) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, g_tm.ev[k], g_tm.ev[k + 1]) == hipSuccess) {
+      g_tm.ms += t;
+      g_tm.launches++;
+    }
+  }
+  g_tm.used = 0;
+}
+
 #define TFS_HIP(call)                                                               \
   do {                                                                              \
     hipError_t e_ = (call);                                                         \
@@ -503,7 +538,9 @@ template <int OP>
 void launch_sweep(const tfs_matrix *A, const Schedule &S, double *d, const double *b, double *x,
                   const KState *st) {
   const RowsCRS M{A->d_row, A->d_col, A->d_dia, A->d_val};
+  const int kc = OP == OP_FACTOR ? TFS_KC_FACTOR : TFS_KC_SWEEP;
   for (const Segment &g : S.segs) {
+    tm_begin(kc);
     if (g.small) {
       hipLaunchKernelGGL(k_levels_small<OP>, dim3(1), dim3(kSmallThreads), 0, g_lib.stream, g.l0,
                          g.l1, S.d_ptr, S.d_perm, M, d, b, x, st);
@@ -512,6 +549,7 @@ void launch_sweep(const tfs_matrix *A, const Schedule &S, double *d, const doubl
       hipLaunchKernelGGL(k_level<OP>, dim3((p1 - p0 + kLevelThreads - 1) / kLevelThreads),
                          dim3(kLevelThreads), 0, g_lib.stream, p0, p1, S.d_perm, M, d, b, x, st);
     }
+    tm_end(kc);
   }
 }
 
@@ -548,9 +586,11 @@ void prec_apply(const tfs_matrix *A, int prec, const double *b, double *x, const
 template <int MODE>
 void launch_spmv(const tfs_matrix *A, const double *x, double *y, const double *w, double *xs,
                  double *rs, KState *st) {
+  tm_begin(TFS_KC_SPMV);
   hipLaunchKernelGGL(k_spmv<MODE>, dim3(grid_for(A->nblk, kSpmvWaves)), dim3(kSpmvThreads), 0,
                      g_lib.stream, A->nblk, A->d_rblk, A->d_row, A->d_col, A->d_val, x, y, w, xs,
                      rs, st, A->d_partials);
+  tm_end(TFS_KC_SPMV);
 }
 
 int ensure_vectors(tfs_matrix *A, int count) {
@@ -607,6 +647,7 @@ void cgs_iteration(tfs_matrix *A, int prec, double *x, double *r1, double *v) {
 int read_state(tfs_matrix *A) {
   TFS_HIP(hipMemcpyAsync(A->h_st, A->d_st, sizeof(KState), hipMemcpyDeviceToHost, g_lib.stream));
   TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  if (g_tm.on) tm_harvest();
   return 0;
 }
 
@@ -939,5 +980,31 @@ extern "C" int tfs_spmv_time(tfs_matrix *A, const double *d_x, double *d_y, int 
   float t = 0.f;
   TFS_HIP(hipEventElapsedTime(&t, g_lib.ev0, g_lib.ev1));
   *ms = (double)t / reps;
+  return TFS_OK;
+}
+
+extern "C" int tfs_timing_start(int kclass) {
+  if (kclass < TFS_KC_SPMV || kclass > TFS_KC_VECTOR) return TFS_ERR_INVALID;
+  TFS_TRY(ensure_init());
+  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  if (g_tm.ev.empty()) {
+    g_tm.ev.resize(2 * 4096);
+    for (auto &e : g_tm.ev) TFS_HIP(hipEventCreate(&e));
+  }
+  g_tm.on = true;
+  g_tm.kclass = kclass;
+  g_tm.used = 0;
+  g_tm.ms = 0.0;
+  g_tm.launches = 0;
+  return TFS_OK;
+}
+
+extern "C" int tfs_timing_stop(double *ms, long long *launches) {
+  if (!g_tm.on) return TFS_ERR_INVALID;
+  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  tm_harvest();
+  g_tm.on = false;
+  if (ms) *ms = g_tm.ms;
+  if (launches) *launches = g_tm.launches;
   return TFS_OK;
 }

@@ -71,6 +71,18 @@ def iters_close(a, b):
     return abs(a - b) <= max(1, int(0.05 * b))
 
 
+def iters_close_cgs(oracle, c, prec, tol, a, b):
+    """CGS's residual is not monotone and its count moves with rounding: the
+    reference recurrence itself needs 84 or 89 iterations on poisson_perm with
+    prec 1 depending only on the dot-product summation order.  The bar is +-5%
+    around the spread of the reference recurrence over summation orders."""
+    if iters_close(a, b):
+        return True
+    from krylov_variants import cgs_spread
+    lo, hi = cgs_spread(oracle, c, prec, tol)
+    return lo - max(1, int(0.05 * lo)) <= a <= hi + max(1, int(0.05 * hi))
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_spmv_and_residual_bitwise(torch, S, oracle, name):
     c = get_case(name)
@@ -113,7 +125,10 @@ def test_solve_parity(torch, S, oracle, name, method, prec):
     x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
     r = dev(torch, c.b)
     niter, ini, fin = S.solve(method, A, x, r, prec, cap, tol)
-    assert iters_close(niter, ref["niter"]), (niter, ref["niter"])
+    if method == "cgs":
+        assert iters_close_cgs(oracle, c, prec, tol, niter, ref["niter"]), (niter, ref["niter"])
+    else:
+        assert iters_close(niter, ref["niter"]), (niter, ref["niter"])
     assert abs(ini - ref["ini_res"]) <= 1e-12 * ref["ini_res"]
     if ref["niter"] <= cap:  # the reference converged
         assert fin < tol and A.status == S.TFS_OK

@@ -337,3 +337,15 @@ def test_planted_factor_mistakes_are_caught(variant, caught):
     M = precond_matrix(c, _prec_form_py(c, variant))
     ok = np.allclose(np.diag(M), np.diag(A), rtol=1e-12)
     assert ok != caught
+
+
+def test_cgs_count_moves_with_summation_order(oracle):
+    """Why CGS parity is judged against a spread: on this case the reference
+    recurrence with left-to-right dots reproduces the oracle exactly (84
+    iterations) while reversed-order dots need 89."""
+    from krylov_variants import ORDERS, cgs_iterations
+    c = mesh.make_case(kind=mesh.POISSON, nx=10, ny=8, nz=6, beta=(0.0, 2.0, 3.0), permute=True)
+    tol = 1e-10 * np.linalg.norm(c.b)
+    ref = oracle.solve("cgs", c.row, c.col, c.dia, c.val, c.b, prec=1, niter=3000, tol=tol)
+    assert cgs_iterations(oracle, c, 1, tol, ORDERS["sequential"]) == ref["niter"]
+    assert abs(cgs_iterations(oracle, c, 1, tol, ORDERS["reversed"]) - ref["niter"]) >= 4
