@@ -26,8 +26,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-HIP_SOURCES = ["tfs_solver.hip"]
-HIP_DEPS = ["tfs_device.hpp"]
+HIP_SOURCES = ["tfs_solver.hip", "tfs_layout.cpp"]
+HIP_DEPS = ["tfs_device.hpp", "tfs_layout.hpp"]
 
 
 def _run(cmd, cwd=None):

@@ -1,0 +1,215 @@
+// tfs_layout.cpp -- host-side construction of the S / L / U layouts.
+#include "tfs_layout.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+
+namespace tfs {
+namespace {
+
+// Breadth-first search from `start` over nodes whose mark differs from
+// `stamp` (marking them); neighbours are taken in ascending degree, ties by
+// number (Cuthill-McKee).  Appends the visit order to `out`, returns the
+// number of BFS levels and, if asked, the nodes of the last level.
+int cm_bfs(int start, int stamp, const std::vector<int> &row, const std::vector<int> &adj,
+           const std::vector<int> &deg, std::vector<int> &mark, std::vector<int> &out,
+           std::vector<int> *last_level) {
+  size_t head = out.size(), level_begin = head, level_end = head + 1;
+  out.push_back(start);
+  mark[start] = stamp;
+  int levels = 1;
+  std::vector<int> nb;
+  while (head < out.size()) {
+    if (head == level_end) {
+      level_begin = level_end;
+      level_end = out.size();
+      ++levels;
+    }
+    const int v = out[head++];
+    nb.clear();
+    for (int e = row[v]; e < row[v + 1]; ++e)
+      if (mark[adj[e]] != stamp) {
+        mark[adj[e]] = stamp;
+        nb.push_back(adj[e]);
+      }
+    std::sort(nb.begin(), nb.end(), [&](int a, int b) {
+      return deg[a] != deg[b] ? deg[a] < deg[b] : a < b;
+    });
+    out.insert(out.end(), nb.begin(), nb.end());
+  }
+  if (last_level) last_level->assign(out.begin() + (long)level_begin, out.end());
+  return levels;
+}
+
+// Reverse Cuthill-McKee (Cuthill & McKee 1969; George 1971), starting each
+// connected component from a pseudo-peripheral node (George & Liu 1979).
+std::vector<int> rcm(int n, const int *row, const int *col) {
+  // symmetric adjacency without the diagonal
+  std::vector<int> cnt(n + 1, 0);
+  for (int i = 0; i < n; ++i)
+    for (int e = row[i]; e < row[i + 1]; ++e)
+      if (col[e] != i) {
+        cnt[i + 1]++;
+        cnt[col[e] + 1]++;
+      }
+  for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+  std::vector<int> adj(cnt[n]), fill(cnt.begin(), cnt.end() - 1);
+  for (int i = 0; i < n; ++i)
+    for (int e = row[i]; e < row[i + 1]; ++e)
+      if (col[e] != i) {
+        adj[fill[i]++] = col[e];
+        adj[fill[col[e]]++] = i;
+      }
+  // drop duplicates (a structurally symmetric pattern lists each pair twice)
+  std::vector<int> arow(n + 1, 0), a2;
+  a2.reserve(adj.size() / 2 + 1);
+  for (int i = 0; i < n; ++i) {
+    std::sort(adj.begin() + cnt[i], adj.begin() + cnt[i + 1]);
+    int prev = -1;
+    for (int e = cnt[i]; e < cnt[i + 1]; ++e)
+      if (adj[e] != prev) a2.push_back(prev = adj[e]);
+    arow[i + 1] = (int)a2.size();
+  }
+  std::vector<int>().swap(adj);
+  std::vector<int> deg(n);
+  for (int i = 0; i < n; ++i) deg[i] = arow[i + 1] - arow[i];
+
+  // mark: 0 = unvisited; 1 = placed in `order`; >= 2 = probe stamps
+  std::vector<int> mark(n, 0), order, tmp, last;
+  order.reserve(n);
+  int stamp = 2;
+  for (int s0 = 0; s0 < n; ++s0) {
+    if (mark[s0] == 1) continue;
+    int r = s0, ecc = -1;
+    for (int it = 0; it < 4; ++it) {
+      tmp.clear();
+      // probe BFS inside the component (placed nodes carry mark 1 and are
+      // in other components, so they are never reached)
+      const int levels = cm_bfs(r, stamp++, arow, a2, deg, mark, tmp, &last);
+      if (levels <= ecc) break;
+      ecc = levels;
+      int best = last[0];
+      for (int v : last)
+        if (deg[v] < deg[best] || (deg[v] == deg[best] && v < best)) best = v;
+      r = best;
+    }
+    cm_bfs(r, 1, arow, a2, deg, mark, order, nullptr);
+  }
+  std::reverse(order.begin(), order.end());
+  return order;  // order[k] = caller row placed at position k
+}
+
+void build_sweep(int n, const int *row, const int *col, const int *dia, bool lower,
+                 const std::vector<int> &s_of_orig, int small_level, SweepLayout &S) {
+  std::vector<int> lev(n, 0);
+  int nlev = 0;
+  if (lower) {
+    for (int i = 0; i < n; ++i) {
+      int l = 0;
+      for (int j = row[i]; j < dia[i]; ++j) l = std::max(l, lev[col[j]] + 1);
+      lev[i] = l;
+      nlev = std::max(nlev, l + 1);
+    }
+  } else {
+    for (int i = n - 1; i >= 0; --i) {
+      int l = 0;
+      for (int j = dia[i] + 1; j < row[i + 1]; ++j) l = std::max(l, lev[col[j]] + 1);
+      lev[i] = l;
+      nlev = std::max(nlev, l + 1);
+    }
+  }
+  S.nlev = nlev;
+  S.lev_ptr.assign(nlev + 1, 0);
+  for (int i = 0; i < n; ++i) S.lev_ptr[lev[i] + 1]++;
+  for (int l = 0; l < nlev; ++l) S.lev_ptr[l + 1] += S.lev_ptr[l];
+  // order: by level, ties in S order (walk caller rows in S order)
+  std::vector<int> s_orig(n);
+  for (int i = 0; i < n; ++i) s_orig[s_of_orig[i]] = i;
+  std::vector<int> fill(S.lev_ptr.begin(), S.lev_ptr.end() - 1);
+  S.orig.assign(n, 0);
+  S.of_orig.assign(n, 0);
+  for (int k = 0; k < n; ++k) {
+    const int i = s_orig[k];
+    const int p = fill[lev[i]]++;
+    S.orig[p] = i;
+    S.of_orig[i] = p;
+  }
+  S.ptr.assign(n + 1, 0);
+  for (int p = 0; p < n; ++p) {
+    const int i = S.orig[p];
+    S.ptr[p + 1] = S.ptr[p] + (lower ? dia[i] - row[i] : row[i + 1] - dia[i] - 1);
+  }
+  S.col.assign(S.ptr[n], 0);
+  S.src.assign(S.ptr[n], 0);
+  S.diag_src.assign(n, 0);
+  for (int p = 0; p < n; ++p) {
+    const int i = S.orig[p];
+    const int j0 = lower ? row[i] : dia[i] + 1, j1 = lower ? dia[i] : row[i + 1];
+    int e = S.ptr[p];
+    for (int j = j0; j < j1; ++j, ++e) {
+      S.col[e] = S.of_orig[col[j]];
+      S.src[e] = j;
+    }
+    S.diag_src[p] = dia[i];
+  }
+  S.segs.clear();
+  for (int l = 0; l < nlev; ++l) {
+    const int rows = S.lev_ptr[l + 1] - S.lev_ptr[l];
+    const bool small = rows <= small_level;
+    if (small && !S.segs.empty() && S.segs.back().small) {
+      S.segs.back().l1 = l + 1;
+      S.segs.back().rows += rows;
+    } else {
+      S.segs.push_back({l, l + 1, rows, small});
+    }
+  }
+}
+
+}  // namespace
+
+void build_layout(int n, const int *row, const int *col, const int *dia, int spmv_rows,
+                  int spmv_cap, int small_level, Layout &out) {
+  out.n = n;
+  out.nnz = n > 0 ? row[n] : 0;
+  out.s_orig = rcm(n, row, col);
+  out.s_of_orig.assign(n, 0);
+  for (int k = 0; k < n; ++k) out.s_of_orig[out.s_orig[k]] = k;
+  out.s_row.assign(n + 1, 0);
+  for (int k = 0; k < n; ++k) {
+    const int i = out.s_orig[k];
+    out.s_row[k + 1] = out.s_row[k] + row[i + 1] - row[i];
+  }
+  out.s_col.assign(out.nnz, 0);
+  out.s_src.assign(out.nnz, 0);
+  out.s_diag_src.assign(n, 0);
+  int bw = 0;
+  for (int k = 0; k < n; ++k) {
+    const int i = out.s_orig[k];
+    int e = out.s_row[k];
+    for (int j = row[i]; j < row[i + 1]; ++j, ++e) {
+      out.s_col[e] = out.s_of_orig[col[j]];
+      out.s_src[e] = j;
+      bw = std::max(bw, std::abs(out.s_col[e] - k));
+    }
+    out.s_diag_src[k] = dia[i];
+  }
+  out.bandwidth = bw;
+  // SpMV row blocks: <= spmv_rows rows and <= spmv_cap entries each
+  out.s_rblk.assign(1, 0);
+  for (int r = 0; r < n;) {
+    int e = r + 1;
+    while (e < n && e - r < spmv_rows && out.s_row[e + 1] - out.s_row[r] <= spmv_cap) ++e;
+    out.s_rblk.push_back(e);
+    r = e;
+  }
+  build_sweep(n, row, col, dia, true, out.s_of_orig, small_level, out.L);
+  build_sweep(n, row, col, dia, false, out.s_of_orig, small_level, out.U);
+  out.l_to_s.assign(n, 0);
+  out.u_to_l.assign(n, 0);
+  out.s_to_u.assign(n, 0);
+  for (int p = 0; p < n; ++p) out.l_to_s[p] = out.s_of_orig[out.L.orig[p]];
+  for (int q = 0; q < n; ++q) out.u_to_l[q] = out.L.of_orig[out.U.orig[q]];
+  for (int k = 0; k < n; ++k) out.s_to_u[k] = out.U.of_orig[out.s_orig[k]];
+}
+
+}  // namespace tfs

@@ -1,0 +1,58 @@
+// tfs_layout.hpp -- device data layouts of one CRS matrix (host-side build).
+//
+// The solver keeps the matrix in three row orders, each chosen for the
+// memory access pattern of the kernels that use it.  None changes any
+// arithmetic: rows keep their entries in the caller's column order, so every
+// row sum is formed exactly as the reference forms it.
+//
+//   S  (SpMV and the Krylov vectors): reverse Cuthill-McKee order of the
+//      pattern, so that the x gathers of a wavefront's rows fall in a narrow
+//      band (L2-resident) even when the caller numbers cells randomly.
+//   L  (factor and forward sweep): rows grouped by level of the lower
+//      dependency DAG of the CALLER's numbering (which defines the
+//      incomplete factor), ties in S order; only the strictly-lower entries
+//      are stored, contiguously, level after level.
+//   U  (backward sweep): the same for the upper DAG and strictly-upper entries.
+#pragma once
+
+#include <vector>
+
+namespace tfs {
+
+struct Segment {
+  int l0, l1;  // levels [l0, l1)
+  int rows;
+  bool small;  // a run of small levels walked by one workgroup
+};
+
+struct SweepLayout {
+  int nlev = 0;
+  std::vector<int> lev_ptr;   // nlev+1 position starts of the levels
+  std::vector<int> ptr;       // n+1 offsets of each position's triangle entries
+  std::vector<int> col;       // neighbour positions in this layout
+  std::vector<int> src;       // index of the entry in the caller's val[]
+  std::vector<int> diag_src;  // index of the row's diagonal in val[]
+  std::vector<int> orig;      // caller's row number of each position
+  std::vector<int> of_orig;   // position of each caller row
+  std::vector<Segment> segs;
+};
+
+struct Layout {
+  int n = 0;
+  long long nnz = 0;
+  // S layout
+  std::vector<int> s_orig, s_of_orig;   // S position <-> caller row
+  std::vector<int> s_row, s_col, s_src, s_diag_src;
+  std::vector<int> s_rblk;              // SpMV row blocks over S rows
+  SweepLayout L, U;
+  std::vector<int> l_to_s;  // S position of the row at L position p
+  std::vector<int> u_to_l;  // L position of the row at U position q
+  std::vector<int> s_to_u;  // U position of the row at S position k
+  int bandwidth = 0;        // max |S(i) - S(j)| over entries (diagnostic)
+};
+
+// row/col/dia are 0-based, validated (diagonal present, columns ascending).
+void build_layout(int n, const int *row, const int *col, const int *dia, int spmv_rows,
+                  int spmv_cap, int small_level, Layout &out);
+
+}  // namespace tfs

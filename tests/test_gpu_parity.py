@@ -99,8 +99,16 @@ def test_spmv_and_residual_bitwise(torch, S, oracle, name):
 
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("prec", [0, 1, 2])
-def test_factor_and_sweeps_bitwise(torch, S, oracle, name, prec):
+@pytest.mark.parametrize("sweep", ["persistent", "levels"])
+def test_factor_and_sweeps_bitwise(torch, S, oracle, name, prec, sweep, monkeypatch):
+    """Both sweep schedules: one persistent launch per sweep, and one launch
+    per level (TFS_SWEEP=levels, read when the matrix is created)."""
     c = get_case(name)
+    print(f"case {name} prec={prec} sweep={sweep}", flush=True)
+    if sweep == "levels":
+        monkeypatch.setenv("TFS_SWEEP", "levels")
+    else:
+        monkeypatch.delenv("TFS_SWEEP", raising=False)
     A = matrix(S, c)
     A.prec_form(prec)
     d = torch.zeros(c.n, dtype=torch.float64, device="cuda")
