@@ -97,6 +97,20 @@ int tfs_solve(tfs_matrix *A, int method, int prec, double *x1, double *r1,
 int tfs_solve_device(tfs_matrix *A, int method, int prec, double *d_x, double *d_r1,
                      int *niter, double tol, double *ini_res, double *fin_res);
 
+/* ---- FP32-inner solve inside FP64 iterative refinement -------------------------
+ * Not in the reference: values and inner Krylov vectors stream in FP32 while
+ * the outer residual b - A x, the solution update and every norm stay FP64
+ * on the device.  Defect correction:  r = b - A x;  repeat { solve A e = r by
+ * FP32 PCG (the prec factor recomputed in FP32 from FP32 values) to
+ * ||r_inner|| < inner_rtol ||r||;  x += e;  r = b - A x }  until ||r|| < tol.
+ * niter: in = cap on refinement steps; out = steps taken (cap+1 if tol was not
+ * reached, which also happens when FP32 stops reducing the defect);
+ * inner_iters = FP32 PCG iterations over all steps; ini_res = ||b - A x0||;
+ * fin_res = ||b - A x|| (the true FP64 residual, left in r1). */
+int tfs_solve_mixed_device(tfs_matrix *A, int prec, double *d_x, double *d_r1, int *niter,
+                           int *inner_iters, double tol, double inner_rtol, double *ini_res,
+                           double *fin_res);
+
 /* ---- building blocks on device vectors ---------------------------------------- */
 /* y := A x  (row sums in the reference order, Cg.f90:128-134). */
 int tfs_spmv_device(tfs_matrix *A, const double *d_x, double *d_y);
@@ -108,6 +122,9 @@ int tfs_prec_form(tfs_matrix *A, int prec);
 int tfs_prec_solve_device(tfs_matrix *A, int prec, const double *d_b, double *d_x);
 /* Copies the n used diagonal entries of the preconditioner D to d_out. */
 int tfs_prec_diag_device(tfs_matrix *A, double *d_out);
+/* The same for the FP32 factor of the refinement solver (built if needed),
+ * widened to FP64. */
+int tfs_prec_diag32_device(tfs_matrix *A, int prec, double *d_out);
 
 /* ---- measurement ----------------------------------------------------------------
  * Runs `reps` back-to-back SpMV launches y := A x on the library's stream and

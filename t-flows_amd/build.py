@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 HIP_SOURCES = ["tfs_solver.hip", "tfs_layout.cpp"]
-HIP_DEPS = ["tfs_device.hpp", "tfs_layout.hpp"]
+HIP_DEPS = ["tfs_device.hpp", "tfs_kernels.hpp", "tfs_layout.hpp"]
 
 
 def _run(cmd, cwd=None):

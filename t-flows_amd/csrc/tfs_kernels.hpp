@@ -1,0 +1,503 @@
+// tfs_kernels.hpp -- the solver's device kernels (gfx950, wave64).
+//
+// Value-carrying kernels are templates on the storage type T (double for the
+// drop-in, float for the inner solves of iterative refinement); every dot
+// product and norm accumulates in double whatever T is.
+//
+//   k_spmv          CSR SpMV over S rows, one wavefront per row block
+//                   (<= 64 rows, <= kSpmvCap entries): col/val streamed with
+//                   consecutive lanes on consecutive entries, products staged
+//                   in LDS, each lane then sums its row left to right -- the
+//                   reference's order (Cg.f90:128-134), so FP64 results are
+//                   bitwise the reference's.  No MFMA: SpMV is not a dense
+//                   contraction.
+//   k_sweep_syncfree  factor / forward / backward sweeps (Prec_Form.f90:32-40,
+//                   Prec_Solve.f90:46-67) in one persistent launch each, rows
+//                   waiting only on their own dependencies (value-as-flag).
+//   k_level, k_levels_small  the same sweeps with one launch per level (used
+//                   where a cooperative launch is unavailable).
+//   vector kernels  single passes writing per-workgroup partial sums; k_finish
+//                   reduces them in a fixed order (deterministic, fence-free)
+//                   and advances the device-resident solve state.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tfs_device.hpp"
+
+namespace tfs {
+
+constexpr int kSpmvWaves = 4;            // waves per SpMV workgroup
+constexpr int kSpmvThreads = kSpmvWaves * kWave;
+constexpr int kSpmvRows = 64;            // rows per SpMV row block (one per lane)
+constexpr int kSpmvCap = 512;            // entries per row block staged in LDS
+constexpr int kRedThreads = 256;         // threads of the vector kernels
+constexpr int kMaxGrid = 2048;           // bound of every partial-sum grid
+constexpr int kLevelThreads = 256;       // threads per block, one level launch
+constexpr int kSmallThreads = 1024;      // one workgroup walking small levels
+constexpr int kSmallLevel = 2048;        // a level this small joins a walked run
+constexpr int kChunk = 256;              // positions per persistent-sweep chunk
+constexpr int kPrefetch = 8;             // row entries held in registers
+constexpr unsigned kSpinLimit = 1u << 20;
+
+enum Epi { EPI_BNRM = 0, EPI_INIRES, EPI_RHO, EPI_ALPHA, EPI_NORM };
+
+// ---------------------------------------------------------------------------
+// Solve-state transitions (Cg.f90:48-58,75-90,108-122,148-153,167-181).
+__device__ inline void epilogue(int epi, double tot, KState *st) {
+  switch (epi) {
+    case EPI_BNRM: {
+      const double b = sqrt(tot);
+      st->bnrm2 = b;
+      if (b < st->tol) {
+        st->done = 1; st->niter_out = 0; st->fin_res = b; st->err = b;
+      }
+      break;
+    }
+    case EPI_INIRES: {
+      const double e = sqrt(tot);
+      st->err = e;
+      st->ini_res = e;
+      if (e < st->tol) {
+        st->done = 1; st->niter_out = 0; st->fin_res = e;
+      }
+      break;
+    }
+    case EPI_RHO:
+      st->rho = tot;
+      if (st->iter + 1 > 1) st->beta = tot / st->rho_old;
+      break;
+    case EPI_ALPHA:
+      st->pq = tot;
+      st->alpha = st->rho / tot;
+      break;
+    case EPI_NORM: {
+      const double e = sqrt(tot);
+      const int it = st->iter + 1;
+      st->iter = it;
+      st->err = e;
+      if (e < st->tol) {
+        st->done = 1; st->niter_out = it; st->fin_res = e; st->status = kStOk;
+      } else {
+        st->rho_old = st->rho;
+        if (!isfinite(e)) {
+          st->done = 1; st->niter_out = st->cap + 1; st->fin_res = e; st->status = kStBreakdown;
+        } else if (it >= st->cap) {
+          st->done = 1; st->niter_out = st->cap + 1; st->fin_res = e;
+          st->status = kStNotConverged;
+        }
+      }
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ void store_partial(double v, double *partials) {
+  __shared__ double lds[NT / kWave];
+  const double s = block_sum<NT>(v, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// Sums the partials of the previous kernel in index order and applies the
+// state transition (one workgroup; visibility comes from the kernel boundary).
+__global__ void __launch_bounds__(kRedThreads)
+k_finish(int nparts, const double *__restrict__ partials, int epi, KState *st) {
+  if (epi != EPI_BNRM && solve_done(st)) return;
+  __shared__ double lds[kRedThreads / kWave];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < nparts; k += kRedThreads) v += partials[k];
+  v = block_sum<kRedThreads>(v, lds);
+  if (threadIdx.x == 0) epilogue(epi, v, st);
+}
+
+#define TFS_GRID_STRIDE(i, n) \
+  for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < (n); i += gridDim.x * kRedThreads)
+
+// ---------------------------------------------------------------------------
+// Vector kernels (single pass each).
+
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_sumsq(int n, const T *__restrict__ v, int epi, const KState *st, double *partials) {
+  if (epi != EPI_BNRM && solve_done(st)) return;
+  double s = 0.0;
+  TFS_GRID_STRIDE(i, n) {
+    const double a = v[i];
+    s += a * a;
+  }
+  store_partial<kRedThreads>(s, partials);
+}
+
+// rho = (r, z) where z comes from: 0 -> z is r itself; 1 -> z = r / d
+// (Prec_Solve.f90:35-38), written; 2 -> z = zU[s_to_u[i]] (the backward sweep
+// result, gathered into S order), written; 3 -> z given.
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_rho(int n, const T *__restrict__ r, T *__restrict__ z, const T *__restrict__ d,
+      const T *__restrict__ zU, const int *__restrict__ s_to_u, int mode, const KState *st,
+      double *partials) {
+  if (solve_done(st)) return;
+  double s = 0.0;
+  TFS_GRID_STRIDE(i, n) {
+    const T ri = r[i];
+    T zi;
+    if (mode == 0) {
+      zi = ri;
+    } else if (mode == 1) {
+      zi = ri / d[i];
+      z[i] = zi;
+    } else if (mode == 2) {
+      zi = zU[s_to_u[i]];
+      z[i] = zi;
+    } else {
+      zi = z[i];
+    }
+    s += (double)ri * (double)zi;
+  }
+  store_partial<kRedThreads>(s, partials);
+}
+
+// p = z (first iteration) or p = z + beta p  (Cg.f90:114-123)
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_pupdate(int n, const T *__restrict__ z, T *__restrict__ p, const KState *st) {
+  if (solve_done(st)) return;
+  const bool first = st->iter == 0;
+  const T beta = (T)st->beta;
+  TFS_GRID_STRIDE(i, n) p[i] = first ? z[i] : z[i] + beta * p[i];
+}
+
+// x += alpha p; r -= alpha q; partial ||r||^2  (Cg.f90:159-172)
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_axpy_norm(int n, T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
+            const T *__restrict__ q, const KState *st, double *partials) {
+  if (solve_done(st)) return;
+  const T alpha = (T)st->alpha;
+  double s = 0.0;
+  TFS_GRID_STRIDE(i, n) {
+    x[i] = x[i] + alpha * p[i];
+    const T ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    s += (double)ri * (double)ri;
+  }
+  store_partial<kRedThreads>(s, partials);
+}
+
+// CGS: u1 = r1 + beta q1; u2 = u1 + beta (q1 + beta u2)  (Cgs.f90:115-126)
+__global__ void __launch_bounds__(kRedThreads)
+k_cgs_u(int n, const double *__restrict__ r1, const double *__restrict__ q1,
+        double *__restrict__ u1, double *__restrict__ u2, const KState *st) {
+  if (solve_done(st)) return;
+  const bool first = st->iter == 0;
+  const double beta = st->beta;
+  TFS_GRID_STRIDE(i, n) {
+    if (first) {
+      u1[i] = r1[i];
+      u2[i] = r1[i];
+    } else {
+      const double a = r1[i] + beta * q1[i];
+      u1[i] = a;
+      u2[i] = a + beta * (q1[i] + beta * u2[i]);
+    }
+  }
+}
+
+// CGS: q1 = u1 - alpha v2; w = u1 + q1  (Cgs.f90:167-176)
+__global__ void __launch_bounds__(kRedThreads)
+k_cgs_q(int n, const double *__restrict__ u1, const double *__restrict__ v2,
+        double *__restrict__ q1, double *__restrict__ w, const KState *st) {
+  if (solve_done(st)) return;
+  const double alpha = st->alpha;
+  TFS_GRID_STRIDE(i, n) {
+    const double q = u1[i] - alpha * v2[i];
+    q1[i] = q;
+    w[i] = u1[i] + q;
+  }
+}
+
+// out[i] = (TO) in[idx[i]] (idx may be null); skipped once the solve is done.
+template <typename TO, typename TI>
+__global__ void __launch_bounds__(kRedThreads)
+k_gather(int n, const TI *__restrict__ in, const int *__restrict__ idx, TO *__restrict__ out,
+         const KState *st) {
+  if (st && solve_done(st)) return;
+  TFS_GRID_STRIDE(i, n) out[i] = (TO)in[idx ? idx[i] : i];
+}
+
+// out[idx[i]] = in[i]
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_scatter(int n, const T *__restrict__ in, const int *__restrict__ idx, T *__restrict__ out) {
+  TFS_GRID_STRIDE(i, n) out[idx[i]] = in[i];
+}
+
+// x = b / d (prec 1) or x = b (prec 0)
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_diag_apply(int n, const T *__restrict__ b, const T *__restrict__ d, T *__restrict__ x, int prec,
+             const KState *st) {
+  if (st && solve_done(st)) return;
+  TFS_GRID_STRIDE(i, n) x[i] = prec == 1 ? b[i] / d[i] : b[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_fill(int n, T *__restrict__ x, T v) {
+  TFS_GRID_STRIDE(i, n) x[i] = v;
+}
+
+// Iterative refinement: x += e (FP64 update with the FP32 correction).
+__global__ void __launch_bounds__(kRedThreads)
+k_add_correction(int n, double *__restrict__ x, const float *__restrict__ e) {
+  TFS_GRID_STRIDE(i, n) x[i] = x[i] + (double)e[i];
+}
+
+// ---------------------------------------------------------------------------
+// SpMV over S rows.  Modes: plain y = A x; residual y = y - A x (term by term,
+// Residual.f90:22-27); dot: y = A x and partial sum of w.y (Cg.f90:148-153,
+// Cgs.f90:157-162); cgs tail: q2 = A p1, xs += alpha p1, rs -= alpha q2 and
+// partial ||rs||^2 (Cgs.f90:182-221, all row-local once alpha is known).
+enum SpmvMode { SPMV_PLAIN = 0, SPMV_RESID, SPMV_DOT, SPMV_CGS_TAIL };
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kSpmvThreads)
+k_spmv(int nblk, const int *__restrict__ rblk, const int *__restrict__ row,
+       const int *__restrict__ col, const T *__restrict__ val, const T *__restrict__ x,
+       T *__restrict__ y, const T *__restrict__ w, T *__restrict__ xs, T *__restrict__ rs,
+       const KState *st, double *partials) {
+  if (st && solve_done(st)) return;
+  __shared__ T prod[kSpmvWaves][kSpmvCap];
+  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  T *lds = prod[wv];
+  const T alpha = MODE == SPMV_CGS_TAIL ? (T)st->alpha : T(0);
+  double acc_red = 0.0;
+  for (int b = blockIdx.x * kSpmvWaves + wv; b < nblk; b += gridDim.x * kSpmvWaves) {
+    const int r0 = rblk[b], r1 = rblk[b + 1];
+    const int e0 = row[r0], e1 = row[r1];
+    T yr = T(0);
+    int r = -1;
+    if (e1 - e0 <= kSpmvCap) {
+      for (int e = e0 + lane; e < e1; e += kWave) lds[e - e0] = val[e] * x[col[e]];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (lane < r1 - r0) {
+        r = r0 + lane;
+        const int a = row[r] - e0, z = row[r + 1] - e0;
+        T acc = MODE == SPMV_RESID ? y[r] : T(0);
+        if (MODE == SPMV_RESID) {
+          for (int k = a; k < z; ++k) acc = acc - lds[k];
+        } else {
+          for (int k = a; k < z; ++k) acc = acc + lds[k];
+        }
+        yr = acc;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      // one long row for the whole wave (tree order; rows this long do not
+      // occur in finite-volume stencils)
+      T acc = T(0);
+      for (int e = e0 + lane; e < e1; e += kWave) acc += val[e] * x[col[e]];
+      acc = wave_sum(acc);
+      if (lane == 0) {
+        r = r0;
+        yr = MODE == SPMV_RESID ? y[r] - acc : acc;
+      }
+    }
+    if (r >= 0) {
+      y[r] = yr;
+      if (MODE == SPMV_DOT) acc_red += (double)w[r] * (double)yr;
+      if (MODE == SPMV_CGS_TAIL) {
+        xs[r] = xs[r] + alpha * x[r];
+        const T ri = rs[r] - alpha * yr;
+        rs[r] = ri;
+        acc_red += (double)ri * (double)ri;
+      }
+    }
+  }
+  if (MODE == SPMV_DOT || MODE == SPMV_CGS_TAIL) store_partial<kSpmvThreads>(acc_red, partials);
+}
+
+// ---------------------------------------------------------------------------
+// The diagonal incomplete Cholesky over a level-contiguous layout.
+//   ptr/col/val : the triangle entries of each position (caller's order),
+//                 col in this layout's positions;
+//   factor : d[p] = 1 / (diag[p] - sum d[col] val val)        (Prec_Form.f90:32-40)
+//   fwd    : x[p] = (b[in_idx[p]] - sum val x[col]) d[p]       (Prec_Solve.f90:46-53)
+//   bwd    : x[p] = (b[in_idx[p]] / (d[p] + TINY) - sum val x[col]) d[p]
+//            -- the scale pass (:55-57) fused into the backward sweep (:60-67).
+template <typename T>
+struct Tri {
+  const int *__restrict__ ptr;
+  const int *__restrict__ col;
+  const T *__restrict__ val;
+  const T *__restrict__ diag;     // factor only
+  const int *__restrict__ in_idx; // sweeps: gather index of the right-hand side
+};
+
+enum SweepOp { OP_FACTOR = 0, OP_FWD, OP_BWD };
+
+template <typename T>
+__device__ __forceinline__ T tiny() { return (T)kTiny; }
+
+template <typename T, int OP>
+__device__ __forceinline__ void sweep_row(const Tri<T> &Tr, T *d, const T *b, T *x, int p) {
+  const int e0 = Tr.ptr[p], e1 = Tr.ptr[p + 1];
+  if (OP == OP_FACTOR) {
+    T s = Tr.diag[p];
+    for (int e = e0; e < e1; ++e) s = s - d[Tr.col[e]] * Tr.val[e] * Tr.val[e];
+    d[p] = T(1) / s;
+  } else {
+    const T dp = d[p];
+    T s = b[Tr.in_idx[p]];
+    if (OP == OP_BWD) s = s / (dp + tiny<T>());
+    for (int e = e0; e < e1; ++e) s = s - Tr.val[e] * x[Tr.col[e]];
+    x[p] = s * dp;
+  }
+}
+
+template <typename T, int OP>
+__global__ void __launch_bounds__(kLevelThreads)
+k_level(int p0, int p1, Tri<T> Tr, T *d, const T *__restrict__ b, T *x, const KState *st) {
+  if (st && solve_done(st)) return;
+  const int p = p0 + blockIdx.x * kLevelThreads + threadIdx.x;
+  if (p < p1) sweep_row<T, OP>(Tr, d, b, x, p);
+}
+
+// A run of small levels [l0, l1) walked by one workgroup; writes of one level
+// are visible to the whole workgroup after the barrier.
+template <typename T, int OP>
+__global__ void __launch_bounds__(kSmallThreads)
+k_levels_small(int l0, int l1, const int *__restrict__ lev_ptr, Tri<T> Tr, T *d,
+               const T *__restrict__ b, T *x, const KState *st) {
+  if (st && solve_done(st)) return;
+  for (int l = l0; l < l1; ++l) {
+    const int p1 = lev_ptr[l + 1];
+    for (int p = lev_ptr[l] + threadIdx.x; p < p1; p += kSmallThreads)
+      sweep_row<T, OP>(Tr, d, b, x, p);
+    __syncthreads();
+  }
+}
+
+// Sentinels: signalling NaNs, which no arithmetic produces (results of
+// arithmetic on NaNs are quiet NaNs).
+template <typename T> struct Sentinel;
+template <> struct Sentinel<double> {
+  static constexpr unsigned long long bits = 0xFFF4DEADBEEFC0DEull;
+  __device__ static double value() { return __longlong_as_double((long long)bits); }
+  __device__ static bool is(double v) { return (unsigned long long)__double_as_longlong(v) == bits; }
+};
+template <> struct Sentinel<float> {
+  static constexpr unsigned int bits = 0xFFB1DEADu;
+  __device__ static float value() { return __int_as_float((int)bits); }
+  __device__ static bool is(float v) { return (unsigned int)__float_as_int(v) == bits; }
+};
+
+template <typename T>
+__device__ __forceinline__ T coherent_load(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void coherent_store(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One persistent launch per sweep, synchronisation-free in the manner of
+// Liu et al., "A Synchronization-Free Algorithm for Parallel Sparse
+// Triangular Solves" (Euro-Par 2016), with the computed value itself as the
+// ready flag: the output array holds a sentinel until a row's result is
+// stored.  Each row waits only on its own dependencies, polling them with
+// device-coherent loads and folding them into its sum strictly in the
+// caller's entry order -- so the result is bitwise the level-scheduled one.
+// Flag and data share one location, so no fence or counter is needed.
+// Workgroup w takes chunks w, w+G, ... (kChunk positions each, in level
+// order); positions depend only on lower positions, and the cooperative launch
+// keeps all G workgroups resident, so the lowest unfinished row always belongs
+// to a running workgroup whose dependencies are complete.  Every wait is
+// bounded (kSpinLimit polls): on expiry the kernel records an error and stores
+// a value, so it always terminates.
+// Sentinel upkeep (no extra passes): FWD arms arm[p] (the backward sweep's
+// output); BWD re-arms b at the entry it consumes (the forward sweep's
+// output); FACTOR's output is armed by the caller.
+template <typename T, int OP>
+__global__ void __launch_bounds__(kChunk)
+k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, int *err) {
+  if (st && solve_done(st)) return;
+  T *out = OP == OP_FACTOR ? d : x;
+  const int nchunks = (n + kChunk - 1) / kChunk;
+  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const int p = c * kChunk + (int)threadIdx.x;
+    if (p >= n) continue;
+    const int e0 = Tr.ptr[p], e1 = Tr.ptr[p + 1], len = e1 - e0;
+    int pc[kPrefetch];
+    T pv[kPrefetch];
+#pragma unroll
+    for (int k = 0; k < kPrefetch; ++k) {
+      if (k < len) {
+        pc[k] = Tr.col[e0 + k];
+        pv[k] = Tr.val[e0 + k];
+      }
+    }
+    T s, dp = T(0);
+    if (OP == OP_FACTOR) {
+      s = Tr.diag[p];
+    } else {
+      dp = d[p];
+      const int src = Tr.in_idx[p];
+      s = b[src];
+      if (OP == OP_FWD) arm[p] = Sentinel<T>::value();
+      if (OP == OP_BWD) {
+        b[src] = Sentinel<T>::value();  // this sweep is the only reader of b[src]
+        s = s / (dp + tiny<T>());
+      }
+    }
+    int used = 0;  // entries folded into s so far, in order
+    unsigned int spins = 0;
+    for (;;) {
+      bool blocked = false;
+#pragma unroll
+      for (int k = 0; k < kPrefetch; ++k) {
+        if (!blocked && k >= used && k < len) {
+          const T v = coherent_load(&out[pc[k]]);
+          if (Sentinel<T>::is(v)) {
+            blocked = true;
+          } else {
+            s = OP == OP_FACTOR ? s - v * pv[k] * pv[k] : s - pv[k] * v;
+            used = k + 1;
+          }
+        }
+      }
+      for (int e = e0 + used; !blocked && e < e1; ++e) {  // entries beyond the prefetch
+        const T v = coherent_load(&out[Tr.col[e]]);
+        if (Sentinel<T>::is(v)) {
+          blocked = true;
+        } else {
+          const T a = Tr.val[e];
+          s = OP == OP_FACTOR ? s - v * a * a : s - a * v;
+          used = e - e0 + 1;
+        }
+      }
+      if (!blocked) break;
+      if (++spins > kSpinLimit) {
+        *err = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    coherent_store(&out[p], OP == OP_FACTOR ? T(1) / s : s * dp);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_arm(int n, T *__restrict__ x) {
+  TFS_GRID_STRIDE(i, n) x[i] = Sentinel<T>::value();
+}
+
+}  // namespace tfs

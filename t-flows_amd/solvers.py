@@ -85,6 +85,11 @@ def _lib():
     lib.tfs_prec_solve_device.argtypes = [_VP, ctypes.c_int, _VP, _VP]
     lib.tfs_prec_diag_device.argtypes = [_VP, _VP]
     lib.tfs_spmv_time.argtypes = [_VP, _VP, _VP, ctypes.c_int, _DP]
+    lib.tfs_solve_mixed_device.argtypes = [_VP, ctypes.c_int, _VP, _VP, _IP, _IP,
+                                           ctypes.c_double, ctypes.c_double, _DP, _DP]
+    lib.tfs_prec_diag32_device.argtypes = [_VP, ctypes.c_int, _VP]
+    lib.tfs_timing_start.argtypes = [ctypes.c_int]
+    lib.tfs_timing_stop.argtypes = [_DP, ctypes.POINTER(ctypes.c_longlong)]
     lib._tfs_typed = True
     return lib
 
@@ -208,6 +213,13 @@ class Matrix:
                "tfs_prec_diag_device")
         return out
 
+    def prec_diag32(self, prec, out):
+        """The FP32 factor of the refinement solver, widened to FP64."""
+        _sync()
+        _check(self._lib.tfs_prec_diag32_device(self._h, prec, _dptr(out, self.n, "out")),
+               "tfs_prec_diag32_device")
+        return out
+
     def spmv_time(self, x, y, reps=20):
         _sync()
         ms = ctypes.c_double()
@@ -254,3 +266,41 @@ def Cgs(A, x, r1, prec, niter, tol, ini_res=0.0):
 
 def solve(method, A, x, r1, prec, niter, tol, ini_res=0.0):
     return _solve(_METHODS.get(method, method), A, x, r1, prec, niter, tol, ini_res)
+
+
+def solve_mixed(A, x, r1, prec, niter, tol, inner_rtol=1e-4):
+    """FP32-inner PCG inside FP64 iterative refinement (GPU tensors only).
+
+    Returns (steps, inner_iterations, ini_res, fin_res); fin_res is the true
+    FP64 residual ||b - A x||, which is also left in r1."""
+    it = ctypes.c_int(int(niter))
+    inner = ctypes.c_int(0)
+    ir = ctypes.c_double(0.0)
+    fr = ctypes.c_double(0.0)
+    _sync()
+    rc = A._lib.tfs_solve_mixed_device(A.handle, prec, _dptr(x, A.n, "x"), _dptr(r1, A.n, "r1"),
+                                       ctypes.byref(it), ctypes.byref(inner), float(tol),
+                                       float(inner_rtol), ctypes.byref(ir), ctypes.byref(fr))
+    A.status = _check(rc, "solve_mixed")
+    return it.value, inner.value, ir.value, fr.value
+
+
+class KernelTimer:
+    """HIP-event timing of one kernel class (TFS_KC_*) on the library stream."""
+    SPMV, SWEEP, FACTOR = 0, 1, 2
+
+    def __init__(self, kclass):
+        self.kclass = kclass
+        self.ms = 0.0
+        self.launches = 0
+
+    def __enter__(self):
+        _check(_lib().tfs_timing_start(self.kclass), "tfs_timing_start")
+        return self
+
+    def __exit__(self, *exc):
+        ms = ctypes.c_double()
+        nl = ctypes.c_longlong()
+        _lib().tfs_timing_stop(ctypes.byref(ms), ctypes.byref(nl))
+        self.ms, self.launches = ms.value, nl.value
+        return False

@@ -1,0 +1,85 @@
+"""FP32-inner PCG inside FP64 iterative refinement against the FP64 oracle.
+
+The refinement solver has no counterpart in the reference; what pins it is the
+FP64 answer: the refined solution must reach the same FP64 tolerance (the
+final residual is the true ||b - A x||, computed in FP64) and agree with the
+oracle's FP64 Cg solution to rel-L2 <= 1e-8 at tol 1e-10 * ||b||.  The FP32
+factor is the reference recurrence in FP32: it agrees with the FP64 oracle
+factor to FP32 rounding (rel 1e-5).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from tflows_amd import cases, mesh
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    if not t.cuda.is_available():
+        pytest.fail("no GPU visible")
+    return t
+
+
+@pytest.fixture(scope="module")
+def S():
+    from tflows_amd import solvers
+    return solvers
+
+
+CASES = {
+    "poisson_stretched20": dict(kind=mesh.POISSON, nx=20, beta=cases.beta_for_aspect(20, 100.0)),
+    "momentum12": dict(kind=mesh.MOMENTUM, nx=12, beta=2.0, jitter=0.1, permute=True,
+                       pin_first=False, rhs_zero_mean=False),
+    "poisson_perm": dict(kind=mesh.POISSON, nx=14, ny=10, nz=8, beta=1.5, permute=True),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_refinement_reaches_fp64_solution(torch, S, oracle, name, prec):
+    c = mesh.make_case(**CASES[name])
+    print(f"case {name} prec={prec} n={c.n}", flush=True)
+    tol = 1e-10 * float(np.linalg.norm(c.b))
+    ref = oracle.solve("cg", c.row, c.col, c.dia, c.val, c.b, prec=prec, niter=20000, tol=tol)
+    A = S.Matrix(c.row, c.col, c.dia, c.val)
+    x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    r = torch.from_numpy(c.b.copy()).cuda()
+    steps, inner, ini, fin = S.solve_mixed(A, x, r, prec, 50, tol, inner_rtol=1e-4)
+    assert A.status == S.TFS_OK and fin < tol, (steps, inner, fin, tol)
+    assert 1 <= steps <= 10
+    assert abs(ini - np.linalg.norm(c.b)) <= 1e-12 * np.linalg.norm(c.b)
+    assert rel_l2(x.cpu().numpy(), ref["x"]) <= 1e-8
+    # the residual left in r1 is the true FP64 residual
+    true_r = torch.from_numpy(c.b.copy()).cuda()
+    A.residual(x, true_r)
+    assert float(torch.linalg.norm(true_r - r)) <= 1e-12 * float(np.linalg.norm(c.b))
+
+
+@pytest.mark.parametrize("name", ["poisson_stretched20", "momentum12"])
+def test_fp32_factor_is_the_reference_recurrence(torch, S, oracle, name):
+    c = mesh.make_case(**CASES[name])
+    A = S.Matrix(c.row, c.col, c.dia, c.val)
+    d32 = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    A.prec_diag32(2, d32)
+    d64 = oracle.prec_form(c.row, c.col, c.dia, c.val, 2)
+    np.testing.assert_allclose(d32.cpu().numpy(), d64, rtol=1e-5)
+    # and it is really FP32: every entry is representable in float
+    v = d32.cpu().numpy()
+    assert np.array_equal(v.astype(np.float32).astype(np.float64), v)
+
+
+def test_refinement_cap_and_early_exit(torch, S):
+    c = mesh.make_case(**CASES["poisson_perm"])
+    A = S.Matrix(c.row, c.col, c.dia, c.val)
+    x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    r = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    steps, inner, ini, fin = S.solve_mixed(A, x, r, 2, 10, 1e-8)
+    assert steps == 0 and inner == 0 and fin == 0.0
+    x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    r = torch.from_numpy(c.b.copy()).cuda()
+    steps, inner, ini, fin = S.solve_mixed(A, x, r, 2, 1, 1e-14 * float(np.linalg.norm(c.b)))
+    assert steps == 2 and A.status == S.TFS_NOT_CONVERGED
