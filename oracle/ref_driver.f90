@@ -177,9 +177,9 @@
   tfs_ref_topology = nnz
   if(nnz > nnz_cap) return
   row = M % row
-  col = M % col(1:nnz)
+  col(1:nnz) = M % col(1:nnz)
   dia = M % dia
-  mir = M % mir(1:nnz)
+  mir(1:nnz) = M % mir(1:nnz)
   pos = M % pos
   end function
 
