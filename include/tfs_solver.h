@@ -111,6 +111,32 @@ int tfs_solve_mixed_device(tfs_matrix *A, int prec, double *d_x, double *d_r1, i
                            int *inner_iters, double tol, double inner_rtol, double *ini_res,
                            double *fin_res);
 
+/* ---- batched solves ----------------------------------------------------------------
+ * Not in the reference (its callers solve one system at a time): many
+ * independent systems of uneven size, e.g. ensemble members or Divide
+ * subdomain blocks, solved together on one GPU.  Internally one
+ * block-diagonal matrix; each system keeps its own scalars, tolerance, cap
+ * and convergence, and one iteration of all systems is replayed from a
+ * hipGraph.  System s: n[s] rows, nnz[s] entries, its own CRS arrays. */
+typedef struct tfs_batch tfs_batch;
+int tfs_batch_create(int nsys, const int *n, const long long *nnz, const int *const *row,
+                     const int *const *col, const int *const *dia, int base, tfs_batch **out);
+int tfs_batch_destroy(tfs_batch *B);
+/* val of system `sys` (host memory, nnz[sys] doubles). */
+int tfs_batch_set_values(tfs_batch *B, int sys, const double *val);
+/* Cg semantics per system (x[s], r1[s]: device arrays of n[s] doubles); the
+ * per-system outputs follow tfs_solve; returns the worst status. */
+int tfs_batch_solve_device(tfs_batch *B, int prec, double *const *d_x, double *const *d_r1,
+                           const int *cap, const double *tol, int *niter, double *ini_res,
+                           double *fin_res, int *status);
+/* FP32-inner refinement per system (tfs_solve_mixed_device semantics). */
+int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *const *d_x,
+                                 double *const *d_r1, int cap, const double *tol,
+                                 double inner_rtol, int *steps, int *inner_iters,
+                                 double *ini_res, double *fin_res, int *status);
+int tfs_batch_get_info(const tfs_batch *B, int *nsys, int *n_total, long long *nnz_total,
+                       int *graph_launches);
+
 /* ---- building blocks on device vectors ---------------------------------------- */
 /* y := A x  (row sums in the reference order, Cg.f90:128-134). */
 int tfs_spmv_device(tfs_matrix *A, const double *d_x, double *d_y);

@@ -257,6 +257,143 @@ k_add_correction(int n, double *__restrict__ x, const float *__restrict__ e) {
 }
 
 // ---------------------------------------------------------------------------
+// Batched (block-diagonal) solves: many independent systems concatenated in
+// one matrix.  Vector work is split into tiles -- contiguous S ranges inside
+// one system -- one workgroup per tile; every system has its own KState, and
+// a tile of a converged system does nothing.  Per-tile partial sums are
+// reduced per system, in tile order, by k_finish_seg (one workgroup per
+// system).  `all` is the batch-wide state: its `done` is set when every
+// system is done, so whole launches return at once after that.
+struct Tiles {
+  const int *__restrict__ sys;
+  const int *__restrict__ beg;
+  const int *__restrict__ end;
+};
+
+#define TFS_TILE_LOOP(i, T_) \
+  for (int i = (T_).beg[blockIdx.x] + threadIdx.x; i < (T_).end[blockIdx.x]; i += kRedThreads)
+
+__device__ __forceinline__ bool tile_skip(const Tiles &tl, const KState *st, const KState *all) {
+  if (solve_done(all)) return true;
+  return __builtin_amdgcn_readfirstlane(st[tl.sys[blockIdx.x]].done) != 0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_sumsq_seg(Tiles tl, const T *__restrict__ v, int epi, const KState *st, const KState *all,
+            double *partials) {
+  if (epi != EPI_BNRM && tile_skip(tl, st, all)) return;
+  double s = 0.0;
+  TFS_TILE_LOOP(i, tl) {
+    const double a = v[i];
+    s += a * a;
+  }
+  store_partial<kRedThreads>(s, partials);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_rho_seg(Tiles tl, const T *__restrict__ r, T *__restrict__ z, const T *__restrict__ d,
+          const T *__restrict__ zU, const int *__restrict__ s_to_u, int mode, const KState *st,
+          const KState *all, double *partials) {
+  if (tile_skip(tl, st, all)) return;
+  double s = 0.0;
+  TFS_TILE_LOOP(i, tl) {
+    const T ri = r[i];
+    T zi;
+    if (mode == 0) {
+      zi = ri;
+    } else if (mode == 1) {
+      zi = ri / d[i];
+      z[i] = zi;
+    } else {
+      zi = zU[s_to_u[i]];
+      z[i] = zi;
+    }
+    s += (double)ri * (double)zi;
+  }
+  store_partial<kRedThreads>(s, partials);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_pupdate_seg(Tiles tl, const T *__restrict__ z, T *__restrict__ p, const KState *st,
+              const KState *all) {
+  if (tile_skip(tl, st, all)) return;
+  const KState &S = st[tl.sys[blockIdx.x]];
+  const bool first = S.iter == 0;
+  const T beta = (T)S.beta;
+  TFS_TILE_LOOP(i, tl) p[i] = first ? z[i] : z[i] + beta * p[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_dot_seg(Tiles tl, const T *__restrict__ a, const T *__restrict__ b, const KState *st,
+          const KState *all, double *partials) {
+  if (tile_skip(tl, st, all)) return;
+  double s = 0.0;
+  TFS_TILE_LOOP(i, tl) s += (double)a[i] * (double)b[i];
+  store_partial<kRedThreads>(s, partials);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_axpy_norm_seg(Tiles tl, T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
+                const T *__restrict__ q, const KState *st, const KState *all, double *partials) {
+  if (tile_skip(tl, st, all)) return;
+  const T alpha = (T)st[tl.sys[blockIdx.x]].alpha;
+  double s = 0.0;
+  TFS_TILE_LOOP(i, tl) {
+    x[i] = x[i] + alpha * p[i];
+    const T ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    s += (double)ri * (double)ri;
+  }
+  store_partial<kRedThreads>(s, partials);
+}
+
+// out = in on the tiles of systems still running (the batched residual is
+// formed in a scratch vector and kept only where the solve continues).
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_select_seg(Tiles tl, const T *__restrict__ in, T *__restrict__ out, const KState *st,
+             const KState *all) {
+  if (tile_skip(tl, st, all)) return;
+  TFS_TILE_LOOP(i, tl) out[i] = in[i];
+}
+
+// One workgroup per system: fixed-order sum of the system's tile partials,
+// the state transition, and the batch-wide done count.
+__global__ void __launch_bounds__(kRedThreads)
+k_finish_seg(const int *__restrict__ tile_first, const double *__restrict__ partials, int epi,
+             KState *st, KState *all) {
+  if (epi != EPI_BNRM && solve_done(all)) return;
+  KState *S = st + blockIdx.x;
+  if (epi != EPI_BNRM && __builtin_amdgcn_readfirstlane(S->done)) return;
+  __shared__ double lds[kRedThreads / kWave];
+  double v = 0.0;
+  for (int k = tile_first[blockIdx.x] + threadIdx.x; k < tile_first[blockIdx.x + 1];
+       k += kRedThreads)
+    v += partials[k];
+  v = block_sum<kRedThreads>(v, lds);
+  if (threadIdx.x == 0) {
+    const int was = S->done;
+    epilogue(epi, v, S);
+    if (!was && S->done) {
+      const unsigned int before = atomicAdd(&all->ctr[0], 1u);
+      if (before + 1 == (unsigned int)all->cap) all->done = 1;
+    }
+  }
+}
+
+// Iterative refinement, batched: x += e on every row (a converged system's
+// correction is zero).
+__global__ void __launch_bounds__(kRedThreads)
+k_add_correction_all(int n, double *__restrict__ x, const float *__restrict__ e) {
+  TFS_GRID_STRIDE(i, n) x[i] = x[i] + (double)e[i];
+}
+
+// ---------------------------------------------------------------------------
 // SpMV over S rows.  Modes: plain y = A x; residual y = y - A x (term by term,
 // Residual.f90:22-27); dot: y = A x and partial sum of w.y (Cg.f90:148-153,
 // Cgs.f90:157-162); cgs tail: q2 = A p1, xs += alpha p1, rs -= alpha q2 and

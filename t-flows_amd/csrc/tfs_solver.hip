@@ -739,15 +739,17 @@ extern "C" int tfs_init(int device) {
   return ensure_init();
 }
 
-extern "C" int tfs_matrix_create(int n, long long nnz, const int *row, const int *col,
-                                 const int *dia, int base, tfs_matrix **out) {
+namespace {
+
+// Validates and rebases the structure (Matrix_Mod.f90:15-34 layout) before
+// touching the device, so a bad structure is rejected without a GPU; then
+// builds the layouts and uploads them.  `keep` receives the host layout.
+int matrix_create(int n, long long nnz, const int *row, const int *col, const int *dia, int base,
+                  tfs_matrix **out, Layout *keep) {
   if (!out) return TFS_ERR_INVALID;
   *out = nullptr;
   if (n < 0 || nnz < n || nnz > 0x7ffffffeLL || (base != 0 && base != 1)) return TFS_ERR_INVALID;
   if (n > 0 && (!row || !col || !dia)) return TFS_ERR_INVALID;
-
-  // Validate and rebase the structure (Matrix_Mod.f90:15-34 layout) before
-  // touching the device, so a bad structure is rejected without a GPU.
   std::vector<int> hr(n + 1), hc((size_t)nnz), hd(n);
   for (int i = 0; i <= n; ++i) hr[i] = row[i] - base;
   if (hr[0] != 0 || hr[n] != nnz) return TFS_ERR_INVALID;
@@ -765,7 +767,8 @@ extern "C" int tfs_matrix_create(int n, long long nnz, const int *row, const int
   }
   TFS_TRY(ensure_init());
 
-  Layout Ly;
+  Layout local;
+  Layout &Ly = keep ? *keep : local;
   build_layout(n, hr.data(), hc.data(), hd.data(), kSpmvRows, kSpmvCap, kSmallLevel, Ly);
   tfs_matrix *A = new tfs_matrix;
   A->n = n;
@@ -777,6 +780,13 @@ extern "C" int tfs_matrix_create(int n, long long nnz, const int *row, const int
   }
   *out = A;
   return TFS_OK;
+}
+
+}  // namespace
+
+extern "C" int tfs_matrix_create(int n, long long nnz, const int *row, const int *col,
+                                 const int *dia, int base, tfs_matrix **out) {
+  return matrix_create(n, nnz, row, col, dia, base, out, nullptr);
 }
 
 extern "C" int tfs_matrix_destroy(tfs_matrix *A) {
@@ -991,5 +1001,495 @@ extern "C" int tfs_timing_stop(double *ms, long long *launches) {
   g_tm.on = false;
   if (ms) *ms = g_tm.ms;
   if (launches) *launches = g_tm.launches;
+  return TFS_OK;
+}
+
+// ============================================================================
+// Batched solves: many independent systems of uneven size in one
+// block-diagonal matrix.  The layouts, SpMV and sweeps run unchanged on the
+// combined matrix (blocks never couple, so the combined DIC factor is the
+// per-system factors); vector work and reductions go per system through
+// tiles, and each system has its own device state.  One iteration of all
+// systems is captured once into a hipGraph and replayed.
+
+struct tfs_batch {
+  int nsys = 0;
+  tfs_matrix *A = nullptr;
+  std::vector<int> n_of, row_off;
+  std::vector<long long> nnz_of, nnz_off;
+  int ntiles = 0;
+  int *tile_sys = nullptr, *tile_beg = nullptr, *tile_end = nullptr, *tile_first = nullptr;
+  // states: [0, nsys) inner, [nsys, 2 nsys) outer, 2 nsys inner-all, 2 nsys + 1 outer-all
+  KState *d_st = nullptr;
+  KState *h_st = nullptr;
+  double *partials = nullptr;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // [0] FP64 iteration, [1] FP32
+  int graph_prec[2] = {-1, -1};
+  bool use_graph = true;
+  int graph_launches = 0;
+};
+
+namespace {
+
+constexpr int kTileRows = 2048;
+constexpr int kGraphIters = 4;  // iterations per captured graph
+
+int batch_alloc(tfs_batch *B, void **p, size_t bytes) { return dev_alloc(B->A, p, bytes); }
+
+Tiles tiles_of(const tfs_batch *B) { return Tiles{B->tile_sys, B->tile_beg, B->tile_end}; }
+
+void finish_seg(tfs_batch *B, int epi, KState *st, KState *all) {
+  hipLaunchKernelGGL(k_finish_seg, dim3(B->nsys), dim3(kRedThreads), 0, g_lib.stream,
+                     B->tile_first, B->partials, epi, st, all);
+}
+
+// One CG iteration of every system (Cg.f90:97-178); converged systems idle.
+template <typename T>
+void batch_cg_iteration(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KState *all) {
+  tfs_matrix *A = B->A;
+  const Tiles tl = tiles_of(B);
+  const int nt = B->ntiles;
+  if (prec == 2) dic_apply<T>(A, V.r, V.yL, V.zU, all);
+  const int mode = prec == 2 ? 2 : prec == 1 ? 1 : 0;
+  T *z = prec == 0 ? V.r : V.z;
+  hipLaunchKernelGGL(k_rho_seg<T>, dim3(nt), dim3(kRedThreads), 0, g_lib.stream, tl, V.r, V.z,
+                     vals<T>(A).s_diag, V.zU, A->s_to_u, mode, st, all, B->partials);
+  finish_seg(B, EPI_RHO, st, all);
+  hipLaunchKernelGGL(k_pupdate_seg<T>, dim3(nt), dim3(kRedThreads), 0, g_lib.stream, tl, z, V.p,
+                     st, all);
+  launch_spmv<T, SPMV_PLAIN>(A, V.p, V.q, nullptr, nullptr, nullptr, all);
+  hipLaunchKernelGGL(k_dot_seg<T>, dim3(nt), dim3(kRedThreads), 0, g_lib.stream, tl, V.p, V.q, st,
+                     all, B->partials);
+  finish_seg(B, EPI_ALPHA, st, all);
+  hipLaunchKernelGGL(k_axpy_norm_seg<T>, dim3(nt), dim3(kRedThreads), 0, g_lib.stream, tl, V.x,
+                     V.r, V.p, V.q, st, all, B->partials);
+  finish_seg(B, EPI_NORM, st, all);
+}
+
+// Captures kGraphIters batched iterations into a graph (once per precision
+// and preconditioner).  If the capture fails -- e.g. a cooperative launch is
+// not capturable -- it is retried with per-level sweeps, and without a graph
+// as a last resort.
+template <typename T>
+int batch_graph(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KState *all,
+                hipGraphExec_t *exec) {
+  const int slot = sizeof(T) == 8 ? 0 : 1;
+  *exec = nullptr;
+  if (!B->use_graph) return 0;
+  if (B->graph[slot] && B->graph_prec[slot] == prec) {
+    *exec = B->graph[slot];
+    return 0;
+  }
+  if (B->graph[slot]) {
+    TFS_HIP(hipGraphExecDestroy(B->graph[slot]));
+    B->graph[slot] = nullptr;
+  }
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    const bool was_persistent = B->A->persistent;
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(g_lib.stream, hipStreamCaptureModeRelaxed) != hipSuccess) break;
+    for (int k = 0; k < kGraphIters; ++k) batch_cg_iteration<T>(B, prec, V, st, all);
+    const hipError_t e1 = hipStreamEndCapture(g_lib.stream, &g);
+    hipError_t e2 = hipErrorUnknown;
+    if (e1 == hipSuccess && g) e2 = hipGraphInstantiate(&B->graph[slot], g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    if (e1 == hipSuccess && e2 == hipSuccess && B->A->persistent == was_persistent) {
+      B->graph_prec[slot] = prec;
+      *exec = B->graph[slot];
+      return 0;
+    }
+    if (B->graph[slot]) {
+      (void)hipGraphExecDestroy(B->graph[slot]);
+      B->graph[slot] = nullptr;
+    }
+    B->A->persistent = false;  // retry with per-level sweeps
+  }
+  std::fprintf(stderr, "tfs: batch iteration not capturable; launching kernels directly\n");
+  B->use_graph = false;
+  return 0;
+}
+
+int batch_read(tfs_batch *B) {
+  TFS_HIP(hipMemcpyAsync(B->h_st, B->d_st, sizeof(KState) * (2 * B->nsys + 2),
+                         hipMemcpyDeviceToHost, g_lib.stream));
+  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  return 0;
+}
+
+int batch_write(tfs_batch *B) {
+  TFS_HIP(hipMemcpyAsync(B->d_st, B->h_st, sizeof(KState) * (2 * B->nsys + 2),
+                         hipMemcpyHostToDevice, g_lib.stream));
+  return 0;
+}
+
+// Replays iterations until every system is done (states read between chunks).
+template <typename T>
+int batch_iterate(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KState *all, int cap) {
+  hipGraphExec_t exec = nullptr;
+  TFS_TRY(batch_graph<T>(B, prec, V, st, all, &exec));
+  const int step = exec ? kGraphIters : 1;
+  const KState *ha = B->h_st + (all - B->d_st);
+  int queued = 0, chunk = step;
+  std::vector<int> it_prev(B->nsys, 0);
+  std::vector<double> err_prev(B->nsys, 0.0);
+  const KState *hs = B->h_st + (st - B->d_st);
+  for (int s = 0; s < B->nsys; ++s) err_prev[s] = hs[s].err;
+  while (!ha->done) {
+    for (int k = 0; k < chunk; k += step) {
+      if (exec) {
+        TFS_HIP(hipGraphLaunch(exec, g_lib.stream));
+        B->graph_launches++;
+      } else {
+        batch_cg_iteration<T>(B, prec, V, st, all);
+      }
+    }
+    queued += chunk;
+    TFS_HIP(hipGetLastError());
+    TFS_TRY(batch_read(B));
+    if (ha->done) break;
+    if (queued > cap + 64 + step) return TFS_ERR_DEVICE;
+    // the slowest unfinished system sets the next chunk
+    int next = 1;
+    for (int s = 0; s < B->nsys; ++s) {
+      if (hs[s].done) continue;
+      int need = 16;
+      const int dit = hs[s].iter - it_prev[s];
+      if (dit > 0 && hs[s].err > 0 && err_prev[s] > 0 && hs[s].err < err_prev[s]) {
+        const double rate = std::pow(hs[s].err / err_prev[s], 1.0 / dit);
+        const double it_need = std::log(hs[s].tol / hs[s].err) / std::log(rate);
+        need = it_need < 1.0 ? 1 : (int)(0.9 * it_need);
+      }
+      next = std::max(next, need);
+      it_prev[s] = hs[s].iter;
+      err_prev[s] = hs[s].err;
+    }
+    chunk = std::max(step, std::min(32, next) / step * step);
+  }
+  return 0;
+}
+
+// caller per-system vectors <-> combined S-order vector
+int batch_in(tfs_batch *B, double *const *src, double *stage, double *dstS) {
+  for (int s = 0; s < B->nsys; ++s)
+    if (B->n_of[s])
+      TFS_HIP(hipMemcpyAsync(stage + B->row_off[s], src[s], sizeof(double) * B->n_of[s],
+                             hipMemcpyDeviceToDevice, g_lib.stream));
+  gather(B->A->n, stage, B->A->s_orig, dstS);
+  return 0;
+}
+int batch_out(tfs_batch *B, const double *srcS, double *stage, double *const *dst) {
+  scatter(B->A->n, srcS, B->A->s_orig, stage);
+  for (int s = 0; s < B->nsys; ++s)
+    if (B->n_of[s])
+      TFS_HIP(hipMemcpyAsync(dst[s], stage + B->row_off[s], sizeof(double) * B->n_of[s],
+                             hipMemcpyDeviceToDevice, g_lib.stream));
+  return 0;
+}
+
+void batch_init_states(tfs_batch *B, int base, const int *cap, const double *tol) {
+  for (int s = 0; s < B->nsys; ++s) {
+    KState &h = B->h_st[base + s];
+    std::memset(&h, 0, sizeof(KState));
+    h.tol = tol[s];
+    h.cap = cap[s];
+  }
+  KState &all = B->h_st[2 * B->nsys + (base ? 1 : 0)];
+  std::memset(&all, 0, sizeof(KState));
+  all.cap = B->nsys;
+}
+
+}  // namespace
+
+extern "C" int tfs_batch_create(int nsys, const int *n, const long long *nnz,
+                                const int *const *row, const int *const *col,
+                                const int *const *dia, int base, tfs_batch **out) {
+  if (!out || nsys < 1 || !n || !nnz || !row || !col || !dia || (base != 0 && base != 1))
+    return TFS_ERR_INVALID;
+  *out = nullptr;
+  // the combined block-diagonal structure (0-based)
+  long long N = 0, NNZ = 0;
+  for (int s = 0; s < nsys; ++s) {
+    if (n[s] < 1 || nnz[s] < n[s]) return TFS_ERR_INVALID;
+    N += n[s];
+    NNZ += nnz[s];
+  }
+  if (N > 0x7ffffffeLL || NNZ > 0x7ffffffeLL) return TFS_ERR_INVALID;
+  std::vector<int> cr(N + 1), cc((size_t)NNZ), cd((size_t)N);
+  auto B = new tfs_batch;
+  B->nsys = nsys;
+  long long ro = 0, eo = 0;
+  for (int s = 0; s < nsys; ++s) {
+    B->n_of.push_back(n[s]);
+    B->row_off.push_back((int)ro);
+    B->nnz_of.push_back(nnz[s]);
+    B->nnz_off.push_back(eo);
+    if (row[s][0] - base != 0 || row[s][n[s]] - base != nnz[s]) {
+      delete B;
+      return TFS_ERR_INVALID;
+    }
+    for (int i = 0; i < n[s]; ++i) {
+      cr[ro + i] = (int)(eo + row[s][i] - base);
+      cd[ro + i] = (int)(eo + dia[s][i] - base);
+    }
+    for (long long k = 0; k < nnz[s]; ++k) {
+      const int c = col[s][k] - base;
+      if (c < 0 || c >= n[s]) {
+        delete B;
+        return TFS_ERR_INVALID;
+      }
+      cc[(size_t)(eo + k)] = (int)(ro + c);
+    }
+    ro += n[s];
+    eo += nnz[s];
+  }
+  cr[N] = (int)NNZ;
+  Layout Ly;
+  int rc = matrix_create((int)N, NNZ, cr.data(), cc.data(), cd.data(), 0, &B->A, &Ly);
+  if (rc < 0) {
+    delete B;
+    return rc;
+  }
+  // every system occupies one contiguous S range (a connected block is laid
+  // out contiguously by RCM; several components of one system stay adjacent)
+  std::vector<int> tsys, tbeg, tend, tfirst;
+  for (int s = 0; s < nsys; ++s) {
+    int lo = (int)N, hi = -1;
+    for (int i = 0; i < n[s]; ++i) {
+      const int k = Ly.s_of_orig[B->row_off[s] + i];
+      lo = std::min(lo, k);
+      hi = std::max(hi, k);
+    }
+    if (hi - lo + 1 != n[s]) {
+      tfs_batch_destroy(B);
+      return TFS_ERR_INVALID;
+    }
+    tfirst.push_back((int)tsys.size());
+    for (int b = lo; b <= hi; b += kTileRows) {
+      tsys.push_back(s);
+      tbeg.push_back(b);
+      tend.push_back(std::min(hi + 1, b + kTileRows));
+    }
+  }
+  tfirst.push_back((int)tsys.size());
+  B->ntiles = (int)tsys.size();
+  tfs_matrix *A = B->A;
+  if ((rc = upload(A, &B->tile_sys, tsys)) < 0 || (rc = upload(A, &B->tile_beg, tbeg)) < 0 ||
+      (rc = upload(A, &B->tile_end, tend)) < 0 || (rc = upload(A, &B->tile_first, tfirst)) < 0 ||
+      (rc = batch_alloc(B, (void **)&B->d_st, sizeof(KState) * (2 * nsys + 2))) < 0 ||
+      (rc = batch_alloc(B, (void **)&B->partials, sizeof(double) * std::max(1, B->ntiles))) < 0) {
+    tfs_batch_destroy(B);
+    return rc;
+  }
+  if (hipHostMalloc((void **)&B->h_st, sizeof(KState) * (2 * nsys + 2), hipHostMallocDefault) !=
+      hipSuccess) {
+    tfs_batch_destroy(B);
+    return TFS_ERR_NOMEM;
+  }
+  *out = B;
+  return TFS_OK;
+}
+
+extern "C" int tfs_batch_destroy(tfs_batch *B) {
+  if (!B) return TFS_OK;
+  if (g_lib.stream) (void)hipStreamSynchronize(g_lib.stream);
+  for (auto &g : B->graph)
+    if (g) (void)hipGraphExecDestroy(g);
+  if (B->h_st) (void)hipHostFree(B->h_st);
+  tfs_matrix_destroy(B->A);
+  delete B;
+  return TFS_OK;
+}
+
+extern "C" int tfs_batch_set_values(tfs_batch *B, int sys, const double *val) {
+  if (!B || sys < 0 || sys >= B->nsys || !val) return TFS_ERR_INVALID;
+  TFS_HIP(hipMemcpyAsync(B->A->val + B->nnz_off[sys], val, sizeof(double) * B->nnz_of[sys],
+                         hipMemcpyHostToDevice, g_lib.stream));
+  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  values_changed(B->A);
+  return TFS_OK;
+}
+
+extern "C" int tfs_batch_solve_device(tfs_batch *B, int prec, double *const *d_x,
+                                      double *const *d_r1, const int *cap, const double *tol,
+                                      int *niter, double *ini_res, double *fin_res,
+                                      int *status) {
+  if (!B || !d_x || !d_r1 || !cap || !tol || !niter || !ini_res || !fin_res || !status)
+    return TFS_ERR_INVALID;
+  if (prec < 0 || prec > 2) prec = 0;
+  tfs_matrix *A = B->A;
+  const int n = A->n, ns = B->nsys;
+  TFS_TRY(ensure_vec64(A, 7));
+  TFS_TRY(prec_form<double>(A, prec));
+  const Vecs<double> V = vecs_of(A->vec64, n);
+  KState *st = B->d_st, *all = B->d_st + 2 * ns;
+  batch_init_states(B, 0, cap, tol);
+  for (int s = 0; s < ns; ++s) B->h_st[s].ini_res = ini_res[s];
+  TFS_TRY(batch_write(B));
+  TFS_TRY(batch_in(B, d_x, A->stage_x, V.x));
+  TFS_TRY(batch_in(B, d_r1, A->stage_r, V.r));
+  const Tiles tl = tiles_of(B);
+  // ||b|| and its early exit (per system); r = b - A x where not exited;
+  // ||r|| and its early exit
+  hipLaunchKernelGGL(k_sumsq_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0, g_lib.stream, tl,
+                     V.r, (int)EPI_BNRM, st, all, B->partials);
+  finish_seg(B, EPI_BNRM, st, all);
+  gather(n, V.r, (const int *)nullptr, V.q);
+  launch_spmv<double, SPMV_RESID>(A, V.x, V.q, nullptr, nullptr, nullptr, all);
+  hipLaunchKernelGGL(k_select_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0, g_lib.stream,
+                     tl, V.q, V.r, st, all);  // r = q for systems still running
+  hipLaunchKernelGGL(k_sumsq_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0, g_lib.stream, tl,
+                     V.r, (int)EPI_INIRES, st, all, B->partials);
+  finish_seg(B, EPI_INIRES, st, all);
+  TFS_HIP(hipGetLastError());
+  TFS_TRY(batch_read(B));
+  int maxcap = 1;
+  for (int s = 0; s < ns; ++s) {
+    maxcap = std::max(maxcap, cap[s]);
+    KState &h = B->h_st[s];
+    if (!h.done && cap[s] < 1) {  // zero-trip loop (Cg.f90:97,181)
+      h.done = 1;
+      h.niter_out = 1;
+      h.fin_res = h.err;
+      h.status = kStNotConverged;
+      B->h_st[2 * ns].ctr[0]++;
+    }
+  }
+  if (B->h_st[2 * ns].ctr[0] == (unsigned)ns) B->h_st[2 * ns].done = 1;
+  TFS_TRY(batch_write(B));
+  TFS_TRY(batch_iterate<double>(B, prec, V, st, all, maxcap));
+  TFS_TRY(batch_out(B, V.x, A->stage_x, d_x));
+  TFS_TRY(batch_out(B, V.r, A->stage_r, d_r1));
+  TFS_HIP(hipGetLastError());
+  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_TRY(check_sweep(A));
+  int worst = TFS_OK;
+  for (int s = 0; s < ns; ++s) {
+    const KState &h = B->h_st[s];
+    niter[s] = h.niter_out;
+    fin_res[s] = h.fin_res;
+    ini_res[s] = h.ini_res;
+    status[s] = status_of(&h);
+    if (status[s] == TFS_ERR_BREAKDOWN || (status[s] == TFS_NOT_CONVERGED && worst == TFS_OK))
+      worst = status[s];
+  }
+  return worst;
+}
+
+// FP32-inner PCG in FP64 iterative refinement for every system at once
+// (see solve_mixed_device); steps / inner iterations are per system.
+extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *const *d_x,
+                                            double *const *d_r1, int cap, const double *tol,
+                                            double inner_rtol, int *steps, int *inner_iters,
+                                            double *ini_res, double *fin_res, int *status) {
+  if (!B || !d_x || !d_r1 || !tol || !steps || !inner_iters || !ini_res || !fin_res || !status)
+    return TFS_ERR_INVALID;
+  if (prec < 0 || prec > 2) prec = 0;
+  inner_rtol = std::min(0.5, std::max(1e-6, inner_rtol));
+  tfs_matrix *A = B->A;
+  const int n = A->n, ns = B->nsys;
+  TFS_TRY(ensure_vec64(A, 7));
+  TFS_TRY(ensure_vec32(A, 7));
+  TFS_TRY(prepare_values<double>(A));
+  TFS_TRY(prec_form<float>(A, prec));
+  const Vecs<double> D = vecs_of(A->vec64, n);
+  const Vecs<float> F = vecs_of(A->vec32, n);
+  KState *si = B->d_st, *so = B->d_st + ns, *alli = B->d_st + 2 * ns, *allo = alli + 1;
+  KState *hi = B->h_st, *ho = B->h_st + ns;
+  const Tiles tl = tiles_of(B);
+  const int g = vgrid(n);
+  TFS_TRY(batch_in(B, d_x, A->stage_x, D.x));
+  TFS_TRY(batch_in(B, d_r1, A->stage_r, D.z));  // b
+  std::vector<int> caps(ns, cap), stalls(ns, 0), outer(ns, 0), st_out(ns, TFS_OK);
+  std::vector<double> err(ns), err0(ns);
+  for (int s = 0; s < ns; ++s) inner_iters[s] = 0;
+  auto outer_residual = [&]() -> int {
+    batch_init_states(B, ns, caps.data(), tol);
+    TFS_TRY(batch_write(B));
+    gather(n, D.z, (const int *)nullptr, D.r);
+    launch_spmv<double, SPMV_RESID>(A, D.x, D.r, nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_sumsq_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0, g_lib.stream,
+                       tl, D.r, (int)EPI_BNRM, so, allo, B->partials);
+    finish_seg(B, EPI_BNRM, so, allo);
+    TFS_HIP(hipGetLastError());
+    return batch_read(B);
+  };
+  TFS_TRY(outer_residual());
+  for (int s = 0; s < ns; ++s) err[s] = err0[s] = ho[s].bnrm2;
+  std::vector<char> active(ns, 0);
+  for (;;) {
+    int nact = 0;
+    for (int s = 0; s < ns; ++s) {
+      active[s] = err[s] >= tol[s] && st_out[s] == TFS_OK;
+      if (active[s] && outer[s] >= cap) {
+        st_out[s] = TFS_NOT_CONVERGED;
+        active[s] = 0;
+      }
+      nact += active[s];
+    }
+    if (!nact) break;
+    // inner FP32 solves of the active systems; the others are done at once
+    std::vector<int> icap(ns, 100000);
+    std::vector<double> itol(ns);
+    for (int s = 0; s < ns; ++s) itol[s] = std::max(inner_rtol * err[s], 0.5 * tol[s]);
+    batch_init_states(B, 0, icap.data(), itol.data());
+    int ndone = 0;
+    for (int s = 0; s < ns; ++s)
+      if (!active[s]) {
+        hi[s].done = 1;
+        ++ndone;
+      }
+    hi[2 * ns].ctr[0] = (unsigned)ndone;
+    TFS_TRY(batch_write(B));
+    gather(n, D.r, (const int *)nullptr, F.r);
+    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, F.x, 0.0f);
+    hipLaunchKernelGGL(k_sumsq_seg<float>, dim3(B->ntiles), dim3(kRedThreads), 0, g_lib.stream,
+                       tl, F.r, (int)EPI_INIRES, si, alli, B->partials);
+    finish_seg(B, EPI_INIRES, si, alli);
+    TFS_HIP(hipGetLastError());
+    TFS_TRY(batch_read(B));
+    TFS_TRY(batch_iterate<float>(B, prec, F, si, alli, 100000));
+    for (int s = 0; s < ns; ++s)
+      if (active[s]) {
+        ++outer[s];
+        inner_iters[s] += std::min(hi[s].niter_out, 100000);
+        if (hi[s].status == kStBreakdown) st_out[s] = TFS_ERR_BREAKDOWN;
+      }
+    hipLaunchKernelGGL(k_add_correction_all, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, D.x,
+                       F.x);
+    TFS_TRY(outer_residual());
+    for (int s = 0; s < ns; ++s) {
+      if (!active[s]) continue;
+      const double e = ho[s].bnrm2;
+      if (!std::isfinite(e)) st_out[s] = TFS_ERR_BREAKDOWN;
+      stalls[s] = e > 0.9 * err[s] ? stalls[s] + 1 : 0;
+      if (stalls[s] >= 3 && e >= tol[s]) st_out[s] = TFS_NOT_CONVERGED;
+      err[s] = e;
+    }
+  }
+  TFS_TRY(batch_out(B, D.x, A->stage_x, d_x));
+  TFS_TRY(batch_out(B, D.r, A->stage_r, d_r1));
+  TFS_HIP(hipGetLastError());
+  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_TRY(check_sweep(A));
+  int worst = TFS_OK;
+  for (int s = 0; s < ns; ++s) {
+    ini_res[s] = err0[s];
+    fin_res[s] = err[s];
+    status[s] = st_out[s];
+    steps[s] = st_out[s] == TFS_NOT_CONVERGED ? cap + 1 : outer[s];
+    if (status[s] == TFS_ERR_BREAKDOWN || (status[s] == TFS_NOT_CONVERGED && worst == TFS_OK))
+      worst = status[s];
+  }
+  return worst;
+}
+
+extern "C" int tfs_batch_get_info(const tfs_batch *B, int *nsys, int *n_total,
+                                  long long *nnz_total, int *graph_launches) {
+  if (!B) return TFS_ERR_INVALID;
+  if (nsys) *nsys = B->nsys;
+  if (n_total) *n_total = B->A->n;
+  if (nnz_total) *nnz_total = B->A->nnz;
+  if (graph_launches) *graph_launches = B->graph_launches;
   return TFS_OK;
 }

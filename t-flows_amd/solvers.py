@@ -304,3 +304,109 @@ class KernelTimer:
         _lib().tfs_timing_stop(ctypes.byref(ms), ctypes.byref(nl))
         self.ms, self.launches = ms.value, nl.value
         return False
+
+
+class Batch:
+    """Many independent systems solved together (tfs_batch_*)."""
+
+    def __init__(self, systems, base=0):
+        """systems: list of (row, col, dia, val) in the reference CRS layout."""
+        lib = _lib()
+        if not getattr(lib, "_tfs_batch_typed", False):
+            PP = ctypes.POINTER(ctypes.c_void_p)
+            lib.tfs_batch_create.argtypes = [ctypes.c_int, _VP, _VP, PP, PP, PP, ctypes.c_int,
+                                             ctypes.POINTER(_VP)]
+            lib.tfs_batch_destroy.argtypes = [_VP]
+            lib.tfs_batch_set_values.argtypes = [_VP, ctypes.c_int, _F64P]
+            lib.tfs_batch_solve_device.argtypes = [_VP, ctypes.c_int, PP, PP, _VP, _VP, _VP, _VP,
+                                                   _VP, _VP]
+            lib.tfs_batch_solve_mixed_device.argtypes = [_VP, ctypes.c_int, PP, PP, ctypes.c_int,
+                                                         _VP, ctypes.c_double, _VP, _VP, _VP,
+                                                         _VP, _VP]
+            lib.tfs_batch_get_info.argtypes = [_VP, _IP, _IP, ctypes.POINTER(ctypes.c_longlong),
+                                               _IP]
+            lib._tfs_batch_typed = True
+        self._lib = lib
+        self._keep = []
+        rows = [np.ascontiguousarray(s[0], np.int32) for s in systems]
+        cols = [np.ascontiguousarray(s[1], np.int32) for s in systems]
+        dias = [np.ascontiguousarray(s[2], np.int32) for s in systems]
+        self._keep += rows + cols + dias
+        self.nsys = len(systems)
+        self.n = np.array([len(r) - 1 for r in rows], np.int32)
+        self.nnz = np.array([int(r[-1]) - base for r in rows], np.int64)
+
+        def ptrs(arrs):
+            return (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        h = _VP()
+        _check(lib.tfs_batch_create(self.nsys, self.n.ctypes.data, self.nnz.ctypes.data,
+                                    ptrs(rows), ptrs(cols), ptrs(dias), base, ctypes.byref(h)),
+               "tfs_batch_create")
+        self._h = h
+        for s, sysd in enumerate(systems):
+            if len(sysd) > 3 and sysd[3] is not None:
+                self.set_values(s, sysd[3])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.tfs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_values(self, s, val):
+        val = np.ascontiguousarray(val, np.float64)
+        _check(self._lib.tfs_batch_set_values(self._h, s, val), "tfs_batch_set_values")
+
+    def info(self):
+        ns, nt, gl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        nz = ctypes.c_longlong()
+        self._lib.tfs_batch_get_info(self._h, ctypes.byref(ns), ctypes.byref(nt),
+                                     ctypes.byref(nz), ctypes.byref(gl))
+        return dict(nsys=ns.value, n_total=nt.value, nnz_total=nz.value,
+                    graph_launches=gl.value)
+
+    def _vecs(self, xs, rs):
+        for k, (x, r) in enumerate(zip(xs, rs)):
+            _dptr(x, int(self.n[k]), "x")
+            _dptr(r, int(self.n[k]), "r1")
+        px = (ctypes.c_void_p * self.nsys)(*[x.data_ptr() for x in xs])
+        pr = (ctypes.c_void_p * self.nsys)(*[r.data_ptr() for r in rs])
+        return px, pr
+
+    def solve(self, xs, rs, prec, caps, tols):
+        """Cg per system; returns arrays (niter, ini_res, fin_res, status)."""
+        px, pr = self._vecs(xs, rs)
+        caps = np.ascontiguousarray(caps, np.int32)
+        tols = np.ascontiguousarray(tols, np.float64)
+        niter = np.zeros(self.nsys, np.int32)
+        ini = np.zeros(self.nsys)
+        fin = np.zeros(self.nsys)
+        st = np.zeros(self.nsys, np.int32)
+        _sync()
+        _check(self._lib.tfs_batch_solve_device(self._h, prec, px, pr, caps.ctypes.data,
+                                                tols.ctypes.data, niter.ctypes.data,
+                                                ini.ctypes.data, fin.ctypes.data, st.ctypes.data),
+               "tfs_batch_solve_device")
+        return niter, ini, fin, st
+
+    def solve_mixed(self, xs, rs, prec, cap, tols, inner_rtol=1e-4):
+        """FP32-inner refinement per system; arrays (steps, inner, ini, fin, status)."""
+        px, pr = self._vecs(xs, rs)
+        tols = np.ascontiguousarray(tols, np.float64)
+        steps = np.zeros(self.nsys, np.int32)
+        inner = np.zeros(self.nsys, np.int32)
+        ini = np.zeros(self.nsys)
+        fin = np.zeros(self.nsys)
+        st = np.zeros(self.nsys, np.int32)
+        _sync()
+        _check(self._lib.tfs_batch_solve_mixed_device(self._h, prec, px, pr, int(cap),
+                                                      tols.ctypes.data, float(inner_rtol),
+                                                      steps.ctypes.data, inner.ctypes.data,
+                                                      ini.ctypes.data, fin.ctypes.data,
+                                                      st.ctypes.data), "tfs_batch_solve_mixed")
+        return steps, inner, ini, fin, st
