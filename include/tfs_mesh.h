@@ -77,6 +77,11 @@ int tfs_case_crs(const tfs_case *c, int base, int *row, int *col, int *dia,
 /* Face list as the reference Grid_Type holds it: faces_c[2*n_faces],
  * 1-based cells, negative c2 for boundary faces. */
 int tfs_case_faces(const tfs_case *c, int *faces_c);
+/* The assembly inputs: per face the coefficient of c2 in row c1 (a12; for a
+ * boundary face the diagonal addition of c1) and of c1 in row c2 (a21); per
+ * cell the extra diagonal (inertia).  val = scatter through pos/dia in face
+ * order, then the extra diagonal, then (Poisson, pin_first) a_11 doubled. */
+int tfs_case_face_coefs(const tfs_case *c, double *a12, double *a21, double *dextra);
 /* Time step used for the momentum inertia term (0 for Poisson). */
 double tfs_case_dt(const tfs_case *c);
 /* Largest cell aspect ratio (max edge / min edge over the cell's 12 edges). */

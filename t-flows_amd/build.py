@@ -55,6 +55,9 @@ def build_oracle():
     _run(["make", "-s", "-C", ROOT / "oracle"])
     if REFERENCE_SOURCES.is_dir():
         _run(["make", "-s", "-C", ROOT / "oracle", "ref"])
+        # the Fortran drop-in linked with the reference's own Matrix_Mod
+        # (needs libtfs_solver.so, built first by build_all)
+        _run(["make", "-s", "-C", ROOT / "oracle", "shim"])
 
 
 def build_all():

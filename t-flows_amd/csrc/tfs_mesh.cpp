@@ -386,6 +386,13 @@ extern "C" int tfs_case_faces(const tfs_case *c, int *faces_c) {
   return 0;
 }
 
+extern "C" int tfs_case_face_coefs(const tfs_case *c, double *a12, double *a21, double *dextra) {
+  std::memcpy(a12, c->a12.data(), sizeof(double) * c->a12.size());
+  std::memcpy(a21, c->a21.data(), sizeof(double) * c->a21.size());
+  std::memcpy(dextra, c->dextra.data(), sizeof(double) * c->dextra.size());
+  return 0;
+}
+
 extern "C" double tfs_case_dt(const tfs_case *c) { return c->dt; }
 extern "C" double tfs_case_max_aspect(const tfs_case *c) { return c->max_aspect; }
 

@@ -65,6 +65,8 @@ def _lib():
                                      _F64P, _F64P]
         lib.tfs_case_faces.restype = ctypes.c_int
         lib.tfs_case_faces.argtypes = [ctypes.c_void_p, _I32P]
+        lib.tfs_case_face_coefs.restype = ctypes.c_int
+        lib.tfs_case_face_coefs.argtypes = [ctypes.c_void_p, _F64P, _F64P, _F64P]
         lib.tfs_case_dt.restype = ctypes.c_double
         lib.tfs_case_dt.argtypes = [ctypes.c_void_p]
         lib.tfs_case_max_aspect.restype = ctypes.c_double
@@ -91,6 +93,7 @@ class Case:
     dt: float = 0.0
     max_aspect: float = 1.0
     faces: np.ndarray = None
+    face_coefs: tuple = None   # (a12, a21, dextra) when built with_faces
     params: dict = field(default_factory=dict)
 
     @property
@@ -128,17 +131,19 @@ def make_case(kind=POISSON, nx=8, ny=None, nz=None, beta=(0.0, 0.0, 0.0), jitter
         rc = lib.tfs_case_crs(h, base, row, col, dia, val, b)
         if rc != 0:
             raise ValueError(f"tfs_case_crs failed ({rc})")
-        faces = None
+        faces = coefs = None
         if with_faces:
             faces = np.empty(2 * nf, np.int32)
             lib.tfs_case_faces(h, faces)
+            coefs = (np.empty(nf), np.empty(nf), np.empty(n))
+            lib.tfs_case_face_coefs(h, *coefs)
         params = dict(kind=kind, nx=nx, ny=ny, nz=nz, beta=tuple(beta), jitter=jitter,
                       permute=bool(permute), pin_first=bool(pin_first),
                       rhs_zero_mean=bool(rhs_zero_mean), nu=nu, cfl=cfl, n_modes=n_modes,
                       seed=seed)
         return Case(n=n, row=row, col=col, dia=dia, val=val, b=b, base=base,
                     dt=lib.tfs_case_dt(h), max_aspect=lib.tfs_case_max_aspect(h),
-                    faces=faces, params=params)
+                    faces=faces, face_coefs=coefs, params=params)
     finally:
         lib.tfs_case_destroy(h)
 
