@@ -10,23 +10,35 @@ the Krylov iterations to tolerance.  Matrix, right-hand side and work vectors
 are resident in HBM before the timed region; each step restores r1 = b and
 x = 0 on the device (part of the timed work).
 
-Workload 2 (default): momentum matrix on a 128^3 hex mesh (tanh clustering
-beta 2, +-10% node jitter, random cell numbering, upwind convection-diffusion
-nu 1e-3, inertia V/dt), Bicg with the reference's incomplete Cholesky (prec
-2) to relative residual 1e-10, FP64.
+Workloads (sizes and seeds in t-flows_amd/cases.py):
+  1  Poisson 32^3, Cg + incomplete Cholesky (prec 2) to relative 1e-8.
+  2  (default) momentum matrix on a 128^3 hex mesh (tanh clustering beta 2,
+     +-10% node jitter, random cell numbering, upwind convection-diffusion nu
+     1e-3, inertia V/dt), Bicg + prec 2 to relative 1e-10, FP64.
+  3  Poisson 256^3 stretched to cell aspect ratio 100: FP32-inner PCG in FP64
+     iterative refinement to 1e-10 (the value), plain FP64 Cg timed beside it.
+  4  64 Poisson systems of uneven size in one block-diagonal batch, FP32+FP64
+     refinement to 1e-10; a step solves all 64.
+  5  serving: an open-loop Poisson-arrival request stream (200 req/s, 24
+     cases of 1e4..4e6 cells, 70% pressure Cg / 30% momentum Bicg, 10 visits
+     each with a perturbed RHS) through the continuous-batching server under
+     a 200 GB HBM budget; a step is one request, latency is arrival to result.
 
 One JSON line on stdout; progress goes to stderr.
 """
 import argparse
+import ctypes
 import importlib.util
 import json
 import os
 import pathlib
 import sys
+import threading
 import time
 
 ROOT = pathlib.Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0
+METRIC = "Poisson solves/s & SpMV HBM GB/s vs plain FP64 at 1e-10 resid; p50/p99 latency"
 
 
 def log(*a):
@@ -51,57 +63,114 @@ def spmv_bytes(n, nnz, val_bytes=8, idx_bytes=4):
     return (val_bytes + idx_bytes) * nnz + idx_bytes * (n + 1) + 2 * val_bytes * n
 
 
-def cpu_baseline(case, solve, tol, reps, x_gpu):
+def cpu_name():
+    return os.popen("grep -m1 'model name' /proc/cpuinfo").read().split(":")[-1].strip()
+
+
+def cpu_solver():
     """The reference's own solver (oracle/_ref, compiled from its sources with
-    its FAST flags), or the C restatement if that library is absent, timed on
-    this host's cores, 1 thread."""
-    import numpy as np
+    its FAST flags) if built, else the C restatement; both 1 thread."""
     sys.path.insert(0, str(ROOT))
     from oracle.oracle import Oracle, Reference, reference_available
-    kind = "reference" if reference_available() else "port"
-    times, out = [], None
-    if kind == "reference":
+    if reference_available():
         R = Reference()
-        R.set_matrix(case.row, case.col, case.dia, case.val)
-        for _ in range(reps):
-            out = R.solve(solve["method"], case.b, prec=solve["prec"], niter=solve["cap"], tol=tol)
-            times.append(out["seconds"])
-    else:
-        O = Oracle()
-        for _ in range(reps):
-            t0 = time.perf_counter()
-            out = O.solve(solve["method"], case.row, case.col, case.dia, case.val, case.b,
-                          prec=solve["prec"], niter=solve["cap"], tol=tol)
-            times.append(time.perf_counter() - t0)
+
+        def run(c, method, prec, cap, tol):
+            R.set_matrix(c.row, c.col, c.dia, c.val)
+            out = R.solve(method, c.b, prec=prec, niter=cap, tol=tol)
+            return out, out["seconds"]
+        return "reference", run
+    O = Oracle()
+
+    def run(c, method, prec, cap, tol):
+        t0 = time.perf_counter()
+        out = O.solve(method, c.row, c.col, c.dia, c.val, c.b, prec=prec, niter=cap, tol=tol)
+        return out, time.perf_counter() - t0
+    return "port", run
+
+
+def cpu_full_solves(case, solve, tol, reps, x_gpu):
+    import numpy as np
+    kind, run = cpu_solver()
+    times, out = [], None
+    for _ in range(reps):
+        out, sec = run(case, solve["method"], solve["prec"], solve["cap"], tol)
+        times.append(sec)
     med = float(np.median(times))
     err = float(np.linalg.norm(x_gpu - out["x"]) / np.linalg.norm(out["x"]))
-    cpu = os.popen("grep -m1 'model name' /proc/cpuinfo").read().split(":")[-1].strip()
     return dict(value=1.0 / med, unit="solves/s", cores=1, kind=kind,
                 sample=f"{reps} full solves of the same system (median {med:.3f} s), "
-                       f"{out['niter']} iterations, 1 thread on {cpu}",
+                       f"{out['niter']} iterations, 1 thread on {cpu_name()}",
                 niter=out["niter"], seconds_per_solve=med), err
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", type=int, default=2, choices=[1, 2])
-    ap.add_argument("--cpu-reps", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              f"(the solver path runs on a single MI355X)", file=sys.stderr)
-        sys.exit(2)
+def cpu_iteration_sample(case, solve, iters, gpu_niter):
+    """Seconds per iteration of the reference solver from a solve capped at
+    `iters` iterations (tol 0), scaled to the GPU solve's iteration count."""
+    kind, run = cpu_solver()
+    _, sec = run(case, solve["method"], solve["prec"], iters, 0.0)
+    per_solve = sec / (iters + 1) * gpu_niter
+    return dict(value=1.0 / per_solve, unit="solves/s", cores=1, kind=kind,
+                sample=f"one solve capped at {iters} iterations ({sec:.2f} s incl. Prec_Form), "
+                       f"scaled to {gpu_niter} iterations, 1 thread on {cpu_name()}",
+                seconds_per_solve=per_solve)
 
+
+def timed(step, steps, warmup, sync):
+    for _ in range(warmup):
+        step()
+    sync()
+    lat = []
+    t0 = time.perf_counter()
+    res = None
+    for _ in range(steps):
+        ts = time.perf_counter()
+        res = step()
+        lat.append(time.perf_counter() - ts)
+    sync()
+    return time.perf_counter() - t0, lat, res
+
+
+def spmv_roofline(A, x, y, n, nnz, reps=50, inloop_ms=None):
+    ms = A.spmv_time(x, y, reps=reps)
+    algo = spmv_bytes(n, nnz)
+    achieved = algo / (ms * 1e-3) / 1e9
+    return {
+        "kernel": "k_spmv (FP64 CSR SpMV)", "bound": "hbm", "achieved": achieved,
+        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "algorithmic_bytes": algo, "launch_us": ms * 1e3,
+        "launch_us_in_solves": None if inloop_ms is None else inloop_ms * 1e3,
+    }
+
+
+def breakdown(lib, step):
+    out = {}
+    for kc, name in ((0, "spmv"), (1, "sweeps"), (2, "factor")):
+        lib.tfs_timing_start(kc)
+        step()
+        ms, nl = ctypes.c_double(), ctypes.c_longlong()
+        lib.tfs_timing_stop(ctypes.byref(ms), ctypes.byref(nl))
+        out[name] = dict(ms_per_solve=ms.value, launches_per_solve=nl.value)
+    return out
+
+
+def line(args, value, elapsed, workload, extra):
+    out = {
+        "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded hex mesh and RHS)", "config": {"workload": workload},
+    }
+    out.update(extra)
+    return out
+
+
+def single_solve(args, k):
+    """Workloads 1 and 2: repeated full solves of one system."""
     import numpy as np
     import torch
-    load_package()
     from tflows_amd import cases, mesh, solvers
-
-    k = args.workload
     kw = cases.config_kwargs(k)
     solve = cases.SOLVE[k]
     log(f"generating workload {k}: {kw}")
@@ -110,7 +179,6 @@ def main():
     log(f"  n={c.n} nnz={c.nnz} ({time.perf_counter() - t0:.1f}s)")
     tol = solve["rtol"] * float(np.linalg.norm(c.b))
     method = solvers._METHODS[solve["method"]]
-
     A = solvers.Matrix(c.row, c.col, c.dia, c.val)
     info = A.info()
     b = torch.from_numpy(c.b).cuda()
@@ -124,95 +192,330 @@ def main():
         A.values_changed()  # Prec_Form runs inside the solve, as in the reference
         return solvers._solve(method, A, x, r, solve["prec"], solve["cap"], tol)
 
-    for _ in range(args.warmup):
-        res = step()
-    torch.cuda.synchronize()
-    lat = []
     lib = solvers._lib()
-    lib.tfs_timing_start(0)  # HIP events around every SpMV launch of the timed region
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        res = step()
-        lat.append(time.perf_counter() - ts)
+    for _ in range(args.warmup):
+        step()
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    import ctypes
+    lib.tfs_timing_start(0)  # HIP events around every SpMV launch of the timed region
+    elapsed, lat, res = timed(step, args.steps, 0, torch.cuda.synchronize)
     spmv_ms, spmv_n = ctypes.c_double(), ctypes.c_longlong()
     lib.tfs_timing_stop(ctypes.byref(spmv_ms), ctypes.byref(spmv_n))
     niter, ini_res, fin_res = res
-    ms_per_step = 1000.0 * elapsed / args.steps
     value = args.steps / elapsed
-    lat_ms = np.array(lat) * 1000.0
-    log(f"  {value:.2f} solves/s, {ms_per_step:.3f} ms/solve, niter={niter}, "
+    log(f"  {value:.2f} solves/s, {1e3 * elapsed / args.steps:.3f} ms/solve, niter={niter}, "
         f"fin_res={fin_res:.3e} (tol {tol:.3e})")
-
-    # Per-class breakdown from one instrumented solve per class (outside the timed region).
-    breakdown = {}
-    for kc, name in ((0, "spmv"), (1, "sweeps"), (2, "factor")):
-        lib.tfs_timing_start(kc)
-        step()
-        ms, nl = ctypes.c_double(), ctypes.c_longlong()
-        lib.tfs_timing_stop(ctypes.byref(ms), ctypes.byref(nl))
-        breakdown[name] = dict(ms_per_solve=ms.value, launches_per_solve=nl.value)
-
-    # Dominant streaming kernel: the SpMV.  Back-to-back launches give the
-    # clean per-launch duration that a kernel trace reports.
+    bd = breakdown(lib, step)
     y = torch.empty_like(b)
-    spmv_b2b_ms = A.spmv_time(x, y, reps=50)
-    algo = spmv_bytes(c.n, c.nnz)
-    spmv_inloop_ms = spmv_ms.value / max(spmv_n.value, 1)
-    achieved = algo / (spmv_b2b_ms * 1e-3) / 1e9
-    log(f"  spmv {spmv_b2b_ms * 1e3:.1f} us back-to-back ({achieved:.0f} GB/s), "
-        f"{spmv_inloop_ms * 1e3:.1f} us inside solves; breakdown {breakdown}")
-
+    roof = spmv_roofline(A, x, y, c.n, c.nnz,
+                         inloop_ms=spmv_ms.value / max(spmv_n.value, 1))
+    log(f"  spmv {roof['launch_us']:.1f} us back-to-back ({roof['achieved']:.0f} GB/s); "
+        f"breakdown {bd}")
+    # the solve again, for the solution compared with the CPU
+    step()
     x_gpu = x.cpu().numpy()
-    cpu, err_vs_cpu = (None, None)
+    cpu, err = None, None
     if not args.no_cpu:
         log(f"timing the CPU baseline ({args.cpu_reps} solves)...")
-        cpu, err_vs_cpu = cpu_baseline(c, solve, tol, args.cpu_reps, x_gpu)
-        log(f"  cpu {cpu['value']:.3f} solves/s ({cpu['kind']}), rel-L2(x_gpu - x_cpu) "
-            f"= {err_vs_cpu:.2e}")
-
-    out = {
-        "metric": "Poisson solves/s & SpMV HBM GB/s vs plain FP64 at 1e-10 resid; p50/p99 latency",
-        "value": value,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded hex mesh and RHS)",
-        "config": {
-            "workload": f"config {k}: " + ("momentum 128^3 hex (tanh beta 2, 10% jitter, "
-                                            "permuted), Bicg + DIC (prec 2) to rel 1e-10"
-                                            if k == 2 else "Poisson 32^3, Cg + DIC to rel 1e-8"),
-            "n_cells": c.n, "nnz": c.nnz, "method": solve["method"], "prec": solve["prec"],
-            "rtol": solve["rtol"], "niter": niter, "fin_res": fin_res, "tol": tol,
-            "levels_lower": info["levels_lower"], "levels_upper": info["levels_upper"],
-        },
+        cpu, err = cpu_full_solves(c, solve, tol, args.cpu_reps, x_gpu)
+        log(f"  cpu {cpu['value']:.3f} solves/s ({cpu['kind']}), rel-L2 {err:.2e}")
+    lat_ms = np.array(lat) * 1e3
+    name = ("config 2: momentum 128^3 hex (tanh beta 2, 10% jitter, permuted), Bicg + DIC "
+            "(prec 2) to rel 1e-10" if k == 2 else "config 1: Poisson 32^3, Cg + DIC to rel 1e-8")
+    out = line(args, value, elapsed, name, {
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
                        "p99": float(np.percentile(lat_ms, 99))},
-        "roofline": {
-            "kernel": "k_spmv (FP64 CSR SpMV)",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "algorithmic_bytes": algo,
-            "launch_us": spmv_b2b_ms * 1e3,
-            "launch_us_in_solves": spmv_inloop_ms * 1e3,
-        },
-        "breakdown_ms_per_solve": {k2: v["ms_per_solve"] for k2, v in breakdown.items()},
-        "cpu_baseline": cpu,
-        "rel_l2_vs_cpu": err_vs_cpu,
-    }
+        "roofline": roof,
+        "breakdown_ms_per_solve": {k2: v["ms_per_solve"] for k2, v in bd.items()},
+        "cpu_baseline": cpu, "rel_l2_vs_cpu": err,
+    })
+    out["config"].update(n_cells=c.n, nnz=c.nnz, method=solve["method"], prec=solve["prec"],
+                         rtol=solve["rtol"], niter=niter, fin_res=fin_res, tol=tol,
+                         levels_lower=info["levels_lower"], levels_upper=info["levels_upper"])
+    return out
+
+
+def mixed_vs_fp64(args):
+    """Workload 3: FP32-inner refinement against the plain FP64 solve."""
+    import numpy as np
+    import torch
+    from tflows_amd import cases, mesh, solvers
+    kw = cases.config_kwargs(3, args.scale)
+    solve = cases.SOLVE[3]
+    log(f"generating workload 3: {kw}")
+    t0 = time.perf_counter()
+    c = mesh.make_case(**kw)
+    log(f"  n={c.n} nnz={c.nnz} ({time.perf_counter() - t0:.1f}s)")
+    tol = solve["rtol"] * float(np.linalg.norm(c.b))
+    A = solvers.Matrix(c.row, c.col, c.dia, c.val)
+    info = A.info()
+    b = torch.from_numpy(c.b).cuda()
+    x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    r = torch.empty_like(b)
+    torch.cuda.synchronize()
+
+    def step64():
+        r.copy_(b)
+        x.zero_()
+        A.values_changed()
+        return solvers.Cg(A, x, r, solve["prec"], solve["cap"], tol)
+
+    def step32():
+        r.copy_(b)
+        x.zero_()
+        A.values_changed()
+        return solvers.solve_mixed(A, x, r, solve["prec"], 200, tol, args.inner_rtol)
+
+    e64, lat64, res64 = timed(step64, args.steps, args.warmup, torch.cuda.synchronize)
+    x64 = x.cpu().numpy()
+    log(f"  fp64: {1e3 * e64 / args.steps:.1f} ms/solve, niter={res64[0]}, fin={res64[2]:.3e}")
+    e32, lat32, res32 = timed(step32, args.steps, args.warmup, torch.cuda.synchronize)
+    x32 = x.cpu().numpy()
+    steps_out, inner, _, fin32 = res32
+    err = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
+    speedup = e64 / e32
+    log(f"  mixed: {1e3 * e32 / args.steps:.1f} ms/solve, {steps_out} refinement steps, "
+        f"{inner} inner iterations, fin={fin32:.3e}; speedup {speedup:.2f}x, rel-L2 vs FP64 "
+        f"{err:.2e}")
+    lib = solvers._lib()
+    bd = breakdown(lib, step64)
+    y = torch.empty_like(b)
+    roof = spmv_roofline(A, x, y, c.n, c.nnz)
+    cpu = None
+    if not args.no_cpu:
+        log("timing the CPU baseline (iteration sample)...")
+        cpu = cpu_iteration_sample(c, solve, args.cpu_iters, res64[0])
+        log(f"  cpu {cpu['value']:.4f} solves/s ({cpu['kind']})")
+    lat = np.array(lat32) * 1e3
+    out = line(args, args.steps / e32, e32,
+               f"config 3: Poisson {kw['nx']}^3 stretched to aspect 100, FP32-inner PCG "
+               f"(DIC) in FP64 refinement to rel 1e-10, plain FP64 Cg beside it", {
+                   "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                                  "p99": float(np.percentile(lat, 99))},
+                   "roofline": roof,
+                   "fp64": {"solves_per_s": args.steps / e64, "ms_per_solve": 1e3 * e64 / args.steps,
+                            "niter": res64[0], "fin_res": res64[2],
+                            "breakdown_ms_per_solve": {k2: v["ms_per_solve"] for k2, v in bd.items()}},
+                   "mixed": {"refinement_steps": steps_out, "inner_iterations": inner,
+                             "fin_res": fin32, "inner_rtol": args.inner_rtol,
+                             "rel_l2_vs_fp64": err},
+                   "speedup_vs_fp64": speedup, "cpu_baseline": cpu})
+    out["dtype"] = "f32 inner / f64 outer"
+    out["config"].update(n_cells=c.n, nnz=c.nnz, rtol=solve["rtol"], tol=tol,
+                         levels_lower=info["levels_lower"], levels_upper=info["levels_upper"])
+    return out
+
+
+def batch(args):
+    """Workload 4: 64 uneven Poisson systems in one batch."""
+    import numpy as np
+    import torch
+    from tflows_amd import cases, mesh, solvers
+    kws = cases.batch_systems()
+    log(f"generating workload 4: {len(kws)} systems")
+    sysl = [mesh.make_case(**kw) for kw in kws]
+    ns = [c.n for c in sysl]
+    log(f"  cells {min(ns)}..{max(ns)}, total {sum(ns)}")
+    B = solvers.Batch([(c.row, c.col, c.dia, c.val) for c in sysl])
+    tols = [1e-10 * float(np.linalg.norm(c.b)) for c in sysl]
+    bs = [torch.from_numpy(c.b).cuda() for c in sysl]
+    xs = [torch.zeros(c.n, dtype=torch.float64, device="cuda") for c in sysl]
+    rs = [torch.empty_like(v) for v in bs]
+    torch.cuda.synchronize()
+
+    def reset():
+        for x, r, b in zip(xs, rs, bs):
+            r.copy_(b)
+            x.zero_()
+
+    def step_mixed():
+        reset()
+        return B.solve_mixed(xs, rs, 2, 200, tols, args.inner_rtol)
+
+    def step64():
+        reset()
+        return B.solve(xs, rs, 2, [20000] * len(sysl), tols)
+
+    e64, _, r64 = timed(step64, args.steps, args.warmup, torch.cuda.synchronize)
+    x64 = [x.cpu().numpy() for x in xs]
+    e32, lat, r32 = timed(step_mixed, args.steps, args.warmup, torch.cuda.synchronize)
+    errs = [float(np.linalg.norm(x.cpu().numpy() - y) / np.linalg.norm(y)) for x, y in zip(xs, x64)]
+    steps_out, inner, _, fin, st = r32
+    assert all(s == 0 for s in st), st.tolist()
+    log(f"  batch fp64 {1e3 * e64 / args.steps:.1f} ms, mixed {1e3 * e32 / args.steps:.1f} ms "
+        f"per 64 solves; max rel-L2 mixed vs fp64 {max(errs):.2e}; info {B.info()}")
+    # the largest system alone, for the SpMV roofline at batch sizes
+    big = int(np.argmax(ns))
+    c = sysl[big]
+    A = solvers.Matrix(c.row, c.col, c.dia, c.val)
+    xb = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    roof = spmv_roofline(A, xb, torch.empty_like(xb), c.n, c.nnz)
+    cpu = None
+    if not args.no_cpu:
+        kind, run = cpu_solver()
+        order = np.argsort(ns)
+        spent, done_cells, cnt = 0.0, 0, 0
+        for i in order:                       # smallest systems first, ~20 s of CPU work
+            out, sec = run(sysl[i], "cg", 2, 20000, tols[i])
+            spent += sec
+            done_cells += ns[i]
+            cnt += 1
+            if spent > args.cpu_seconds:
+                break
+        per_cell = spent / done_cells
+        cpu_batch = per_cell * sum(ns)
+        cpu = dict(value=len(sysl) / cpu_batch, unit="solves/s", cores=1, kind=kind,
+                   sample=f"FP64 Cg+DIC solves of the {cnt} smallest systems ({done_cells} cells, "
+                          f"{spent:.1f} s), scaled per cell to all {sum(ns)} cells, 1 thread on "
+                          f"{cpu_name()}")
+    out = line(args, len(sysl) * args.steps / e32, e32,
+               "config 4: 64 Poisson systems, cells per axis log-uniform [16,96], beta [1,3], "
+               "one block-diagonal batch, FP32+FP64 refinement (DIC) to rel 1e-10", {
+                   "latency_ms": {"p50": float(np.percentile(np.array(lat) * 1e3, 50)),
+                                  "p99": float(np.percentile(np.array(lat) * 1e3, 99))},
+                   "roofline": roof,
+                   "fp64_batch_solves_per_s": len(sysl) * args.steps / e64,
+                   "speedup_vs_fp64": e64 / e32,
+                   "max_rel_l2_vs_fp64": max(errs),
+                   "refinement_steps_max": int(np.max(steps_out)),
+                   "inner_iterations_max": int(np.max(inner)),
+                   "batch_info": B.info(), "cpu_baseline": cpu})
+    out["dtype"] = "f32 inner / f64 outer"
+    out["config"].update(n_systems=len(sysl), cells_total=int(sum(ns)),
+                         cells_min=int(min(ns)), cells_max=int(max(ns)))
+    return out
+
+
+def serving(args):
+    """Workload 5: open-loop request stream through the serving loop."""
+    import numpy as np
+    import torch
+    from tflows_amd import cases, mesh, solvers
+    spec = cases.serving_cases(count=args.cases)
+    log(f"generating workload 5: {len(spec)} cases")
+    t0 = time.perf_counter()
+    cs = [(kind, mesh.make_case(**kw)) for kind, kw in spec]
+    log(f"  cells {[c.n for _, c in cs]} ({time.perf_counter() - t0:.1f}s)")
+    sched = cases.serving_schedule(len(cs), visits=args.visits, rate=args.rate)
+    reqs = []
+    for t_arr, ci, v in sched:
+        kind, c = cs[ci]
+        sv = cases.SERVING_SOLVE[kind]
+        b = cases.perturbed_rhs(c.b, ci, v)
+        reqs.append((t_arr, ci, v, sv, b, sv["rtol"] * float(np.linalg.norm(b))))
+    torch.cuda.synchronize()
+    # warm-up: one small solve through a server of its own (loads kernels)
+    warm = solvers.Server(1 << 30, workers=1)
+    kind, c = min(cs, key=lambda kc: kc[1].n)
+    wid = warm.add_case(c.row, c.col, c.dia, c.val)
+    for _ in range(max(args.warmup, 1)):
+        warm.wait(warm.submit(wid, "cg", 2, c.b, 1e-6 * float(np.linalg.norm(c.b)), 100))
+    warm.close()
+
+    srv = solvers.Server(int(args.budget_gb * 1e9), workers=args.workers)
+    ids = [srv.add_case(c.row, c.col, c.dia, c.val) for _, c in cs]
+    handles = [None] * len(reqs)
+    t_base = time.perf_counter()
+    srv_base = None
+    for i, (t_arr, ci, v, sv, b, tol) in enumerate(reqs):
+        delay = t_base + t_arr - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        handles[i] = srv.submit(ids[ci], sv["method"], sv["prec"], b, tol, sv["cap"],
+                                release=(v == args.visits - 1))
+    res = [srv.wait(h, want_x=False) for h in handles]
+    t_end = time.perf_counter()
+    st = srv.stats()
+    srv.close()
+    bad = [(i, r["status"]) for i, r in enumerate(res) if r["status"] != 0]
+    if bad:
+        log(f"  requests not converged or failed: {bad[:10]}")
+    # latency from the scheduled arrival (the submit time) to the result
+    lat = np.array([r["t_end"] - r["t_submit"] for r in res]) * 1e3
+    svc = np.array([r["t_end"] - r["t_start"] for r in res]) * 1e3
+    first = min(r["t_submit"] for r in res)
+    last = max(r["t_end"] for r in res)
+    makespan = last - first
+    value = len(res) / makespan
+    log(f"  {len(res)} requests in {makespan:.2f} s ({value:.1f} solves/s), latency p50 "
+        f"{np.percentile(lat, 50):.1f} ms p99 {np.percentile(lat, 99):.1f} ms; service p50 "
+        f"{np.percentile(svc, 50):.1f} ms; HBM peak {st['hbm_peak'] / 1e9:.2f} GB")
+    kind, c = max(cs, key=lambda kc: kc[1].nnz)
+    A = solvers.Matrix(c.row, c.col, c.dia, c.val)
+    xb = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    roof = spmv_roofline(A, xb, torch.empty_like(xb), c.n, c.nnz)
+    cpu = None
+    if not args.no_cpu:
+        kind_cpu, run = cpu_solver()
+        by_size = sorted(range(len(reqs)), key=lambda i: cs[reqs[i][1]][1].n)
+        spent, cells, cnt = 0.0, 0, 0
+        for i in by_size:
+            t_arr, ci, v, sv, b, tol = reqs[i]
+            kind, cc = cs[ci]
+            case_b = mesh.Case(n=cc.n, row=cc.row, col=cc.col, dia=cc.dia, val=cc.val, b=b)
+            _, sec = run(case_b, sv["method"], sv["prec"], sv["cap"], tol)
+            spent += sec
+            cells += cc.n
+            cnt += 1
+            if spent > args.cpu_seconds:
+                break
+        total_cells = sum(cs[r[1]][1].n for r in reqs)
+        cpu_total = spent / cells * total_cells
+        cpu = dict(value=len(reqs) / cpu_total, unit="solves/s", cores=1, kind=kind_cpu,
+                   sample=f"{cnt} smallest requests solved ({cells} cells, {spent:.1f} s), "
+                          f"scaled per cell to the stream's {total_cells} cells, 1 thread on "
+                          f"{cpu_name()}")
+    out = line(args, value, makespan,
+               f"config 5: serving, Poisson arrivals {args.rate:g} req/s, {len(cs)} cases "
+               f"log-uniform [1e4,4e6] cells, 70% pressure Cg / 30% momentum Bicg (DIC), "
+               f"{args.visits} visits each, rel 1e-10, HBM budget {args.budget_gb:g} GB", {
+                   "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                                  "p99": float(np.percentile(lat, 99))},
+                   "service_ms": {"p50": float(np.percentile(svc, 50)),
+                                  "p99": float(np.percentile(svc, 99))},
+                   "roofline": roof,
+                   "server": {"workers": args.workers, "hbm_peak_bytes": st["hbm_peak"],
+                              "requests": len(res), "failed": len(bad)},
+                   "cpu_baseline": cpu})
+    out["steps"] = len(res)
+    out["ms_per_step"] = 1e3 * makespan / len(res)
+    out["config"].update(n_cases=len(cs), cells=[c.n for _, c in cs])
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 4, 5])
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-iters", type=int, default=20,
+                    help="workload 3: iterations in the CPU sample")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="workloads 4-5: CPU seconds of the sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scale", type=float, default=1.0, help="workload 3: mesh scale per axis")
+    ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    ap.add_argument("--cases", type=int, default=24)
+    ap.add_argument("--visits", type=int, default=10)
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              f"(the solver path runs on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+    import torch  # noqa: F401  (loads the HIP runtime the library shares)
+    load_package()
+    if args.workload in (1, 2):
+        out = single_solve(args, args.workload)
+    elif args.workload == 3:
+        out = mixed_vs_fp64(args)
+    elif args.workload == 4:
+        out = batch(args)
+    else:
+        out = serving(args)
     print(json.dumps(out), flush=True)
 
 

@@ -46,6 +46,7 @@ extern "C" {
 enum {
   TFS_OK = 0,
   TFS_NOT_CONVERGED = 1,    /* niter came back as cap+1 (the reference's only signal) */
+  TFS_CANCELLED = 2,        /* serving: the request was withdrawn before it ran       */
   TFS_ERR_INVALID = -1,     /* bad argument or inconsistent CRS structure            */
   TFS_ERR_NOMEM = -2,       /* device allocation failed                              */
   TFS_ERR_BREAKDOWN = -3,   /* rho, (p,q) or ||r|| became 0, Inf or NaN              */
@@ -78,6 +79,14 @@ int tfs_matrix_set_values_device(tfs_matrix *A, const double *d_val);
 int tfs_matrix_values_changed(tfs_matrix *A);
 /* Device address of the matrix's FP64 values (nnz entries). */
 double *tfs_matrix_values_device(tfs_matrix *A);
+
+/* Schedule of the preconditioner sweeps (bitwise the same results):
+ *   TFS_SWEEP_LEVELS     one launch per level of the dependency DAG;
+ *   TFS_SWEEP_STATIC     one persistent cooperative launch per sweep (default);
+ *   TFS_SWEEP_DYNAMIC    one persistent launch taking chunks from a counter --
+ *                        safe when several solves share the GPU concurrently. */
+enum { TFS_SWEEP_LEVELS = 0, TFS_SWEEP_STATIC = 1, TFS_SWEEP_DYNAMIC = 2 };
+int tfs_matrix_set_sweep_schedule(tfs_matrix *A, int schedule);
 
 typedef struct {
   int n;
@@ -136,6 +145,40 @@ int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *const *d_x,
                                  double *ini_res, double *fin_res, int *status);
 int tfs_batch_get_info(const tfs_batch *B, int *nsys, int *n_total, long long *nnz_total,
                        int *graph_launches);
+
+/* ---- serving: continuous batching of solves on one GPU ------------------------
+ * Not in the reference.  A case (one system) is registered with its structure
+ * and values; its GPU copy is made when its first solve is admitted and stays
+ * resident across visits until a solve is marked as the case's last.
+ * Requests queue in arrival order; `workers` host threads each take the first
+ * request whose case is idle and resident or fits in the HBM budget, and solve
+ * it on the case's own stream, so solves of different cases overlap on the
+ * device.  A result does not depend on what else runs (per-case state,
+ * fixed-order reductions).  Calls on one server may come from any thread;
+ * times are seconds since the server was created. */
+typedef struct tfs_server tfs_server;
+int tfs_server_create(long long hbm_budget_bytes, int workers, tfs_server **out);
+/* Waits for all queued requests, then frees every resident case. */
+int tfs_server_destroy(tfs_server *S);
+/* Copies the structure and values; TFS_ERR_NOMEM if it could never fit.
+ * When a case does not fit, idle resident cases with no queued request are
+ * evicted, least recently used first (their next visit uploads them again). */
+int tfs_server_add_case(tfs_server *S, int n, long long nnz, const int *row, const int *col,
+                        const int *dia, const double *val, int base, int *case_id);
+/* Queues one solve (tfs_solve semantics, x0 = 0, b copied); release != 0
+ * frees the case's GPU copy after this solve. */
+int tfs_server_submit(tfs_server *S, int case_id, int method, int prec, const double *b,
+                      double tol, int cap, int release, long long *req);
+/* Blocks until `req` is done; x (n doubles) may be NULL.  Each request can be
+ * waited for once. */
+int tfs_server_wait(tfs_server *S, long long req, double *x, int *niter, double *ini_res,
+                    double *fin_res, int *status, double *t_submit, double *t_start,
+                    double *t_end);
+/* Withdraws a request that has not started (its wait then reports
+ * TFS_CANCELLED); TFS_ERR_INVALID once it runs or is done. */
+int tfs_server_cancel(tfs_server *S, long long req);
+int tfs_server_stats(tfs_server *S, long long *hbm_in_use, long long *hbm_peak,
+                     int *resident_cases, long long *completed);
 
 /* ---- building blocks on device vectors ---------------------------------------- */
 /* y := A x  (row sums in the reference order, Cg.f90:128-134). */

@@ -75,6 +75,60 @@ SOLVE = {
 }
 
 
+SERVING_SOLVE = {
+    "pressure": dict(method="cg", prec=2, rtol=1e-10, cap=20000),
+    "momentum": dict(method="bicg", prec=2, rtol=1e-10, cap=20000),
+}
+
+
+def serving_cases(count=24, lo=1e4, hi=4e6, p_pressure=0.7, k=5):
+    """Workload 5: per-case (kind name, make_case kwargs).  Cells log-uniform
+    in [lo, hi] on a cube-shaped hex mesh, tanh stretching uniform in [1, 3],
+    +-10% node jitter, random numbering; 70% pressure (Poisson, Cg) and 30%
+    momentum (Bicg)."""
+    s = seed(k)
+    out = []
+    for i in range(count):
+        cells = math.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * mesh.rng_u01(s, 300 + i, 0))
+        nx = max(2, int(round(cells ** (1.0 / 3.0))))
+        beta = 1.0 + 2.0 * mesh.rng_u01(s, 500 + i, 0)
+        if mesh.rng_u01(s, 400 + i, 0) < p_pressure:
+            out.append(("pressure", dict(kind=mesh.POISSON, nx=nx, beta=beta, jitter=0.1,
+                                         permute=True, pin_first=True, rhs_zero_mean=True,
+                                         seed=s * 1000 + i)))
+        else:
+            out.append(("momentum", dict(kind=mesh.MOMENTUM, nx=nx, beta=beta, jitter=0.1,
+                                         permute=True, pin_first=False, rhs_zero_mean=False,
+                                         nu=1e-3, cfl=1.0, n_modes=8, seed=s * 1000 + i)))
+    return out
+
+
+def serving_schedule(n_cases, visits=10, rate=200.0, k=5):
+    """Workload 5 request stream: (arrival time [s], case index, visit) in
+    arrival order.  Exponential inter-arrival times at `rate` requests/s
+    (a Poisson process); each case gets `visits` requests, the order of cases
+    drawn from the seed."""
+    rng = np.random.default_rng(seed(k))
+    order = np.repeat(np.arange(n_cases), visits)
+    rng.shuffle(order)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=order.size))
+    seen = np.zeros(n_cases, int)
+    out = []
+    for ti, ci in zip(t, order):
+        out.append((float(ti), int(ci), int(seen[ci])))
+        seen[ci] += 1
+    return out
+
+
+def perturbed_rhs(b, case_index, visit, eps=0.01, k=5):
+    """RHS of one time step: b scaled entrywise by 1 + eps*u, u uniform in
+    [-1, 1) from the seed (case, visit); visit 0 is b itself."""
+    if visit == 0:
+        return b.copy()
+    rng = np.random.default_rng([seed(k), case_index, visit])
+    return b * (1.0 + eps * (2.0 * rng.random(b.size) - 1.0))
+
+
 def batch_systems(count=64, lo=16, hi=96, k=4):
     """Workload 4: per-system make_case kwargs (log-uniform cells per axis)."""
     s = seed(k)

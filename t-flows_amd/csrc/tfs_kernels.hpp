@@ -562,13 +562,29 @@ __device__ __forceinline__ void coherent_store(T *p, T v) {
 // Sentinel upkeep (no extra passes): FWD arms arm[p] (the backward sweep's
 // output); BWD re-arms b at the entry it consumes (the forward sweep's
 // output); FACTOR's output is armed by the caller.
-template <typename T, int OP>
+//
+// DYN selects the schedule: false -- static (chunk w, w+G, ...; needs the
+// cooperative launch's co-residency); true -- dynamic: chunks are taken in
+// order from a counter (ctr[0]), so a chunk is only ever held by a running
+// workgroup and the schedule is deadlock-free whatever else shares the GPU
+// (used when several solves run concurrently).  The last workgroup to leave
+// (ctr[1]) resets both counters for the next launch.
+template <typename T, int OP, bool DYN>
 __global__ void __launch_bounds__(kChunk)
-k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, int *err) {
+k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, int *err,
+                 unsigned int *ctr) {
   if (st && solve_done(st)) return;
   T *out = OP == OP_FACTOR ? d : x;
   const int nchunks = (n + kChunk - 1) / kChunk;
-  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+  __shared__ int s_chunk;
+  for (int c = DYN ? -1 : (int)blockIdx.x;; c += DYN ? 0 : (int)gridDim.x) {
+    if (DYN) {
+      __syncthreads();  // everyone has read the previous s_chunk
+      if (threadIdx.x == 0) s_chunk = (int)atomicAdd(&ctr[0], 1u);
+      __syncthreads();
+      c = s_chunk;
+    }
+    if (c >= nchunks) break;
     const int p = c * kChunk + (int)threadIdx.x;
     if (p >= n) continue;
     const int e0 = Tr.ptr[p], e1 = Tr.ptr[p + 1], len = e1 - e0;
@@ -628,6 +644,14 @@ k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, i
       __builtin_amdgcn_s_sleep(2);
     }
     coherent_store(&out[p], OP == OP_FACTOR ? T(1) / s : s * dp);
+  }
+  if (DYN && threadIdx.x == 0) {
+    const unsigned int prev =
+        __hip_atomic_fetch_add(&ctr[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == gridDim.x - 1) {  // everyone else has left
+      ctr[0] = 0u;
+      ctr[1] = 0u;
+    }
   }
 }
 

@@ -35,10 +35,19 @@ struct Lib {
   std::mutex mu;
   bool ready = false;
   int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 Lib g_lib;
+
+// Every matrix (and batch) owns a HIP stream; an API call runs all its work
+// on the stream of the matrix it was given, so solves of different matrices
+// issued from different host threads overlap on the device.
+thread_local hipStream_t t_stream = nullptr;
+inline hipStream_t cur_stream() { return t_stream; }
+struct StreamScope {
+  hipStream_t prev;
+  explicit StreamScope(hipStream_t s) : prev(t_stream) { t_stream = s; }
+  ~StreamScope() { t_stream = prev; }
+};
 
 // Optional HIP-event timing of one kernel class on the library stream: an
 // event pair brackets every launch of that class; pairs are harvested at the
@@ -57,11 +66,11 @@ inline bool tm_active(int kc) {
   return g_tm.on && g_tm.kclass == kc && g_tm.used + 2 <= g_tm.ev.size();
 }
 inline void tm_begin(int kc) {
-  if (tm_active(kc)) (void)hipEventRecord(g_tm.ev[g_tm.used], g_lib.stream);
+  if (tm_active(kc)) (void)hipEventRecord(g_tm.ev[g_tm.used], cur_stream());
 }
 inline void tm_end(int kc) {
   if (tm_active(kc)) {
-    (void)hipEventRecord(g_tm.ev[g_tm.used + 1], g_lib.stream);
+    (void)hipEventRecord(g_tm.ev[g_tm.used + 1], cur_stream());
     g_tm.used += 2;
   }
 }
@@ -142,7 +151,11 @@ struct tfs_matrix {
   KState *h_st = nullptr;      // pinned, 2 entries
   double *partials = nullptr;
   int *sweep_err = nullptr;    // set by a persistent sweep whose wait ran out
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool persistent = true;      // one launch per sweep (else one per level)
+  bool dynamic = false;        // persistent sweeps take chunks from a counter
+  unsigned int *sweep_ctr = nullptr;  // the dynamic schedule's counters
   int grid_persistent = 0;
   long long bytes = 0;
   int bandwidth = 0;
@@ -182,9 +195,6 @@ int ensure_init() {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return TFS_ERR_NO_DEVICE;
   if (g_lib.device >= ndev) return TFS_ERR_INVALID;
   TFS_HIP(hipSetDevice(g_lib.device));
-  TFS_HIP(hipStreamCreateWithFlags(&g_lib.stream, hipStreamNonBlocking));
-  TFS_HIP(hipEventCreate(&g_lib.ev0));
-  TFS_HIP(hipEventCreate(&g_lib.ev1));
   g_lib.ready = true;
   return 0;
 }
@@ -200,20 +210,20 @@ inline int spmv_parts(const tfs_matrix *A) { return grid_for(A->nblk, kSpmvWaves
 
 template <typename TO, typename TI>
 void gather(int n, const TI *in, const int *idx, TO *out, const KState *st = nullptr) {
-  hipLaunchKernelGGL((k_gather<TO, TI>), dim3(vgrid(n)), dim3(kRedThreads), 0, g_lib.stream, n,
+  hipLaunchKernelGGL((k_gather<TO, TI>), dim3(vgrid(n)), dim3(kRedThreads), 0, cur_stream(), n,
                      in, idx, out, st);
 }
 template <typename T>
 void scatter(int n, const T *in, const int *idx, T *out) {
-  hipLaunchKernelGGL(k_scatter<T>, dim3(vgrid(n)), dim3(kRedThreads), 0, g_lib.stream, n, in, idx,
+  hipLaunchKernelGGL(k_scatter<T>, dim3(vgrid(n)), dim3(kRedThreads), 0, cur_stream(), n, in, idx,
                      out);
 }
 template <typename T>
 void arm(int n, T *x) {
-  hipLaunchKernelGGL(k_arm<T>, dim3(vgrid(n)), dim3(kRedThreads), 0, g_lib.stream, n, x);
+  hipLaunchKernelGGL(k_arm<T>, dim3(vgrid(n)), dim3(kRedThreads), 0, cur_stream(), n, x);
 }
 void finish(int nparts, const double *partials, int epi, KState *st) {
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, g_lib.stream, nparts, partials, epi,
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, cur_stream(), nparts, partials, epi,
                      st);
 }
 template <typename T>
@@ -227,6 +237,14 @@ template <typename T, int OP>
 void launch_sweep(tfs_matrix *A, const DevSweep &S, const Tri<T> &Tr, T *d, const T *b, T *x,
                   const KState *st, T *armv = nullptr) {
   const int kc = OP == OP_FACTOR ? TFS_KC_FACTOR : TFS_KC_SWEEP;
+  if (A->persistent && A->dynamic) {
+    tm_begin(kc);
+    hipLaunchKernelGGL((k_sweep_syncfree<T, OP, true>), dim3(A->grid_persistent), dim3(kChunk), 0,
+                       cur_stream(), A->n, Tr, d, const_cast<T *>(b), x, armv, st, A->sweep_err,
+                       A->sweep_ctr);
+    tm_end(kc);
+    return;
+  }
   if (A->persistent) {
     tm_begin(kc);
     int nn = A->n;
@@ -234,10 +252,11 @@ void launch_sweep(tfs_matrix *A, const DevSweep &S, const Tri<T> &Tr, T *d, cons
     T *dd = d, *bb = const_cast<T *>(b), *xx = x, *aa = armv;
     const KState *ss = st;
     int *ee = A->sweep_err;
-    void *args[] = {&nn, &TT, &dd, &bb, &xx, &aa, &ss, &ee};
+    unsigned int *cc = A->sweep_ctr;
+    void *args[] = {&nn, &TT, &dd, &bb, &xx, &aa, &ss, &ee, &cc};
     const hipError_t e = hipLaunchCooperativeKernel(
-        reinterpret_cast<const void *>(k_sweep_syncfree<T, OP>), dim3(A->grid_persistent),
-        dim3(kChunk), args, 0, g_lib.stream);
+        reinterpret_cast<const void *>(k_sweep_syncfree<T, OP, false>), dim3(A->grid_persistent),
+        dim3(kChunk), args, 0, cur_stream());
     tm_end(kc);
     if (e == hipSuccess) return;
     // No cooperative launch here: fall back to one launch per level for this
@@ -250,12 +269,12 @@ void launch_sweep(tfs_matrix *A, const DevSweep &S, const Tri<T> &Tr, T *d, cons
   for (const Segment &g : S.segs) {
     tm_begin(kc);
     if (g.small) {
-      hipLaunchKernelGGL((k_levels_small<T, OP>), dim3(1), dim3(kSmallThreads), 0, g_lib.stream,
+      hipLaunchKernelGGL((k_levels_small<T, OP>), dim3(1), dim3(kSmallThreads), 0, cur_stream(),
                          g.l0, g.l1, S.lev, Tr, d, b, x, st);
     } else {
       const int p0 = S.lev_ptr[g.l0], p1 = S.lev_ptr[g.l1];
       hipLaunchKernelGGL((k_level<T, OP>), dim3((p1 - p0 + kLevelThreads - 1) / kLevelThreads),
-                         dim3(kLevelThreads), 0, g_lib.stream, p0, p1, Tr, d, b, x, st);
+                         dim3(kLevelThreads), 0, cur_stream(), p0, p1, Tr, d, b, x, st);
     }
     tm_end(kc);
   }
@@ -333,7 +352,7 @@ void prec_apply_S(tfs_matrix *A, int prec, const T *bS, T *xS, T *yL, T *zU, con
     dic_apply<T>(A, bS, yL, zU, st);
     gather(A->n, zU, A->s_to_u, xS, st);
   } else {
-    hipLaunchKernelGGL(k_diag_apply<T>, dim3(vgrid(A->n)), dim3(kRedThreads), 0, g_lib.stream,
+    hipLaunchKernelGGL(k_diag_apply<T>, dim3(vgrid(A->n)), dim3(kRedThreads), 0, cur_stream(),
                        A->n, bS, vals<T>(A).s_diag, xS, prec, st);
   }
 }
@@ -341,7 +360,7 @@ void prec_apply_S(tfs_matrix *A, int prec, const T *bS, T *xS, T *yL, T *zU, con
 template <typename T, int MODE>
 void launch_spmv(tfs_matrix *A, const T *x, T *y, const T *w, T *xs, T *rs, const KState *st) {
   tm_begin(TFS_KC_SPMV);
-  hipLaunchKernelGGL((k_spmv<T, MODE>), dim3(spmv_parts(A)), dim3(kSpmvThreads), 0, g_lib.stream,
+  hipLaunchKernelGGL((k_spmv<T, MODE>), dim3(spmv_parts(A)), dim3(kSpmvThreads), 0, cur_stream(),
                      A->nblk, A->rblk, A->s_row, A->s_col, vals<T>(A).s_val, x, y, w, xs, rs, st,
                      A->partials);
   tm_end(TFS_KC_SPMV);
@@ -383,13 +402,13 @@ void cg_iteration(tfs_matrix *A, int prec, const Vecs<T> &V, KState *st) {
   if (prec == 2) dic_apply<T>(A, V.r, V.yL, V.zU, st);
   const int mode = prec == 2 ? 2 : prec == 1 ? 1 : 0;
   T *z = prec == 0 ? V.r : V.z;  // prec 0: z is a copy of r (Prec_Solve.f90:72-75)
-  hipLaunchKernelGGL(k_rho<T>, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, V.r, V.z,
+  hipLaunchKernelGGL(k_rho<T>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, V.r, V.z,
                      vals<T>(A).s_diag, V.zU, A->s_to_u, mode, st, A->partials);
   finish(g, A->partials, EPI_RHO, st);
-  hipLaunchKernelGGL(k_pupdate<T>, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, z, V.p, st);
+  hipLaunchKernelGGL(k_pupdate<T>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, z, V.p, st);
   launch_spmv<T, SPMV_DOT>(A, V.p, V.q, V.p, nullptr, nullptr, st);
   finish(spmv_parts(A), A->partials, EPI_ALPHA, st);
-  hipLaunchKernelGGL(k_axpy_norm<T>, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, V.x, V.r,
+  hipLaunchKernelGGL(k_axpy_norm<T>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, V.x, V.r,
                      V.p, V.q, st, A->partials);
   finish(g, A->partials, EPI_NORM, st);
 }
@@ -397,10 +416,10 @@ void cg_iteration(tfs_matrix *A, int prec, const Vecs<T> &V, KState *st) {
 // One CGS iteration (Cgs.f90:104-226), FP64.
 void cgs_iteration(tfs_matrix *A, int prec, const Vecs<double> &V, KState *st) {
   const int n = A->n, g = vgrid(n);
-  hipLaunchKernelGGL(k_rho<double>, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, V.r, V.r2,
+  hipLaunchKernelGGL(k_rho<double>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, V.r, V.r2,
                      nullptr, nullptr, nullptr, 3, st, A->partials);  // rho = (r1, r2)
   finish(g, A->partials, EPI_RHO, st);
-  hipLaunchKernelGGL(k_cgs_u, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, V.r, V.q1, V.u1,
+  hipLaunchKernelGGL(k_cgs_u, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, V.r, V.q1, V.u1,
                      V.u2, st);
   double *pp2 = V.u2;  // prec 0: p2 is a copy of u2
   if (prec != 0) {
@@ -409,7 +428,7 @@ void cgs_iteration(tfs_matrix *A, int prec, const Vecs<double> &V, KState *st) {
   }
   launch_spmv<double, SPMV_DOT>(A, pp2, V.v2, V.r2, nullptr, nullptr, st);  // v2 = A p2, (r2,v2)
   finish(spmv_parts(A), A->partials, EPI_ALPHA, st);
-  hipLaunchKernelGGL(k_cgs_q, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, V.u1, V.v2, V.q1,
+  hipLaunchKernelGGL(k_cgs_q, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, V.u1, V.v2, V.q1,
                      V.w, st);
   double *pp1 = V.w;
   if (prec != 0) {
@@ -431,15 +450,15 @@ int check_sweep(tfs_matrix *A) {
   TFS_HIP(hipMemset(A->sweep_err, 0, sizeof(int)));
   if (A->vec64) arm(A->n, A->vec64 + 2 * (size_t)A->n);
   if (A->vec32) arm(A->n, A->vec32 + 2 * (size_t)A->n);
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   std::fprintf(stderr, "tfs: a preconditioner sweep timed out waiting for its dependencies\n");
   return TFS_ERR_DEVICE;
 }
 
 int read_state(tfs_matrix *A, int k = 0) {
   TFS_HIP(hipMemcpyAsync(A->h_st + k, A->d_st + k, sizeof(KState), hipMemcpyDeviceToHost,
-                         g_lib.stream));
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+                         cur_stream()));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   if (g_tm.on) tm_harvest();
   return 0;
 }
@@ -450,7 +469,7 @@ int init_state(tfs_matrix *A, int k, double tol, int cap, double ini_res) {
   h->tol = tol;
   h->cap = cap;
   h->ini_res = ini_res;
-  TFS_HIP(hipMemcpyAsync(A->d_st + k, h, sizeof(KState), hipMemcpyHostToDevice, g_lib.stream));
+  TFS_HIP(hipMemcpyAsync(A->d_st + k, h, sizeof(KState), hipMemcpyHostToDevice, cur_stream()));
   return 0;
 }
 
@@ -517,12 +536,12 @@ int solve_device(tfs_matrix *A, int method, int prec, double *x, double *r1, int
   gather(n, x, A->s_orig, V.x);
   gather(n, r1, A->s_orig, V.r);
   // ||b|| and its early exit; r = b - A x; ||r|| and its early exit
-  hipLaunchKernelGGL(k_sumsq<double>, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, V.r,
+  hipLaunchKernelGGL(k_sumsq<double>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, V.r,
                      (int)EPI_BNRM, st, A->partials);
   finish(g, A->partials, EPI_BNRM, st);
   launch_spmv<double, SPMV_RESID>(A, V.x, V.r, nullptr, nullptr, nullptr, st);
   if (method == TFS_CGS) gather(n, V.r, (const int *)nullptr, V.r2, st);  // r2 = r1 (Cgs.f90:75-77)
-  hipLaunchKernelGGL(k_sumsq<double>, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, V.r,
+  hipLaunchKernelGGL(k_sumsq<double>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, V.r,
                      (int)EPI_INIRES, st, A->partials);
   finish(g, A->partials, EPI_INIRES, st);
   TFS_HIP(hipGetLastError());
@@ -535,7 +554,7 @@ int solve_device(tfs_matrix *A, int method, int prec, double *x, double *r1, int
   scatter(n, V.x, A->s_orig, x);
   scatter(n, V.r, A->s_orig, r1);
   TFS_HIP(hipGetLastError());
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   TFS_TRY(check_sweep(A));
   const KState *h = A->h_st;
   *niter = h->niter_out;
@@ -577,7 +596,7 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
   auto outer_residual = [&]() {
     gather(n, D.z, (const int *)nullptr, D.r);
     launch_spmv<double, SPMV_RESID>(A, D.x, D.r, nullptr, nullptr, nullptr, nullptr);
-    hipLaunchKernelGGL(k_sumsq<double>, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, D.r,
+    hipLaunchKernelGGL(k_sumsq<double>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.r,
                        (int)EPI_BNRM, so, A->partials);
     finish(g, A->partials, EPI_BNRM, so);
   };
@@ -597,9 +616,9 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
     ++outer;
     // inner solve, FP32: e = 0, r32 = r
     gather(n, D.r, (const int *)nullptr, F.r);
-    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, F.x, 0.0f);
+    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.x, 0.0f);
     TFS_TRY(init_state(A, 0, std::max(inner_rtol * err, 0.5 * tol), 100000, 0.0));
-    hipLaunchKernelGGL(k_sumsq<float>, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, F.r,
+    hipLaunchKernelGGL(k_sumsq<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.r,
                        (int)EPI_INIRES, si, A->partials);
     finish(g, A->partials, EPI_INIRES, si);
     TFS_HIP(hipGetLastError());
@@ -611,7 +630,7 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
       break;
     }
     // x += e; r = b - A x; ||r||
-    hipLaunchKernelGGL(k_add_correction, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, D.x,
+    hipLaunchKernelGGL(k_add_correction, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x,
                        F.x);
     TFS_TRY(init_state(A, 1, tol, cap, 0.0));
     outer_residual();
@@ -633,7 +652,7 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
   scatter(n, D.x, A->s_orig, x);
   scatter(n, D.r, A->s_orig, r1);
   TFS_HIP(hipGetLastError());
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   TFS_TRY(check_sweep(A));
   *ini_res = err0;
   *fin_res = err;
@@ -659,13 +678,17 @@ template <typename T, int OP>
 int occupancy(int *per_cu) {
   int b = 0;
   TFS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &b, reinterpret_cast<const void *>(k_sweep_syncfree<T, OP>), kChunk, 0));
+      &b, reinterpret_cast<const void *>(k_sweep_syncfree<T, OP, false>), kChunk, 0));
   *per_cu = std::min(*per_cu, b);
   return 0;
 }
 
 int create_device(tfs_matrix *A, const Layout &Ly) {
   const int n = A->n;
+  TFS_HIP(hipStreamCreateWithFlags(&A->stream, hipStreamNonBlocking));
+  TFS_HIP(hipEventCreate(&A->ev0));
+  TFS_HIP(hipEventCreate(&A->ev1));
+  StreamScope scope(A->stream);
   TFS_TRY(upload(A, &A->s_row, Ly.s_row));
   TFS_TRY(upload(A, &A->s_col, Ly.s_col));
   TFS_TRY(upload(A, &A->s_src, Ly.s_src));
@@ -687,6 +710,8 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
   TFS_TRY(dev_alloc(A, (void **)&A->partials, sizeof(double) * kMaxGrid));
   TFS_TRY(dev_alloc(A, (void **)&A->sweep_err, sizeof(int)));
   TFS_HIP(hipMemset(A->sweep_err, 0, sizeof(int)));
+  TFS_TRY(dev_alloc(A, (void **)&A->sweep_ctr, 2 * sizeof(unsigned int)));
+  TFS_HIP(hipMemset(A->sweep_ctr, 0, 2 * sizeof(unsigned int)));
   TFS_HIP(hipMemset(A->d_st, 0, 2 * sizeof(KState)));
   TFS_HIP(hipMemset(A->val, 0, sizeof(double) * (size_t)std::max(A->nnz, 1LL)));
   TFS_HIP(hipHostMalloc((void **)&A->h_st, 2 * sizeof(KState), hipHostMallocDefault));
@@ -707,8 +732,13 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
   A->persistent = coop != 0 && per_cu > 0;
   if (const char *e = std::getenv("TFS_SWEEP")) {
     if (std::strcmp(e, "levels") == 0) A->persistent = false;
+    if (std::strcmp(e, "dynamic") == 0) A->dynamic = true;
   }
   A->bandwidth = Ly.bandwidth;
+  // the uploads and fills above ran on the null stream; the matrix stream is
+  // non-blocking, so finish them before any work is queued there (the null
+  // stream only: other matrices' streams keep running)
+  TFS_HIP(hipStreamSynchronize(nullptr));
   return 0;
 }
 
@@ -723,6 +753,7 @@ extern "C" const char *tfs_status_string(int s) {
     case TFS_NOT_CONVERGED: return "not converged (niter = cap+1)";
     case TFS_ERR_INVALID: return "invalid argument or CRS structure";
     case TFS_ERR_NOMEM: return "device out of memory";
+    case TFS_CANCELLED: return "cancelled before it ran";
     case TFS_ERR_BREAKDOWN: return "breakdown (zero, Inf or NaN in a Krylov scalar)";
     case TFS_ERR_DEVICE: return "HIP runtime error";
     case TFS_ERR_NO_DEVICE: return "no GPU visible";
@@ -791,27 +822,34 @@ extern "C" int tfs_matrix_create(int n, long long nnz, const int *row, const int
 
 extern "C" int tfs_matrix_destroy(tfs_matrix *A) {
   if (!A) return TFS_OK;
-  if (g_lib.stream) (void)hipStreamSynchronize(g_lib.stream);
+  if (A->stream) (void)hipStreamSynchronize(A->stream);
   for (void *p : A->allocs) (void)hipFree(p);
   if (A->h_st) (void)hipHostFree(A->h_st);
+  if (A->ev0) (void)hipEventDestroy(A->ev0);
+  if (A->ev1) (void)hipEventDestroy(A->ev1);
+  if (A->stream) (void)hipStreamDestroy(A->stream);
   delete A;
   return TFS_OK;
 }
 
 extern "C" int tfs_matrix_set_values(tfs_matrix *A, const double *val) {
+  if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
   if (!A || (A->nnz > 0 && !val)) return TFS_ERR_INVALID;
   TFS_HIP(hipMemcpyAsync(A->val, val, sizeof(double) * (size_t)A->nnz, hipMemcpyHostToDevice,
-                         g_lib.stream));
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+                         cur_stream()));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   values_changed(A);
   return TFS_OK;
 }
 
 extern "C" int tfs_matrix_set_values_device(tfs_matrix *A, const double *d_val) {
+  if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
   if (!A || (A->nnz > 0 && !d_val)) return TFS_ERR_INVALID;
   TFS_HIP(hipMemcpyAsync(A->val, d_val, sizeof(double) * (size_t)A->nnz, hipMemcpyDeviceToDevice,
-                         g_lib.stream));
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+                         cur_stream()));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   values_changed(A);
   return TFS_OK;
 }
@@ -837,24 +875,27 @@ extern "C" int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info) {
 
 extern "C" int tfs_solve_device(tfs_matrix *A, int method, int prec, double *d_x, double *d_r1,
                                 int *niter, double tol, double *ini_res, double *fin_res) {
+  if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
   return solve_device(A, method, prec, d_x, d_r1, niter, tol, ini_res, fin_res);
 }
 
 extern "C" int tfs_solve(tfs_matrix *A, int method, int prec, double *x1, double *r1, int *niter,
                          double tol, double *ini_res, double *fin_res) {
   if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
   if (A->n > 0 && (!x1 || !r1)) return TFS_ERR_INVALID;
   const size_t bytes = sizeof(double) * (size_t)A->n;
   if (A->n > 0) {
-    TFS_HIP(hipMemcpyAsync(A->stage_x, x1, bytes, hipMemcpyHostToDevice, g_lib.stream));
-    TFS_HIP(hipMemcpyAsync(A->stage_r, r1, bytes, hipMemcpyHostToDevice, g_lib.stream));
+    TFS_HIP(hipMemcpyAsync(A->stage_x, x1, bytes, hipMemcpyHostToDevice, cur_stream()));
+    TFS_HIP(hipMemcpyAsync(A->stage_r, r1, bytes, hipMemcpyHostToDevice, cur_stream()));
   }
   const int rc = solve_device(A, method, prec, A->stage_x, A->stage_r, niter, tol, ini_res, fin_res);
   if (rc < 0 && rc != TFS_ERR_BREAKDOWN) return rc;
   if (A->n > 0) {
-    TFS_HIP(hipMemcpyAsync(x1, A->stage_x, bytes, hipMemcpyDeviceToHost, g_lib.stream));
-    TFS_HIP(hipMemcpyAsync(r1, A->stage_r, bytes, hipMemcpyDeviceToHost, g_lib.stream));
-    TFS_HIP(hipStreamSynchronize(g_lib.stream));
+    TFS_HIP(hipMemcpyAsync(x1, A->stage_x, bytes, hipMemcpyDeviceToHost, cur_stream()));
+    TFS_HIP(hipMemcpyAsync(r1, A->stage_r, bytes, hipMemcpyDeviceToHost, cur_stream()));
+    TFS_HIP(hipStreamSynchronize(cur_stream()));
   }
   return rc;
 }
@@ -862,12 +903,16 @@ extern "C" int tfs_solve(tfs_matrix *A, int method, int prec, double *x1, double
 extern "C" int tfs_solve_mixed_device(tfs_matrix *A, int prec, double *d_x, double *d_r1,
                                       int *niter, int *inner_iters, double tol,
                                       double inner_rtol, double *ini_res, double *fin_res) {
+  if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
   return solve_mixed_device(A, prec, d_x, d_r1, niter, inner_iters, tol, inner_rtol, ini_res,
                             fin_res);
 }
 
 // Building blocks on caller-order vectors: permuted into the S layout and back.
 extern "C" int tfs_spmv_device(tfs_matrix *A, const double *d_x, double *d_y) {
+  if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
   if (!A || (A->n > 0 && (!d_x || !d_y))) return TFS_ERR_INVALID;
   if (A->n == 0) return TFS_OK;
   TFS_TRY(ensure_vec64(A, 7));
@@ -877,11 +922,13 @@ extern "C" int tfs_spmv_device(tfs_matrix *A, const double *d_x, double *d_y) {
   launch_spmv<double, SPMV_PLAIN>(A, V.x, V.q, nullptr, nullptr, nullptr, nullptr);
   scatter(A->n, V.q, A->s_orig, d_y);
   TFS_HIP(hipGetLastError());
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   return TFS_OK;
 }
 
 extern "C" int tfs_residual_device(tfs_matrix *A, const double *d_x, double *d_r) {
+  if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
   if (!A || (A->n > 0 && (!d_x || !d_r))) return TFS_ERR_INVALID;
   if (A->n == 0) return TFS_OK;
   TFS_TRY(ensure_vec64(A, 7));
@@ -892,19 +939,23 @@ extern "C" int tfs_residual_device(tfs_matrix *A, const double *d_x, double *d_r
   launch_spmv<double, SPMV_RESID>(A, V.x, V.r, nullptr, nullptr, nullptr, nullptr);
   scatter(A->n, V.r, A->s_orig, d_r);
   TFS_HIP(hipGetLastError());
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   return TFS_OK;
 }
 
 extern "C" int tfs_prec_form(tfs_matrix *A, int prec) {
   if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
+  if (!A) return TFS_ERR_INVALID;
   TFS_TRY(prec_form<double>(A, prec));
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   TFS_TRY(check_sweep(A));
   return TFS_OK;
 }
 
 extern "C" int tfs_prec_solve_device(tfs_matrix *A, int prec, const double *d_b, double *d_x) {
+  if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
   if (!A || (A->n > 0 && (!d_b || !d_x))) return TFS_ERR_INVALID;
   if (prec < 0 || prec > 2) prec = 0;
   if (A->v64.factor_prec != prec) return TFS_ERR_INVALID;  // tfs_prec_form first
@@ -915,12 +966,14 @@ extern "C" int tfs_prec_solve_device(tfs_matrix *A, int prec, const double *d_b,
   prec_apply_S<double>(A, prec, V.r, V.z, V.yL, V.zU, nullptr);
   scatter(A->n, V.z, A->s_orig, d_x);
   TFS_HIP(hipGetLastError());
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   TFS_TRY(check_sweep(A));
   return TFS_OK;
 }
 
 extern "C" int tfs_prec_diag_device(tfs_matrix *A, double *d_out) {
+  if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
   if (!A || (A->n > 0 && !d_out)) return TFS_ERR_INVALID;
   const int fp = A->v64.factor_prec;
   if (fp < 0) return TFS_ERR_INVALID;
@@ -930,15 +983,17 @@ extern "C" int tfs_prec_diag_device(tfs_matrix *A, double *d_out) {
   } else if (fp == 1) {
     gather(A->n, A->v64.s_diag, A->s_of_orig, d_out);
   } else {
-    hipLaunchKernelGGL(k_fill<double>, dim3(vgrid(A->n)), dim3(kRedThreads), 0, g_lib.stream,
+    hipLaunchKernelGGL(k_fill<double>, dim3(vgrid(A->n)), dim3(kRedThreads), 0, cur_stream(),
                        A->n, d_out, 1.0);
   }
   TFS_HIP(hipGetLastError());
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   return TFS_OK;
 }
 
 extern "C" int tfs_prec_diag32_device(tfs_matrix *A, int prec, double *d_out) {
+  if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
   if (!A || (A->n > 0 && !d_out)) return TFS_ERR_INVALID;
   if (prec < 0 || prec > 2) prec = 0;
   if (A->n == 0) return TFS_OK;
@@ -948,16 +1003,18 @@ extern "C" int tfs_prec_diag32_device(tfs_matrix *A, int prec, double *d_out) {
   } else if (prec == 1) {
     gather(A->n, A->v32.s_diag, A->s_of_orig, d_out);
   } else {
-    hipLaunchKernelGGL(k_fill<double>, dim3(vgrid(A->n)), dim3(kRedThreads), 0, g_lib.stream,
+    hipLaunchKernelGGL(k_fill<double>, dim3(vgrid(A->n)), dim3(kRedThreads), 0, cur_stream(),
                        A->n, d_out, 1.0);
   }
   TFS_HIP(hipGetLastError());
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   TFS_TRY(check_sweep(A));
   return TFS_OK;
 }
 
 extern "C" int tfs_spmv_time(tfs_matrix *A, const double *d_x, double *d_y, int reps, double *ms) {
+  if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
   if (!A || !ms || reps < 1 || (A->n > 0 && (!d_x || !d_y))) return TFS_ERR_INVALID;
   if (A->n == 0) {
     *ms = 0.0;
@@ -966,14 +1023,14 @@ extern "C" int tfs_spmv_time(tfs_matrix *A, const double *d_x, double *d_y, int 
   TFS_TRY(prepare_values<double>(A));
   // The vectors are taken as S-ordered: the kernel's traffic is the same.
   launch_spmv<double, SPMV_PLAIN>(A, d_x, d_y, nullptr, nullptr, nullptr, nullptr);  // warm
-  TFS_HIP(hipEventRecord(g_lib.ev0, g_lib.stream));
+  TFS_HIP(hipEventRecord(A->ev0, cur_stream()));
   for (int k = 0; k < reps; ++k)
     launch_spmv<double, SPMV_PLAIN>(A, d_x, d_y, nullptr, nullptr, nullptr, nullptr);
-  TFS_HIP(hipEventRecord(g_lib.ev1, g_lib.stream));
+  TFS_HIP(hipEventRecord(A->ev1, cur_stream()));
   TFS_HIP(hipGetLastError());
-  TFS_HIP(hipEventSynchronize(g_lib.ev1));
+  TFS_HIP(hipEventSynchronize(A->ev1));
   float t = 0.f;
-  TFS_HIP(hipEventElapsedTime(&t, g_lib.ev0, g_lib.ev1));
+  TFS_HIP(hipEventElapsedTime(&t, A->ev0, A->ev1));
   *ms = (double)t / reps;
   return TFS_OK;
 }
@@ -981,7 +1038,7 @@ extern "C" int tfs_spmv_time(tfs_matrix *A, const double *d_x, double *d_y, int 
 extern "C" int tfs_timing_start(int kclass) {
   if (kclass < TFS_KC_SPMV || kclass > TFS_KC_VECTOR) return TFS_ERR_INVALID;
   TFS_TRY(ensure_init());
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_HIP(hipDeviceSynchronize());
   if (g_tm.ev.empty()) {
     g_tm.ev.resize(2 * 4096);
     for (auto &e : g_tm.ev) TFS_HIP(hipEventCreate(&e));
@@ -996,7 +1053,7 @@ extern "C" int tfs_timing_start(int kclass) {
 
 extern "C" int tfs_timing_stop(double *ms, long long *launches) {
   if (!g_tm.on) return TFS_ERR_INVALID;
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_HIP(hipDeviceSynchronize());
   tm_harvest();
   g_tm.on = false;
   if (ms) *ms = g_tm.ms;
@@ -1039,7 +1096,7 @@ int batch_alloc(tfs_batch *B, void **p, size_t bytes) { return dev_alloc(B->A, p
 Tiles tiles_of(const tfs_batch *B) { return Tiles{B->tile_sys, B->tile_beg, B->tile_end}; }
 
 void finish_seg(tfs_batch *B, int epi, KState *st, KState *all) {
-  hipLaunchKernelGGL(k_finish_seg, dim3(B->nsys), dim3(kRedThreads), 0, g_lib.stream,
+  hipLaunchKernelGGL(k_finish_seg, dim3(B->nsys), dim3(kRedThreads), 0, cur_stream(),
                      B->tile_first, B->partials, epi, st, all);
 }
 
@@ -1052,16 +1109,16 @@ void batch_cg_iteration(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KS
   if (prec == 2) dic_apply<T>(A, V.r, V.yL, V.zU, all);
   const int mode = prec == 2 ? 2 : prec == 1 ? 1 : 0;
   T *z = prec == 0 ? V.r : V.z;
-  hipLaunchKernelGGL(k_rho_seg<T>, dim3(nt), dim3(kRedThreads), 0, g_lib.stream, tl, V.r, V.z,
+  hipLaunchKernelGGL(k_rho_seg<T>, dim3(nt), dim3(kRedThreads), 0, cur_stream(), tl, V.r, V.z,
                      vals<T>(A).s_diag, V.zU, A->s_to_u, mode, st, all, B->partials);
   finish_seg(B, EPI_RHO, st, all);
-  hipLaunchKernelGGL(k_pupdate_seg<T>, dim3(nt), dim3(kRedThreads), 0, g_lib.stream, tl, z, V.p,
+  hipLaunchKernelGGL(k_pupdate_seg<T>, dim3(nt), dim3(kRedThreads), 0, cur_stream(), tl, z, V.p,
                      st, all);
   launch_spmv<T, SPMV_PLAIN>(A, V.p, V.q, nullptr, nullptr, nullptr, all);
-  hipLaunchKernelGGL(k_dot_seg<T>, dim3(nt), dim3(kRedThreads), 0, g_lib.stream, tl, V.p, V.q, st,
+  hipLaunchKernelGGL(k_dot_seg<T>, dim3(nt), dim3(kRedThreads), 0, cur_stream(), tl, V.p, V.q, st,
                      all, B->partials);
   finish_seg(B, EPI_ALPHA, st, all);
-  hipLaunchKernelGGL(k_axpy_norm_seg<T>, dim3(nt), dim3(kRedThreads), 0, g_lib.stream, tl, V.x,
+  hipLaunchKernelGGL(k_axpy_norm_seg<T>, dim3(nt), dim3(kRedThreads), 0, cur_stream(), tl, V.x,
                      V.r, V.p, V.q, st, all, B->partials);
   finish_seg(B, EPI_NORM, st, all);
 }
@@ -1087,9 +1144,9 @@ int batch_graph(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KState *al
   for (int attempt = 0; attempt < 2; ++attempt) {
     const bool was_persistent = B->A->persistent;
     hipGraph_t g = nullptr;
-    if (hipStreamBeginCapture(g_lib.stream, hipStreamCaptureModeRelaxed) != hipSuccess) break;
+    if (hipStreamBeginCapture(cur_stream(), hipStreamCaptureModeRelaxed) != hipSuccess) break;
     for (int k = 0; k < kGraphIters; ++k) batch_cg_iteration<T>(B, prec, V, st, all);
-    const hipError_t e1 = hipStreamEndCapture(g_lib.stream, &g);
+    const hipError_t e1 = hipStreamEndCapture(cur_stream(), &g);
     hipError_t e2 = hipErrorUnknown;
     if (e1 == hipSuccess && g) e2 = hipGraphInstantiate(&B->graph[slot], g, nullptr, nullptr, 0);
     if (g) (void)hipGraphDestroy(g);
@@ -1112,14 +1169,14 @@ int batch_graph(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KState *al
 
 int batch_read(tfs_batch *B) {
   TFS_HIP(hipMemcpyAsync(B->h_st, B->d_st, sizeof(KState) * (2 * B->nsys + 2),
-                         hipMemcpyDeviceToHost, g_lib.stream));
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+                         hipMemcpyDeviceToHost, cur_stream()));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   return 0;
 }
 
 int batch_write(tfs_batch *B) {
   TFS_HIP(hipMemcpyAsync(B->d_st, B->h_st, sizeof(KState) * (2 * B->nsys + 2),
-                         hipMemcpyHostToDevice, g_lib.stream));
+                         hipMemcpyHostToDevice, cur_stream()));
   return 0;
 }
 
@@ -1138,7 +1195,7 @@ int batch_iterate(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KState *
   while (!ha->done) {
     for (int k = 0; k < chunk; k += step) {
       if (exec) {
-        TFS_HIP(hipGraphLaunch(exec, g_lib.stream));
+        TFS_HIP(hipGraphLaunch(exec, cur_stream()));
         B->graph_launches++;
       } else {
         batch_cg_iteration<T>(B, prec, V, st, all);
@@ -1174,7 +1231,7 @@ int batch_in(tfs_batch *B, double *const *src, double *stage, double *dstS) {
   for (int s = 0; s < B->nsys; ++s)
     if (B->n_of[s])
       TFS_HIP(hipMemcpyAsync(stage + B->row_off[s], src[s], sizeof(double) * B->n_of[s],
-                             hipMemcpyDeviceToDevice, g_lib.stream));
+                             hipMemcpyDeviceToDevice, cur_stream()));
   gather(B->A->n, stage, B->A->s_orig, dstS);
   return 0;
 }
@@ -1183,7 +1240,7 @@ int batch_out(tfs_batch *B, const double *srcS, double *stage, double *const *ds
   for (int s = 0; s < B->nsys; ++s)
     if (B->n_of[s])
       TFS_HIP(hipMemcpyAsync(dst[s], stage + B->row_off[s], sizeof(double) * B->n_of[s],
-                             hipMemcpyDeviceToDevice, g_lib.stream));
+                             hipMemcpyDeviceToDevice, cur_stream()));
   return 0;
 }
 
@@ -1282,7 +1339,7 @@ extern "C" int tfs_batch_create(int nsys, const int *n, const long long *nnz,
     return rc;
   }
   if (hipHostMalloc((void **)&B->h_st, sizeof(KState) * (2 * nsys + 2), hipHostMallocDefault) !=
-      hipSuccess) {
+      hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) {
     tfs_batch_destroy(B);
     return TFS_ERR_NOMEM;
   }
@@ -1292,7 +1349,7 @@ extern "C" int tfs_batch_create(int nsys, const int *n, const long long *nnz,
 
 extern "C" int tfs_batch_destroy(tfs_batch *B) {
   if (!B) return TFS_OK;
-  if (g_lib.stream) (void)hipStreamSynchronize(g_lib.stream);
+  if (B->A && B->A->stream) (void)hipStreamSynchronize(B->A->stream);
   for (auto &g : B->graph)
     if (g) (void)hipGraphExecDestroy(g);
   if (B->h_st) (void)hipHostFree(B->h_st);
@@ -1302,10 +1359,12 @@ extern "C" int tfs_batch_destroy(tfs_batch *B) {
 }
 
 extern "C" int tfs_batch_set_values(tfs_batch *B, int sys, const double *val) {
+  if (!B) return TFS_ERR_INVALID;
+  StreamScope scope(B->A->stream);
   if (!B || sys < 0 || sys >= B->nsys || !val) return TFS_ERR_INVALID;
   TFS_HIP(hipMemcpyAsync(B->A->val + B->nnz_off[sys], val, sizeof(double) * B->nnz_of[sys],
-                         hipMemcpyHostToDevice, g_lib.stream));
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+                         hipMemcpyHostToDevice, cur_stream()));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   values_changed(B->A);
   return TFS_OK;
 }
@@ -1314,6 +1373,8 @@ extern "C" int tfs_batch_solve_device(tfs_batch *B, int prec, double *const *d_x
                                       double *const *d_r1, const int *cap, const double *tol,
                                       int *niter, double *ini_res, double *fin_res,
                                       int *status) {
+  if (!B) return TFS_ERR_INVALID;
+  StreamScope scope(B->A->stream);
   if (!B || !d_x || !d_r1 || !cap || !tol || !niter || !ini_res || !fin_res || !status)
     return TFS_ERR_INVALID;
   if (prec < 0 || prec > 2) prec = 0;
@@ -1331,14 +1392,14 @@ extern "C" int tfs_batch_solve_device(tfs_batch *B, int prec, double *const *d_x
   const Tiles tl = tiles_of(B);
   // ||b|| and its early exit (per system); r = b - A x where not exited;
   // ||r|| and its early exit
-  hipLaunchKernelGGL(k_sumsq_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0, g_lib.stream, tl,
+  hipLaunchKernelGGL(k_sumsq_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0, cur_stream(), tl,
                      V.r, (int)EPI_BNRM, st, all, B->partials);
   finish_seg(B, EPI_BNRM, st, all);
   gather(n, V.r, (const int *)nullptr, V.q);
   launch_spmv<double, SPMV_RESID>(A, V.x, V.q, nullptr, nullptr, nullptr, all);
-  hipLaunchKernelGGL(k_select_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0, g_lib.stream,
+  hipLaunchKernelGGL(k_select_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0, cur_stream(),
                      tl, V.q, V.r, st, all);  // r = q for systems still running
-  hipLaunchKernelGGL(k_sumsq_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0, g_lib.stream, tl,
+  hipLaunchKernelGGL(k_sumsq_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0, cur_stream(), tl,
                      V.r, (int)EPI_INIRES, st, all, B->partials);
   finish_seg(B, EPI_INIRES, st, all);
   TFS_HIP(hipGetLastError());
@@ -1361,7 +1422,7 @@ extern "C" int tfs_batch_solve_device(tfs_batch *B, int prec, double *const *d_x
   TFS_TRY(batch_out(B, V.x, A->stage_x, d_x));
   TFS_TRY(batch_out(B, V.r, A->stage_r, d_r1));
   TFS_HIP(hipGetLastError());
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   TFS_TRY(check_sweep(A));
   int worst = TFS_OK;
   for (int s = 0; s < ns; ++s) {
@@ -1382,6 +1443,8 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
                                             double *const *d_r1, int cap, const double *tol,
                                             double inner_rtol, int *steps, int *inner_iters,
                                             double *ini_res, double *fin_res, int *status) {
+  if (!B) return TFS_ERR_INVALID;
+  StreamScope scope(B->A->stream);
   if (!B || !d_x || !d_r1 || !tol || !steps || !inner_iters || !ini_res || !fin_res || !status)
     return TFS_ERR_INVALID;
   if (prec < 0 || prec > 2) prec = 0;
@@ -1408,7 +1471,7 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
     TFS_TRY(batch_write(B));
     gather(n, D.z, (const int *)nullptr, D.r);
     launch_spmv<double, SPMV_RESID>(A, D.x, D.r, nullptr, nullptr, nullptr, nullptr);
-    hipLaunchKernelGGL(k_sumsq_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0, g_lib.stream,
+    hipLaunchKernelGGL(k_sumsq_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0, cur_stream(),
                        tl, D.r, (int)EPI_BNRM, so, allo, B->partials);
     finish_seg(B, EPI_BNRM, so, allo);
     TFS_HIP(hipGetLastError());
@@ -1442,8 +1505,8 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
     hi[2 * ns].ctr[0] = (unsigned)ndone;
     TFS_TRY(batch_write(B));
     gather(n, D.r, (const int *)nullptr, F.r);
-    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, F.x, 0.0f);
-    hipLaunchKernelGGL(k_sumsq_seg<float>, dim3(B->ntiles), dim3(kRedThreads), 0, g_lib.stream,
+    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.x, 0.0f);
+    hipLaunchKernelGGL(k_sumsq_seg<float>, dim3(B->ntiles), dim3(kRedThreads), 0, cur_stream(),
                        tl, F.r, (int)EPI_INIRES, si, alli, B->partials);
     finish_seg(B, EPI_INIRES, si, alli);
     TFS_HIP(hipGetLastError());
@@ -1455,7 +1518,7 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
         inner_iters[s] += std::min(hi[s].niter_out, 100000);
         if (hi[s].status == kStBreakdown) st_out[s] = TFS_ERR_BREAKDOWN;
       }
-    hipLaunchKernelGGL(k_add_correction_all, dim3(g), dim3(kRedThreads), 0, g_lib.stream, n, D.x,
+    hipLaunchKernelGGL(k_add_correction_all, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x,
                        F.x);
     TFS_TRY(outer_residual());
     for (int s = 0; s < ns; ++s) {
@@ -1470,7 +1533,7 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
   TFS_TRY(batch_out(B, D.x, A->stage_x, d_x));
   TFS_TRY(batch_out(B, D.r, A->stage_r, d_r1));
   TFS_HIP(hipGetLastError());
-  TFS_HIP(hipStreamSynchronize(g_lib.stream));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
   TFS_TRY(check_sweep(A));
   int worst = TFS_OK;
   for (int s = 0; s < ns; ++s) {
@@ -1491,5 +1554,21 @@ extern "C" int tfs_batch_get_info(const tfs_batch *B, int *nsys, int *n_total,
   if (n_total) *n_total = B->A->n;
   if (nnz_total) *nnz_total = B->A->nnz;
   if (graph_launches) *graph_launches = B->graph_launches;
+  return TFS_OK;
+}
+
+extern "C" int tfs_matrix_set_sweep_schedule(tfs_matrix *A, int schedule) {
+  if (!A || schedule < TFS_SWEEP_LEVELS || schedule > TFS_SWEEP_DYNAMIC) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
+  TFS_HIP(hipStreamSynchronize(A->stream));
+  const bool was_levels = !A->persistent;
+  A->persistent = schedule != TFS_SWEEP_LEVELS;
+  A->dynamic = schedule == TFS_SWEEP_DYNAMIC;
+  if (was_levels && A->persistent) {
+    // the persistent sweeps need their outputs armed
+    if (A->vec64) arm(A->n, A->vec64 + 2 * (size_t)A->n);
+    if (A->vec32) arm(A->n, A->vec32 + 2 * (size_t)A->n);
+    TFS_HIP(hipStreamSynchronize(A->stream));
+  }
   return TFS_OK;
 }

@@ -28,6 +28,7 @@ from . import _native
 
 TFS_OK = 0
 TFS_NOT_CONVERGED = 1
+TFS_CANCELLED = 2
 TFS_ERR_INVALID = -1
 TFS_ERR_NOMEM = -2
 TFS_ERR_BREAKDOWN = -3
@@ -410,3 +411,83 @@ class Batch:
                                                       ini.ctypes.data, fin.ctypes.data,
                                                       st.ctypes.data), "tfs_batch_solve_mixed")
         return steps, inner, ini, fin, st
+
+
+class Server:
+    """Continuous-batching request server (tfs_server_*): host numpy in/out."""
+
+    def __init__(self, hbm_budget_bytes, workers=4):
+        lib = _lib()
+        if not getattr(lib, "_tfs_server_typed", False):
+            LLP = ctypes.POINTER(ctypes.c_longlong)
+            lib.tfs_server_create.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(_VP)]
+            lib.tfs_server_destroy.argtypes = [_VP]
+            lib.tfs_server_add_case.argtypes = [_VP, ctypes.c_int, ctypes.c_longlong, _I32P, _I32P,
+                                                _I32P, _F64P, ctypes.c_int, _IP]
+            lib.tfs_server_submit.argtypes = [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _F64P,
+                                              ctypes.c_double, ctypes.c_int, ctypes.c_int, LLP]
+            lib.tfs_server_wait.argtypes = [_VP, ctypes.c_longlong, _VP, _IP, _DP, _DP, _IP, _DP,
+                                            _DP, _DP]
+            lib.tfs_server_stats.argtypes = [_VP, LLP, LLP, _IP, LLP]
+            lib.tfs_server_cancel.argtypes = [_VP, ctypes.c_longlong]
+            lib._tfs_server_typed = True
+        self._lib = lib
+        self._n = {}
+        h = _VP()
+        _check(lib.tfs_server_create(int(hbm_budget_bytes), int(workers), ctypes.byref(h)),
+               "tfs_server_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.tfs_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_case(self, row, col, dia, val, base=0):
+        row = np.ascontiguousarray(row, np.int32)
+        n = len(row) - 1
+        cid = ctypes.c_int()
+        _check(self._lib.tfs_server_add_case(self._h, n, int(row[-1]) - base, row,
+                                             np.ascontiguousarray(col, np.int32),
+                                             np.ascontiguousarray(dia, np.int32),
+                                             np.ascontiguousarray(val, np.float64), base,
+                                             ctypes.byref(cid)), "tfs_server_add_case")
+        self._n[cid.value] = n
+        return cid.value
+
+    def submit(self, case_id, method, prec, b, tol, cap, release=False):
+        req = ctypes.c_longlong()
+        _check(self._lib.tfs_server_submit(self._h, case_id, _METHODS.get(method, method), prec,
+                                           np.ascontiguousarray(b, np.float64), float(tol),
+                                           int(cap), int(release), ctypes.byref(req)),
+               "tfs_server_submit")
+        return (req.value, self._n[case_id])
+
+    def wait(self, handle, want_x=True):
+        req, n = handle
+        x = np.zeros(n) if want_x else None
+        it, st = ctypes.c_int(), ctypes.c_int()
+        ir, fr, t0, t1, t2 = (ctypes.c_double() for _ in range(5))
+        _check(self._lib.tfs_server_wait(self._h, req, x.ctypes.data if want_x else None,
+                                         ctypes.byref(it), ctypes.byref(ir), ctypes.byref(fr),
+                                         ctypes.byref(st), ctypes.byref(t0), ctypes.byref(t1),
+                                         ctypes.byref(t2)), "tfs_server_wait")
+        return dict(x=x, niter=it.value, ini_res=ir.value, fin_res=fr.value, status=st.value,
+                    t_submit=t0.value, t_start=t1.value, t_end=t2.value)
+
+    def cancel(self, handle):
+        """True if the request was withdrawn before it started."""
+        return self._lib.tfs_server_cancel(self._h, handle[0]) == 0
+
+    def stats(self):
+        a, b, d = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        c = ctypes.c_int()
+        self._lib.tfs_server_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                   ctypes.byref(d))
+        return dict(hbm_in_use=a.value, hbm_peak=b.value, resident=c.value, completed=d.value)

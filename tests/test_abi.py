@@ -68,3 +68,15 @@ def test_invalid_structure_rejected_without_gpu():
     assert create([0, 2, 4], [0, 2, 0, 1], [0, 3]) == -1   # column out of range
     assert create([1, 3, 5], [1, 2, 1, 2], [1, 3], base=1) == -1  # 1-based, row 2 dia wrong
     assert create([0, 1, 1], [0], [0, 0]) == -1             # empty row (no diagonal)
+
+
+def test_server_arguments_rejected_without_gpu():
+    """tfs_server_create checks its arguments before it looks for a device."""
+    lib = ctypes.CDLL(str(HEADERS["tfs_solver.h"]))
+    p = ctypes.c_void_p()
+    lib.tfs_server_create.argtypes = [ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p]
+    assert lib.tfs_server_create(0, 1, ctypes.byref(p)) == -1
+    assert lib.tfs_server_create(1 << 30, 0, ctypes.byref(p)) == -1
+    assert lib.tfs_server_create(1 << 30, 1, None) == -1
+    lib.tfs_server_cancel.argtypes = [ctypes.c_void_p, ctypes.c_longlong]
+    assert lib.tfs_server_cancel(None, 1) == -1

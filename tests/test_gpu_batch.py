@@ -1,9 +1,11 @@
 """Batched solves (many independent systems in one block-diagonal matrix).
 
-Each system of the batch must give what it gives alone: the same iteration
-count and the same solution (bitwise for the FP64 Cg path -- every kernel's
-arithmetic per row and the per-system reductions are independent of the
-other systems), and, against the oracle, the parity bar of the single solve.
+Each system of the batch must give what it gives in a batch of its own: the
+same iteration count and, bitwise, the same solution (every kernel's
+arithmetic per row and the per-system reductions, in tile order, are
+independent of the other systems).  Against the single-matrix solve (whose
+dot products are summed in another order) and the oracle it meets the parity
+bar of the single solve.
 """
 import numpy as np
 import pytest
@@ -47,13 +49,19 @@ def test_batch_equals_single_solves(torch, S, oracle, prec):
     niter, ini, fin, st = B.solve(xs, rs, prec, [3000] * len(sysl), tols)
     print("batch niter", niter.tolist(), "info", B.info(), flush=True)
     for k, c in enumerate(sysl):
+        B1 = S.Batch([(c.row, c.col, c.dia, c.val)])
+        xb = [torch.zeros(c.n, dtype=torch.float64, device="cuda")]
+        rb = [torch.from_numpy(c.b.copy()).cuda()]
+        nb, inib, finb, stb = B1.solve(xb, rb, prec, [3000], [tols[k]])
+        assert niter[k] == nb[0] and st[k] == S.TFS_OK and stb[0] == S.TFS_OK
+        assert np.array_equal(xs[k].cpu().numpy(), xb[0].cpu().numpy()), k
+        assert fin[k] == finb[0] and ini[k] == inib[0]
         A = S.Matrix(c.row, c.col, c.dia, c.val)
         x1 = torch.zeros(c.n, dtype=torch.float64, device="cuda")
         r1 = torch.from_numpy(c.b.copy()).cuda()
         it1, ini1, fin1 = S.Cg(A, x1, r1, prec, 3000, tols[k])
-        assert niter[k] == it1 and st[k] == S.TFS_OK
-        assert np.array_equal(xs[k].cpu().numpy(), x1.cpu().numpy())
-        assert fin[k] == fin1 and ini[k] == ini1
+        assert abs(int(niter[k]) - it1) <= max(1, it1 // 50)
+        assert rel_l2(xs[k].cpu().numpy(), x1.cpu().numpy()) <= 1e-10
         ref = oracle.solve("cg", c.row, c.col, c.dia, c.val, c.b, prec=prec, niter=3000,
                            tol=tols[k])
         assert abs(niter[k] - ref["niter"]) <= max(1, ref["niter"] // 20)
