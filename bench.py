@@ -33,7 +33,6 @@ import json
 import os
 import pathlib
 import sys
-import threading
 import time
 
 ROOT = pathlib.Path(__file__).resolve().parent
@@ -501,12 +500,24 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB, 5 times (the "
+                         "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               f"(the solver path runs on a single MI355X)", file=sys.stderr)
         sys.exit(2)
     import torch  # noqa: F401  (loads the HIP runtime the library shares)
+    if args.copy_mib:
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device copy", "bytes_read": src.numel() * 4,
+                          "bytes_written": src.numel() * 4, "launches": 5}), flush=True)
+        return
     load_package()
     if args.workload in (1, 2):
         out = single_solve(args, args.workload)

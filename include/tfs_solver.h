@@ -95,6 +95,7 @@ typedef struct {
   int levels_upper;     /* depth of the backward-sweep DAG                   */
   int row_blocks;       /* SpMV row blocks (wavefront work units)            */
   long long device_bytes;
+  int inner_colours;    /* colours of the inner order (0 until first built)  */
 } tfs_matrix_info;
 int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info);
 
@@ -119,6 +120,17 @@ int tfs_solve_device(tfs_matrix *A, int method, int prec, double *d_x, double *d
 int tfs_solve_mixed_device(tfs_matrix *A, int prec, double *d_x, double *d_r1, int *niter,
                            int *inner_iters, double tol, double inner_rtol, double *ini_res,
                            double *fin_res);
+/* Row order of the FP32 inner solves (the outer FP64 loop is unaffected):
+ *   TFS_INNER_REFERENCE  the caller's order: the inner factor is the
+ *                        reference's, its sweeps have the caller DAG's depth;
+ *   TFS_INNER_COLOURED   (default) a greedy colouring of the pattern: rows of
+ *                        one colour are independent, so each sweep has one
+ *                        level per colour (2 for a hex mesh in lexicographic
+ *                        order).  The inner preconditioner is then the same
+ *                        incomplete Cholesky formula on the permuted matrix;
+ *                        the refinement still converges to the FP64 tol. */
+enum { TFS_INNER_REFERENCE = 0, TFS_INNER_COLOURED = 1 };
+int tfs_matrix_set_inner_order(tfs_matrix *A, int order);
 
 /* ---- batched solves ----------------------------------------------------------------
  * Not in the reference (its callers solve one system at a time): many

@@ -252,8 +252,12 @@ k_fill(int n, T *__restrict__ x, T v) {
 
 // Iterative refinement: x += e (FP64 update with the FP32 correction).
 __global__ void __launch_bounds__(kRedThreads)
-k_add_correction(int n, double *__restrict__ x, const float *__restrict__ e) {
-  TFS_GRID_STRIDE(i, n) x[i] = x[i] + (double)e[i];
+k_add_correction(int n, double *__restrict__ x, const float *__restrict__ e,
+                 const int *__restrict__ idx) {
+  TFS_GRID_STRIDE(i, n) {
+    const int j = idx ? idx[i] : i;  // idx: a permutation, so no two lanes share a j
+    x[j] = x[j] + (double)e[i];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -418,7 +422,23 @@ k_spmv(int nblk, const int *__restrict__ rblk, const int *__restrict__ row,
     T yr = T(0);
     int r = -1;
     if (e1 - e0 <= kSpmvCap) {
-      for (int e = e0 + lane; e < e1; e += kWave) lds[e - e0] = val[e] * x[col[e]];
+      // all of the block's col/val loads are issued before the first x
+      // gather (kSpmvCap / kWave per lane), so each wave keeps up to 8
+      // independent loads in flight instead of one dependent pair
+      constexpr int kPer = kSpmvCap / kWave;
+      int ci[kPer];
+      T vi[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = e0 + lane + u * kWave;
+        ci[u] = e < e1 ? col[e] : 0;
+        vi[u] = e < e1 ? val[e] : T(0);
+      }
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = e0 + lane + u * kWave;
+        if (e < e1) lds[e - e0] = vi[u] * x[ci[u]];
+      }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -437,14 +457,30 @@ k_spmv(int nblk, const int *__restrict__ rblk, const int *__restrict__ row,
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     } else {
-      // one long row for the whole wave (tree order; rows this long do not
-      // occur in finite-volume stencils)
-      T acc = T(0);
-      for (int e = e0 + lane; e < e1; e += kWave) acc += val[e] * x[col[e]];
-      acc = wave_sum(acc);
+      // one row longer than the block capacity: the wave forms its products
+      // kSpmvCap at a time, lane 0 adds them in the row's order (as short
+      // rows are added, so the result is the same)
+      T acc = MODE == SPMV_RESID ? y[r0] : T(0);
+      for (int c0 = e0; c0 < e1; c0 += kSpmvCap) {
+        const int c1 = min(e1, c0 + kSpmvCap);
+        for (int e = c0 + lane; e < c1; e += kWave) lds[e - c0] = val[e] * x[col[e]];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane == 0) {
+          if (MODE == SPMV_RESID) {
+            for (int k = 0; k < c1 - c0; ++k) acc = acc - lds[k];
+          } else {
+            for (int k = 0; k < c1 - c0; ++k) acc = acc + lds[k];
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
       if (lane == 0) {
         r = r0;
-        yr = MODE == SPMV_RESID ? y[r] - acc : acc;
+        yr = acc;
       }
     }
     if (r >= 0) {
@@ -610,9 +646,14 @@ k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, i
         s = s / (dp + tiny<T>());
       }
     }
+    // A row's value is stored INSIDE the polling loop, in the iteration in
+    // which its last dependency arrives: a chunk can hold two levels, so a
+    // row may wait on another lane of its own wavefront, and a lane that left
+    // the loop would only run its store once every lane of the wave had left.
     int used = 0;  // entries folded into s so far, in order
     unsigned int spins = 0;
-    for (;;) {
+    bool done = false;
+    while (!done) {
       bool blocked = false;
 #pragma unroll
       for (int k = 0; k < kPrefetch; ++k) {
@@ -636,14 +677,20 @@ k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, i
           used = e - e0 + 1;
         }
       }
-      if (!blocked) break;
-      if (++spins > kSpinLimit) {
-        *err = 1;
-        break;
+      if (!blocked) {
+        coherent_store(&out[p], OP == OP_FACTOR ? T(1) / s : s * dp);
+        done = true;
+      } else if (++spins > kSpinLimit ||
+                 ((spins & 255u) == 0 && coherent_load(err) != 0)) {
+        // out of patience, or another row already gave up: record the error
+        // and store a value so that this row's dependants finish too
+        coherent_store(err, 1);
+        coherent_store(&out[p], OP == OP_FACTOR ? T(1) / s : s * dp);
+        done = true;
+      } else {
+        __builtin_amdgcn_s_sleep(2);
       }
-      __builtin_amdgcn_s_sleep(2);
     }
-    coherent_store(&out[p], OP == OP_FACTOR ? T(1) / s : s * dp);
   }
   if (DYN && threadIdx.x == 0) {
     const unsigned int prev =

@@ -167,6 +167,46 @@ void build_sweep(int n, const int *row, const int *col, const int *dia, bool low
 
 }  // namespace
 
+int colour_order(int n, const int *row, const int *col, std::vector<int> &perm) {
+  // symmetrised adjacency: the pattern may be unsymmetric
+  std::vector<int> deg(n + 1, 0);
+  for (int i = 0; i < n; ++i)
+    for (int k = row[i]; k < row[i + 1]; ++k)
+      if (col[k] != i) {
+        deg[i + 1]++;
+        deg[col[k] + 1]++;
+      }
+  for (int i = 0; i < n; ++i) deg[i + 1] += deg[i];
+  std::vector<int> adj(deg[n]), fill(deg.begin(), deg.end() - 1);
+  for (int i = 0; i < n; ++i)
+    for (int k = row[i]; k < row[i + 1]; ++k)
+      if (col[k] != i) {
+        adj[fill[i]++] = col[k];
+        adj[fill[col[k]]++] = i;
+      }
+  std::vector<int> colour(n, -1), seen;  // seen[c] == i: colour c taken by a neighbour of i
+  int ncol = 0;
+  for (int i = 0; i < n; ++i) {
+    for (int k = deg[i]; k < deg[i + 1]; ++k) {
+      const int c = colour[adj[k]];
+      if (c >= 0) seen[c] = i;
+    }
+    int c = 0;
+    while (c < ncol && seen[c] == i) ++c;
+    if (c == ncol) {
+      ++ncol;
+      seen.push_back(-1);
+    }
+    colour[i] = c;
+  }
+  std::vector<int> start(ncol + 1, 0);
+  for (int i = 0; i < n; ++i) start[colour[i] + 1]++;
+  for (int c = 0; c < ncol; ++c) start[c + 1] += start[c];
+  perm.assign(n, 0);
+  for (int i = 0; i < n; ++i) perm[start[colour[i]]++] = i;
+  return ncol;
+}
+
 void build_layout(int n, const int *row, const int *col, const int *dia, int spmv_rows,
                   int spmv_cap, int small_level, Layout &out) {
   out.n = n;

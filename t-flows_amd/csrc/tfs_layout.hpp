@@ -51,6 +51,14 @@ struct Layout {
   int bandwidth = 0;        // max |S(i) - S(j)| over entries (diagnostic)
 };
 
+// Greedy colouring of the (symmetrised) pattern, vertices visited in index
+// order, each taking the smallest colour none of its neighbours has.
+// `perm` lists the vertices by colour, index order within a colour (new
+// position -> vertex); returns the number of colours.  Rows of one colour
+// are independent, so an incomplete factor in this order has one level per
+// colour.
+int colour_order(int n, const int *row, const int *col, std::vector<int> &perm);
+
 // row/col/dia are 0-based, validated (diagonal present, columns ascending).
 void build_layout(int n, const int *row, const int *col, const int *dia, int spmv_rows,
                   int spmv_cap, int small_level, Layout &out);

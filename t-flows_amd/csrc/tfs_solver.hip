@@ -160,9 +160,20 @@ struct tfs_matrix {
   long long bytes = 0;
   int bandwidth = 0;
   std::vector<void *> allocs;
+  // FP32 inner solves of the refinement on a colour-ordered copy P A P^T,
+  // P a greedy colouring of the S-order graph (built on first use)
+  int inner_order = TFS_INNER_COLOURED;
+  tfs_matrix *inner = nullptr;
+  int *inner_emap = nullptr;     // inner's caller-order entry k <- entry of val
+  int *inner_from_s = nullptr;   // S position of this matrix for inner's S position i
+  bool inner_ready = false;      // inner's values match val
+  int inner_colours = 0;
 };
 
 namespace {
+
+int matrix_create(int n, long long nnz, const int *row, const int *col, const int *dia, int base,
+                  tfs_matrix **out, Layout *keep);
 
 int dev_alloc(tfs_matrix *A, void **p, size_t bytes) {
   if (bytes == 0) bytes = 8;
@@ -316,6 +327,7 @@ int prepare_values(tfs_matrix *A) {
 void values_changed(tfs_matrix *A) {
   A->v64.ready = A->v32.ready = false;
   A->v64.factor_prec = A->v32.factor_prec = -1;
+  A->inner_ready = false;
 }
 
 template <typename T>
@@ -563,6 +575,62 @@ int solve_device(tfs_matrix *A, int method, int prec, double *x, double *r1, int
   return status_of(h);
 }
 
+// The colour-ordered copy used by the FP32 inner solves: P A P^T with P a
+// greedy colouring of this matrix's S-order graph, built from the device
+// copy of the S layout (the caller's arrays are not kept).  Its own layouts
+// follow (RCM for its SpMV; L/U levels of the colour order: one per colour).
+int build_inner(tfs_matrix *A) {
+  if (A->inner) return 0;
+  const int n = A->n;
+  const long long nnz = A->nnz;
+  std::vector<int> row(n + 1), col((size_t)nnz), src((size_t)nnz);
+  TFS_HIP(hipMemcpyAsync(row.data(), A->s_row, sizeof(int) * (n + 1), hipMemcpyDeviceToHost,
+                         cur_stream()));
+  TFS_HIP(hipMemcpyAsync(col.data(), A->s_col, sizeof(int) * nnz, hipMemcpyDeviceToHost,
+                         cur_stream()));
+  TFS_HIP(hipMemcpyAsync(src.data(), A->s_src, sizeof(int) * nnz, hipMemcpyDeviceToHost,
+                         cur_stream()));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
+  std::vector<int> perm;
+  const int ncol = colour_order(n, row.data(), col.data(), perm);
+  std::vector<int> pos(n);
+  for (int i = 0; i < n; ++i) pos[perm[i]] = i;
+  std::vector<int> prow(n + 1, 0), pcol((size_t)nnz), pdia(n, -1), emap((size_t)nnz);
+  std::vector<std::pair<int, int>> ent;
+  long long q = 0;
+  for (int i = 0; i < n; ++i) {
+    const int s = perm[i];
+    ent.clear();
+    for (int k = row[s]; k < row[s + 1]; ++k) ent.emplace_back(pos[col[k]], src[k]);
+    std::sort(ent.begin(), ent.end());
+    for (const auto &e : ent) {
+      if (e.first == i) pdia[i] = (int)q;
+      pcol[(size_t)q] = e.first;
+      emap[(size_t)q] = e.second;
+      ++q;
+    }
+    prow[i + 1] = (int)q;
+  }
+  Layout Ly;
+  tfs_matrix *C = nullptr;
+  TFS_TRY(matrix_create(n, nnz, prow.data(), pcol.data(), pdia.data(), 0, &C, &Ly));
+  std::vector<int> from_s(n);
+  for (int i = 0; i < n; ++i) from_s[i] = perm[Ly.s_orig[i]];
+  int rc = upload(A, &A->inner_emap, emap);
+  if (rc == 0) rc = upload(A, &A->inner_from_s, from_s);
+  if (rc < 0) {
+    tfs_matrix_destroy(C);
+    return rc;
+  }
+  C->persistent = A->persistent;
+  C->dynamic = A->dynamic;
+  A->inner = C;
+  A->inner_colours = ncol;
+  A->inner_ready = false;
+  A->bytes += C->bytes;
+  return 0;
+}
+
 // FP32-inner PCG inside FP64 iterative refinement (defect correction).
 //   r = b - A x (FP64);  repeat: solve A e = r in FP32 (PCG with the factor
 //   recomputed in FP32 from FP32 values) to ||r_in|| < inner_rtol ||r||;
@@ -582,13 +650,28 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
   }
   if (!x || !r1) return TFS_ERR_INVALID;
   inner_rtol = std::min(0.5, std::max(1e-6, inner_rtol));
+  // The inner solves run on I: this matrix, or its colour-ordered copy when
+  // the factor's order matters (prec 2).  The diagonal preconditioners do not
+  // depend on the order.
+  const bool coloured = A->inner_order == TFS_INNER_COLOURED && prec == 2;
+  tfs_matrix *I = A;
+  if (coloured) {
+    TFS_TRY(build_inner(A));
+    I = A->inner;
+    if (!A->inner_ready) {
+      gather((int)A->nnz, A->val, A->inner_emap, I->val);
+      values_changed(I);
+      A->inner_ready = true;
+    }
+  }
+  const int *to_s = coloured ? A->inner_from_s : nullptr;  // I's S position -> A's
   TFS_TRY(ensure_vec64(A, 7));
-  TFS_TRY(ensure_vec32(A, 7));
+  TFS_TRY(ensure_vec32(I, 7));
   TFS_TRY(prepare_values<double>(A));
-  TFS_TRY(prec_form<float>(A, prec));
+  TFS_TRY(prec_form<float>(I, prec));
   const Vecs<double> D = vecs_of(A->vec64, n);   // x, r (residual), z (b)
-  const Vecs<float> F = vecs_of(A->vec32, n);    // inner solve: x = correction e
-  KState *so = A->d_st + 1, *si = A->d_st;
+  const Vecs<float> F = vecs_of(I->vec32, n);    // inner solve: x = correction e
+  KState *so = A->d_st + 1, *si = I->d_st;
   const int g = vgrid(n);
   gather(n, x, A->s_orig, D.x);
   gather(n, r1, A->s_orig, D.z);  // b kept for the outer residuals
@@ -614,24 +697,24 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
       break;
     }
     ++outer;
-    // inner solve, FP32: e = 0, r32 = r
-    gather(n, D.r, (const int *)nullptr, F.r);
+    // inner solve, FP32: e = 0, r32 = r (in I's S order)
+    gather(n, D.r, to_s, F.r);
     hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.x, 0.0f);
-    TFS_TRY(init_state(A, 0, std::max(inner_rtol * err, 0.5 * tol), 100000, 0.0));
+    TFS_TRY(init_state(I, 0, std::max(inner_rtol * err, 0.5 * tol), 100000, 0.0));
     hipLaunchKernelGGL(k_sumsq<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.r,
-                       (int)EPI_INIRES, si, A->partials);
-    finish(g, A->partials, EPI_INIRES, si);
+                       (int)EPI_INIRES, si, I->partials);
+    finish(g, I->partials, EPI_INIRES, si);
     TFS_HIP(hipGetLastError());
-    TFS_TRY(read_state(A));
-    TFS_TRY(iterate(A, 100000, [&] { cg_iteration<float>(A, prec, F, si); }));
-    *inner_iters += std::min(A->h_st->niter_out, 100000);
-    if (A->h_st->status == kStBreakdown) {
+    TFS_TRY(read_state(I));
+    TFS_TRY(iterate(I, 100000, [&] { cg_iteration<float>(I, prec, F, si); }));
+    *inner_iters += std::min(I->h_st->niter_out, 100000);
+    if (I->h_st->status == kStBreakdown) {
       status = TFS_ERR_BREAKDOWN;
       break;
     }
     // x += e; r = b - A x; ||r||
     hipLaunchKernelGGL(k_add_correction, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x,
-                       F.x);
+                       F.x, to_s);
     TFS_TRY(init_state(A, 1, tol, cap, 0.0));
     outer_residual();
     TFS_HIP(hipGetLastError());
@@ -654,6 +737,7 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
   TFS_HIP(hipGetLastError());
   TFS_HIP(hipStreamSynchronize(cur_stream()));
   TFS_TRY(check_sweep(A));
+  if (I != A) TFS_TRY(check_sweep(I));
   *ini_res = err0;
   *fin_res = err;
   *niter = status == TFS_NOT_CONVERGED ? cap + 1 : outer;
@@ -822,6 +906,7 @@ extern "C" int tfs_matrix_create(int n, long long nnz, const int *row, const int
 
 extern "C" int tfs_matrix_destroy(tfs_matrix *A) {
   if (!A) return TFS_OK;
+  if (A->inner) tfs_matrix_destroy(A->inner);
   if (A->stream) (void)hipStreamSynchronize(A->stream);
   for (void *p : A->allocs) (void)hipFree(p);
   if (A->h_st) (void)hipHostFree(A->h_st);
@@ -870,6 +955,13 @@ extern "C" int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info) {
   info->levels_upper = A->U.nlev;
   info->row_blocks = A->nblk;
   info->device_bytes = A->bytes;
+  info->inner_colours = A->inner_colours;
+  return TFS_OK;
+}
+
+extern "C" int tfs_matrix_set_inner_order(tfs_matrix *A, int order) {
+  if (!A || (order != TFS_INNER_REFERENCE && order != TFS_INNER_COLOURED)) return TFS_ERR_INVALID;
+  A->inner_order = order;
   return TFS_OK;
 }
 
@@ -1570,5 +1662,6 @@ extern "C" int tfs_matrix_set_sweep_schedule(tfs_matrix *A, int schedule) {
     if (A->vec32) arm(A->n, A->vec32 + 2 * (size_t)A->n);
     TFS_HIP(hipStreamSynchronize(A->stream));
   }
+  if (A->inner) return tfs_matrix_set_sweep_schedule(A->inner, schedule);
   return TFS_OK;
 }

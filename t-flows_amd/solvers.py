@@ -36,6 +36,7 @@ TFS_ERR_DEVICE = -4
 TFS_ERR_NO_DEVICE = -5
 
 CG, BICG, CGS = 0, 1, 2
+TFS_INNER_REFERENCE, TFS_INNER_COLOURED = 0, 1
 _METHODS = {"cg": CG, "bicg": BICG, "cgs": CGS}
 
 _I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -54,7 +55,8 @@ class SolverError(RuntimeError):
 class _Info(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("nnz", ctypes.c_longlong),
                 ("levels_lower", ctypes.c_int), ("levels_upper", ctypes.c_int),
-                ("row_blocks", ctypes.c_int), ("device_bytes", ctypes.c_longlong)]
+                ("row_blocks", ctypes.c_int), ("device_bytes", ctypes.c_longlong),
+                ("inner_colours", ctypes.c_int)]
 
 
 def _lib():
@@ -89,6 +91,8 @@ def _lib():
     lib.tfs_solve_mixed_device.argtypes = [_VP, ctypes.c_int, _VP, _VP, _IP, _IP,
                                            ctypes.c_double, ctypes.c_double, _DP, _DP]
     lib.tfs_prec_diag32_device.argtypes = [_VP, ctypes.c_int, _VP]
+    lib.tfs_matrix_set_inner_order.argtypes = [_VP, ctypes.c_int]
+    lib.tfs_matrix_set_sweep_schedule.argtypes = [_VP, ctypes.c_int]
     lib.tfs_timing_start.argtypes = [ctypes.c_int]
     lib.tfs_timing_stop.argtypes = [_DP, ctypes.POINTER(ctypes.c_longlong)]
     lib._tfs_typed = True
@@ -167,7 +171,12 @@ class Matrix:
         _check(self._lib.tfs_matrix_get_info(self._h, ctypes.byref(i)), "tfs_matrix_get_info")
         return dict(n=i.n, nnz=i.nnz, levels_lower=i.levels_lower,
                     levels_upper=i.levels_upper, row_blocks=i.row_blocks,
-                    device_bytes=i.device_bytes)
+                    device_bytes=i.device_bytes, inner_colours=i.inner_colours)
+
+    def set_inner_order(self, order):
+        """TFS_INNER_REFERENCE or TFS_INNER_COLOURED for solve_mixed."""
+        _check(self._lib.tfs_matrix_set_inner_order(self._h, int(order)),
+               "tfs_matrix_set_inner_order")
 
     def set_values(self, val):
         if _is_device_tensor(val):

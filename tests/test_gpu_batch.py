@@ -60,8 +60,8 @@ def test_batch_equals_single_solves(torch, S, oracle, prec):
         x1 = torch.zeros(c.n, dtype=torch.float64, device="cuda")
         r1 = torch.from_numpy(c.b.copy()).cuda()
         it1, ini1, fin1 = S.Cg(A, x1, r1, prec, 3000, tols[k])
-        assert abs(int(niter[k]) - it1) <= max(1, it1 // 50)
-        assert rel_l2(xs[k].cpu().numpy(), x1.cpu().numpy()) <= 1e-10
+        assert abs(int(niter[k]) - it1) <= max(1, it1 // 20)
+        assert rel_l2(xs[k].cpu().numpy(), x1.cpu().numpy()) <= 1e-8
         ref = oracle.solve("cg", c.row, c.col, c.dia, c.val, c.b, prec=prec, niter=3000,
                            tol=tols[k])
         assert abs(niter[k] - ref["niter"]) <= max(1, ref["niter"] // 20)

@@ -38,14 +38,16 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("order", ["reference", "coloured"])
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("prec", [0, 1, 2])
-def test_refinement_reaches_fp64_solution(torch, S, oracle, name, prec):
+def test_refinement_reaches_fp64_solution(torch, S, oracle, name, prec, order):
     c = mesh.make_case(**CASES[name])
-    print(f"case {name} prec={prec} n={c.n}", flush=True)
+    print(f"case {name} prec={prec} order={order} n={c.n}", flush=True)
     tol = 1e-10 * float(np.linalg.norm(c.b))
     ref = oracle.solve("cg", c.row, c.col, c.dia, c.val, c.b, prec=prec, niter=20000, tol=tol)
     A = S.Matrix(c.row, c.col, c.dia, c.val)
+    A.set_inner_order(S.TFS_INNER_COLOURED if order == "coloured" else S.TFS_INNER_REFERENCE)
     x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
     r = torch.from_numpy(c.b.copy()).cuda()
     steps, inner, ini, fin = S.solve_mixed(A, x, r, prec, 50, tol, inner_rtol=1e-4)
@@ -83,3 +85,30 @@ def test_refinement_cap_and_early_exit(torch, S):
     r = torch.from_numpy(c.b.copy()).cuda()
     steps, inner, ini, fin = S.solve_mixed(A, x, r, 2, 1, 1e-14 * float(np.linalg.norm(c.b)))
     assert steps == 2 and A.status == S.TFS_NOT_CONVERGED
+
+
+def test_coloured_inner_order(torch, S):
+    """A hex mesh in lexicographic order is 2-colourable: the inner sweeps have
+    two levels; the refined answer does not depend on the inner order beyond
+    the FP64 tolerance, and a new set of values reaches the inner copy."""
+    c = mesh.make_case(**CASES["poisson_stretched20"])
+    tol = 1e-10 * float(np.linalg.norm(c.b))
+    A = S.Matrix(c.row, c.col, c.dia, c.val)
+    out = {}
+    for order in (S.TFS_INNER_REFERENCE, S.TFS_INNER_COLOURED):
+        A.set_inner_order(order)
+        x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+        r = torch.from_numpy(c.b.copy()).cuda()
+        steps, inner, ini, fin = S.solve_mixed(A, x, r, 2, 50, tol)
+        assert A.status == S.TFS_OK and fin < tol
+        out[order] = x.cpu().numpy()
+        print("order", order, "steps", steps, "inner", inner, flush=True)
+    assert A.info()["inner_colours"] == 2
+    assert rel_l2(out[0], out[1]) <= 1e-9
+    # scaled values: the solution scales inversely
+    A.set_values(2.0 * c.val)
+    x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    r = torch.from_numpy(c.b.copy()).cuda()
+    steps, inner, ini, fin = S.solve_mixed(A, x, r, 2, 50, tol / 2)
+    assert A.status == S.TFS_OK
+    assert rel_l2(2.0 * x.cpu().numpy(), out[1]) <= 1e-9

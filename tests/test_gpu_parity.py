@@ -83,6 +83,41 @@ def iters_close_cgs(oracle, c, prec, tol, a, b):
     return lo - max(1, int(0.05 * lo)) <= a <= hi + max(1, int(0.05 * hi))
 
 
+def test_spmv_long_rows_bitwise(torch, S, oracle):
+    """Rows longer than one SpMV block's staging capacity (an 'arrow' row
+    coupled to every cell, and one of 1300 entries) keep the reference's
+    left-to-right sum."""
+    n = 1400
+    rows = []
+    for i in range(n):
+        cols = {i, 0}
+        if i > 0:
+            cols |= {i - 1}
+        if i + 1 < n:
+            cols |= {i + 1}
+        if i == 0:
+            cols |= set(range(n))
+        if i == 7:
+            cols |= set(range(100, 1400))
+        rows.append(sorted(cols))
+    row = np.zeros(n + 1, np.int32)
+    row[1:] = np.cumsum([len(r) for r in rows])
+    col = np.concatenate([np.array(r, np.int32) for r in rows])
+    dia = np.array([row[i] + rows[i].index(i) for i in range(n)], np.int32)
+    rng = np.random.default_rng(5)
+    val = rng.uniform(-1.0, 1.0, col.size)
+    val[dia] = 2000.0
+    A = S.Matrix(row, col, dia, val)
+    x = rng.uniform(-1.0, 1.0, n)
+    b = rng.uniform(-1.0, 1.0, n)
+    y = torch.zeros(n, dtype=torch.float64, device="cuda")
+    A.spmv(dev(torch, x), y)
+    assert np.array_equal(host(y), oracle.spmv(row, col, dia, val, x))
+    r = dev(torch, b)
+    A.residual(dev(torch, x), r)
+    assert np.array_equal(host(r), oracle.residual(row, col, dia, val, x, b))
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_spmv_and_residual_bitwise(torch, S, oracle, name):
     c = get_case(name)
