@@ -87,6 +87,19 @@ double tfs_case_dt(const tfs_case *c);
 /* Largest cell aspect ratio (max edge / min edge over the cell's 12 edges). */
 double tfs_case_max_aspect(const tfs_case *c);
 
+/* ---- the solver's host-side layouts (no device; for inspection and tests) ----
+ * Greedy colouring of the symmetrised pattern in index order: colour[i] per
+ * row, perm = rows sorted by colour (index order within a colour); returns
+ * the number of colours.  The FP32 inner solves of the refinement run in
+ * this order (S-order rows as input there). */
+int tfs_layout_colour(int n, const int *row, const int *col, int *perm, int *colour);
+/* The layouts the solver builds for a matrix (0-based CRS): s_orig = caller
+ * row at each S (reverse Cuthill-McKee) position; level of each caller row in
+ * the lower and upper dependency DAGs of the caller's numbering; bandwidth of
+ * the pattern in S order. */
+int tfs_layout_summary(int n, const int *row, const int *col, const int *dia, int *s_orig,
+                       int *level_lower, int *level_upper, int *bandwidth);
+
 #ifdef __cplusplus
 }
 #endif

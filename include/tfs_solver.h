@@ -96,6 +96,8 @@ typedef struct {
   int row_blocks;       /* SpMV row blocks (wavefront work units)            */
   long long device_bytes;
   int inner_colours;    /* colours of the inner order (0 until first built)  */
+  int spmv_rows;        /* SpMV shape: rows per row block (one wavefront)    */
+  int spmv_waves;       /*             wavefronts per workgroup              */
 } tfs_matrix_info;
 int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info);
 
@@ -120,6 +122,18 @@ int tfs_solve_device(tfs_matrix *A, int method, int prec, double *d_x, double *d
 int tfs_solve_mixed_device(tfs_matrix *A, int prec, double *d_x, double *d_r1, int *niter,
                            int *inner_iters, double tol, double inner_rtol, double *ini_res,
                            double *fin_res);
+/* ---- SpMV shape ------------------------------------------------------------------
+ * Not in the reference.  The SpMV works on row blocks of at most 64 rows and
+ * 512 entries, one wavefront each, 2, 4 or 8 wavefronts per workgroup
+ * (default 64 rows, 4 wavefronts).  Row sums are the same for every shape
+ * (each row is summed left to right); the dot products fused into the SpMV
+ * of the Krylov iterations are summed per workgroup, so their last-bit
+ * rounding follows the shape.  tfs_matrix_tune_spmv times 16/32/64 rows x
+ * 2/4/8 wavefronts and keeps the fastest (the batch does this at creation,
+ * where no result depends on it); tfs_matrix_set_spmv_shape fixes a shape. */
+int tfs_matrix_tune_spmv(tfs_matrix *A, int *rows_per_block, int *waves_per_group);
+int tfs_matrix_set_spmv_shape(tfs_matrix *A, int rows_per_block, int waves_per_group);
+
 /* Row order of the FP32 inner solves (the outer FP64 loop is unaffected):
  *   TFS_INNER_REFERENCE  the caller's order: the inner factor is the
  *                        reference's, its sweeps have the caller DAG's depth;
@@ -155,8 +169,9 @@ int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *const *d_x,
                                  double *const *d_r1, int cap, const double *tol,
                                  double inner_rtol, int *steps, int *inner_iters,
                                  double *ini_res, double *fin_res, int *status);
+/* spmv_rows / spmv_waves: the SpMV shape the batch's tuner picked at creation. */
 int tfs_batch_get_info(const tfs_batch *B, int *nsys, int *n_total, long long *nnz_total,
-                       int *graph_launches);
+                       int *graph_launches, int *spmv_rows, int *spmv_waves);
 
 /* ---- serving: continuous batching of solves on one GPU ------------------------
  * Not in the reference.  A case (one system) is registered with its structure

@@ -404,19 +404,21 @@ k_add_correction_all(int n, double *__restrict__ x, const float *__restrict__ e)
 // partial ||rs||^2 (Cgs.f90:182-221, all row-local once alpha is known).
 enum SpmvMode { SPMV_PLAIN = 0, SPMV_RESID, SPMV_DOT, SPMV_CGS_TAIL };
 
-template <typename T, int MODE>
-__global__ void __launch_bounds__(kSpmvThreads)
+// WAVES: wavefronts per workgroup (2, 4 or 8; the row blocks, at most 64 rows
+// each, are a runtime layout) -- the two shapes the SpMV tuner picks.
+template <typename T, int MODE, int WAVES>
+__global__ void __launch_bounds__(WAVES * kWave)
 k_spmv(int nblk, const int *__restrict__ rblk, const int *__restrict__ row,
        const int *__restrict__ col, const T *__restrict__ val, const T *__restrict__ x,
        T *__restrict__ y, const T *__restrict__ w, T *__restrict__ xs, T *__restrict__ rs,
        const KState *st, double *partials) {
   if (st && solve_done(st)) return;
-  __shared__ T prod[kSpmvWaves][kSpmvCap];
+  __shared__ T prod[WAVES][kSpmvCap];
   const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
   T *lds = prod[wv];
   const T alpha = MODE == SPMV_CGS_TAIL ? (T)st->alpha : T(0);
   double acc_red = 0.0;
-  for (int b = blockIdx.x * kSpmvWaves + wv; b < nblk; b += gridDim.x * kSpmvWaves) {
+  for (int b = blockIdx.x * WAVES + wv; b < nblk; b += gridDim.x * WAVES) {
     const int r0 = rblk[b], r1 = rblk[b + 1];
     const int e0 = row[r0], e1 = row[r1];
     T yr = T(0);
@@ -494,7 +496,7 @@ k_spmv(int nblk, const int *__restrict__ rblk, const int *__restrict__ row,
       }
     }
   }
-  if (MODE == SPMV_DOT || MODE == SPMV_CGS_TAIL) store_partial<kSpmvThreads>(acc_red, partials);
+  if (MODE == SPMV_DOT || MODE == SPMV_CGS_TAIL) store_partial<WAVES * kWave>(acc_red, partials);
 }
 
 // ---------------------------------------------------------------------------

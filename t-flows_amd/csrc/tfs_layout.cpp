@@ -167,7 +167,8 @@ void build_sweep(int n, const int *row, const int *col, const int *dia, bool low
 
 }  // namespace
 
-int colour_order(int n, const int *row, const int *col, std::vector<int> &perm) {
+int colour_order(int n, const int *row, const int *col, std::vector<int> &perm,
+                 std::vector<int> *colour_out) {
   // symmetrised adjacency: the pattern may be unsymmetric
   std::vector<int> deg(n + 1, 0);
   for (int i = 0; i < n; ++i)
@@ -204,6 +205,7 @@ int colour_order(int n, const int *row, const int *col, std::vector<int> &perm) 
   for (int c = 0; c < ncol; ++c) start[c + 1] += start[c];
   perm.assign(n, 0);
   for (int i = 0; i < n; ++i) perm[start[colour[i]]++] = i;
+  if (colour_out) *colour_out = std::move(colour);
   return ncol;
 }
 

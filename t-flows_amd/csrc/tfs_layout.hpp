@@ -57,7 +57,8 @@ struct Layout {
 // position -> vertex); returns the number of colours.  Rows of one colour
 // are independent, so an incomplete factor in this order has one level per
 // colour.
-int colour_order(int n, const int *row, const int *col, std::vector<int> &perm);
+int colour_order(int n, const int *row, const int *col, std::vector<int> &perm,
+                 std::vector<int> *colour = nullptr);
 
 // row/col/dia are 0-based, validated (diagonal present, columns ascending).
 void build_layout(int n, const int *row, const int *col, const int *dia, int spmv_rows,

@@ -1,0 +1,35 @@
+// tfs_layout_capi.cpp -- the solver's host-side layout builders behind a C
+// ABI in the CPU library (libtfs_mesh.so), so that the permutations and level
+// structures the device code relies on can be checked without a GPU.
+#include <vector>
+
+#include "tfs_layout.hpp"
+#include "tfs_mesh.h"
+
+extern "C" int tfs_layout_colour(int n, const int *row, const int *col, int *perm, int *colour) {
+  if (n < 0 || (n > 0 && (!row || !col || !perm || !colour))) return -1;
+  std::vector<int> p, c;
+  const int ncol = tfs::colour_order(n, row, col, p, &c);
+  for (int i = 0; i < n; ++i) {
+    perm[i] = p[i];
+    colour[i] = c[i];
+  }
+  return ncol;
+}
+
+extern "C" int tfs_layout_summary(int n, const int *row, const int *col, const int *dia,
+                                  int *s_orig, int *level_lower, int *level_upper,
+                                  int *bandwidth) {
+  if (n < 1 || !row || !col || !dia || !s_orig || !level_lower || !level_upper || !bandwidth)
+    return -1;
+  tfs::Layout L;
+  tfs::build_layout(n, row, col, dia, 64, 512, 2048, L);
+  for (int i = 0; i < n; ++i) s_orig[i] = L.s_orig[i];
+  for (const tfs::SweepLayout *S : {&L.L, &L.U}) {
+    int *lev = S == &L.L ? level_lower : level_upper;
+    for (int l = 0; l < S->nlev; ++l)
+      for (int q = S->lev_ptr[l]; q < S->lev_ptr[l + 1]; ++q) lev[S->orig[q]] = l;
+  }
+  *bandwidth = L.bandwidth;
+  return 0;
+}

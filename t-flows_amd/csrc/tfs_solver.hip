@@ -137,6 +137,8 @@ struct tfs_matrix {
   int *s_row = nullptr, *s_col = nullptr, *s_src = nullptr, *s_diag_src = nullptr;
   int *s_orig = nullptr, *s_of_orig = nullptr, *rblk = nullptr;
   int nblk = 0;
+  int spmv_rows = kSpmvRows;    // rows per SpMV row block (one wavefront each)
+  int spmv_waves = kSpmvWaves;  // wavefronts per SpMV workgroup
   DevSweep L, U;
   int *l_to_s = nullptr, *u_to_l = nullptr, *s_to_u = nullptr;
   double *val = nullptr;       // caller-order FP64 values
@@ -217,7 +219,7 @@ inline int grid_for(long long items, int per_block, int cap = kMaxGrid) {
   return (int)g;
 }
 inline int vgrid(int n) { return grid_for(n, kRedThreads * 4); }
-inline int spmv_parts(const tfs_matrix *A) { return grid_for(A->nblk, kSpmvWaves); }
+inline int spmv_parts(const tfs_matrix *A) { return grid_for(A->nblk, A->spmv_waves); }
 
 template <typename TO, typename TI>
 void gather(int n, const TI *in, const int *idx, TO *out, const KState *st = nullptr) {
@@ -372,9 +374,21 @@ void prec_apply_S(tfs_matrix *A, int prec, const T *bS, T *xS, T *yL, T *zU, con
 template <typename T, int MODE>
 void launch_spmv(tfs_matrix *A, const T *x, T *y, const T *w, T *xs, T *rs, const KState *st) {
   tm_begin(TFS_KC_SPMV);
-  hipLaunchKernelGGL((k_spmv<T, MODE>), dim3(spmv_parts(A)), dim3(kSpmvThreads), 0, cur_stream(),
-                     A->nblk, A->rblk, A->s_row, A->s_col, vals<T>(A).s_val, x, y, w, xs, rs, st,
-                     A->partials);
+  const int g = spmv_parts(A);
+  const T *v = vals<T>(A).s_val;
+  switch (A->spmv_waves) {
+    case 2:
+      hipLaunchKernelGGL((k_spmv<T, MODE, 2>), dim3(g), dim3(2 * kWave), 0, cur_stream(), A->nblk,
+                         A->rblk, A->s_row, A->s_col, v, x, y, w, xs, rs, st, A->partials);
+      break;
+    case 8:
+      hipLaunchKernelGGL((k_spmv<T, MODE, 8>), dim3(g), dim3(8 * kWave), 0, cur_stream(), A->nblk,
+                         A->rblk, A->s_row, A->s_col, v, x, y, w, xs, rs, st, A->partials);
+      break;
+    default:
+      hipLaunchKernelGGL((k_spmv<T, MODE, 4>), dim3(g), dim3(4 * kWave), 0, cur_stream(), A->nblk,
+                         A->rblk, A->s_row, A->s_col, v, x, y, w, xs, rs, st, A->partials);
+  }
   tm_end(TFS_KC_SPMV);
 }
 
@@ -744,6 +758,87 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
   return status;
 }
 
+// ---- SpMV shape: rows per row block (one wavefront each) and wavefronts per
+// workgroup.  Row sums do not depend on it (each row is summed left to right
+// by one lane); the dot products fused into the SpMV are summed per
+// workgroup, so their rounding follows the shape.
+
+// Row blocks over the S rows: at most `rows` rows and kSpmvCap entries each
+// (a longer row is a block of its own).
+std::vector<int> row_blocks(const std::vector<int> &s_row, int rows) {
+  const int n = (int)s_row.size() - 1;
+  std::vector<int> rb(1, 0);
+  for (int r = 0; r < n;) {
+    int e = r + 1;
+    while (e < n && e - r < rows && s_row[e + 1] - s_row[r] <= kSpmvCap) ++e;
+    rb.push_back(e);
+    r = e;
+  }
+  return rb;
+}
+
+int download_s_row(tfs_matrix *A, std::vector<int> *s_row) {
+  s_row->resize((size_t)A->n + 1);
+  TFS_HIP(hipMemcpyAsync(s_row->data(), A->s_row, sizeof(int) * (A->n + 1),
+                         hipMemcpyDeviceToHost, cur_stream()));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
+  return 0;
+}
+
+int set_spmv_shape(tfs_matrix *A, const std::vector<int> &s_row, int rows, int waves) {
+  const std::vector<int> rb = row_blocks(s_row, rows);
+  int *d = nullptr;
+  TFS_HIP(hipStreamSynchronize(cur_stream()));  // no launch still reads the old blocks
+  TFS_TRY(upload(A, &d, rb));
+  TFS_TRY(dev_free(A, A->rblk, sizeof(int) * ((size_t)A->nblk + 1)));
+  A->rblk = d;
+  A->nblk = (int)rb.size() - 1;
+  A->spmv_rows = rows;
+  A->spmv_waves = waves;
+  return 0;
+}
+
+// Times the plain FP64 SpMV for rows per block 16/32/64 x wavefronts per
+// workgroup 2/4/8 (20 back-to-back launches each) and keeps the fastest.
+int tune_spmv(tfs_matrix *A) {
+  if (A->n == 0) return 0;
+  std::vector<int> s_row;
+  TFS_TRY(download_s_row(A, &s_row));
+  const bool was_ready = A->v64.ready;
+  TFS_TRY(prepare_values<double>(A));
+  double *xy = nullptr;
+  const size_t nb = sizeof(double) * (size_t)A->n * 2;
+  TFS_TRY(dev_alloc(A, (void **)&xy, nb));
+  hipLaunchKernelGGL(k_fill<double>, dim3(vgrid(A->n)), dim3(kRedThreads), 0, cur_stream(), A->n,
+                     xy, 1.0);
+  float best = 0.f;
+  int br = A->spmv_rows, bw = A->spmv_waves;
+  for (int rows : {16, 32, 64}) {
+    TFS_TRY(set_spmv_shape(A, s_row, rows, 4));
+    for (int waves : {2, 4, 8}) {
+      A->spmv_waves = waves;
+      launch_spmv<double, SPMV_PLAIN>(A, xy, xy + A->n, nullptr, nullptr, nullptr, nullptr);
+      TFS_HIP(hipEventRecord(A->ev0, cur_stream()));
+      for (int k = 0; k < 20; ++k)
+        launch_spmv<double, SPMV_PLAIN>(A, xy, xy + A->n, nullptr, nullptr, nullptr, nullptr);
+      TFS_HIP(hipEventRecord(A->ev1, cur_stream()));
+      TFS_HIP(hipGetLastError());
+      TFS_HIP(hipEventSynchronize(A->ev1));
+      float t = 0.f;
+      TFS_HIP(hipEventElapsedTime(&t, A->ev0, A->ev1));
+      if (best == 0.f || t < best) {
+        best = t;
+        br = rows;
+        bw = waves;
+      }
+    }
+  }
+  TFS_TRY(set_spmv_shape(A, s_row, br, bw));
+  TFS_TRY(dev_free(A, xy, nb));
+  if (!was_ready) values_changed(A);
+  return 0;
+}
+
 int upload_sweep(tfs_matrix *A, const SweepLayout &S, DevSweep &D) {
   D.nlev = S.nlev;
   D.lev_ptr = S.lev_ptr;
@@ -956,7 +1051,29 @@ extern "C" int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info) {
   info->row_blocks = A->nblk;
   info->device_bytes = A->bytes;
   info->inner_colours = A->inner_colours;
+  info->spmv_rows = A->spmv_rows;
+  info->spmv_waves = A->spmv_waves;
   return TFS_OK;
+}
+
+extern "C" int tfs_matrix_tune_spmv(tfs_matrix *A, int *rows_per_block, int *waves_per_group) {
+  if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
+  TFS_TRY(tune_spmv(A));
+  if (rows_per_block) *rows_per_block = A->spmv_rows;
+  if (waves_per_group) *waves_per_group = A->spmv_waves;
+  return TFS_OK;
+}
+
+extern "C" int tfs_matrix_set_spmv_shape(tfs_matrix *A, int rows_per_block, int waves_per_group) {
+  if (!A || rows_per_block < 1 || rows_per_block > kSpmvRows ||
+      (waves_per_group != 2 && waves_per_group != 4 && waves_per_group != 8))
+    return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
+  if (A->n == 0) return TFS_OK;
+  std::vector<int> s_row;
+  TFS_TRY(download_s_row(A, &s_row));
+  return set_spmv_shape(A, s_row, rows_per_block, waves_per_group);
 }
 
 extern "C" int tfs_matrix_set_inner_order(tfs_matrix *A, int order) {
@@ -1435,6 +1552,15 @@ extern "C" int tfs_batch_create(int nsys, const int *n, const long long *nnz,
     tfs_batch_destroy(B);
     return TFS_ERR_NOMEM;
   }
+  {
+    // the batch's SpMV only forms products (its reductions are per tile), so
+    // the tuned shape cannot change any result
+    StreamScope scope(A->stream);
+    if ((rc = tune_spmv(A)) < 0) {
+      tfs_batch_destroy(B);
+      return rc;
+    }
+  }
   *out = B;
   return TFS_OK;
 }
@@ -1640,12 +1766,15 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
 }
 
 extern "C" int tfs_batch_get_info(const tfs_batch *B, int *nsys, int *n_total,
-                                  long long *nnz_total, int *graph_launches) {
+                                  long long *nnz_total, int *graph_launches,
+                                  int *spmv_rows, int *spmv_waves) {
   if (!B) return TFS_ERR_INVALID;
   if (nsys) *nsys = B->nsys;
   if (n_total) *n_total = B->A->n;
   if (nnz_total) *nnz_total = B->A->nnz;
   if (graph_launches) *graph_launches = B->graph_launches;
+  if (spmv_rows) *spmv_rows = B->A->spmv_rows;
+  if (spmv_waves) *spmv_waves = B->A->spmv_waves;
   return TFS_OK;
 }
 

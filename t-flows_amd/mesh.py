@@ -71,6 +71,11 @@ def _lib():
         lib.tfs_case_dt.argtypes = [ctypes.c_void_p]
         lib.tfs_case_max_aspect.restype = ctypes.c_double
         lib.tfs_case_max_aspect.argtypes = [ctypes.c_void_p]
+        lib.tfs_layout_colour.restype = ctypes.c_int
+        lib.tfs_layout_colour.argtypes = [ctypes.c_int, _I32P, _I32P, _I32P, _I32P]
+        lib.tfs_layout_summary.restype = ctypes.c_int
+        lib.tfs_layout_summary.argtypes = [ctypes.c_int, _I32P, _I32P, _I32P, _I32P, _I32P,
+                                           _I32P, ctypes.POINTER(ctypes.c_int)]
         lib._tfs_typed = True
     return lib
 
@@ -164,3 +169,32 @@ def topology(n_cells, faces_c, base=0, with_pos=False):
     if rc != 0:
         raise ValueError(f"tfs_topology_build failed ({rc})")
     return (row, col, dia, pos) if with_pos else (row, col, dia)
+
+
+def layout_colour(row, col):
+    """Greedy colouring the FP32 inner solves use: (ncolours, perm, colour)."""
+    row = np.ascontiguousarray(row, np.int32)
+    n = len(row) - 1
+    perm = np.empty(n, np.int32)
+    colour = np.empty(n, np.int32)
+    k = _lib().tfs_layout_colour(n, row, np.ascontiguousarray(col, np.int32), perm, colour)
+    if k < 0:
+        raise ValueError("tfs_layout_colour failed")
+    return k, perm, colour
+
+
+def layout_summary(row, col, dia):
+    """The solver's layouts of a 0-based CRS matrix: dict(s_orig, level_lower,
+    level_upper, bandwidth)."""
+    row = np.ascontiguousarray(row, np.int32)
+    n = len(row) - 1
+    s_orig = np.empty(n, np.int32)
+    ll = np.empty(n, np.int32)
+    lu = np.empty(n, np.int32)
+    bw = ctypes.c_int()
+    rc = _lib().tfs_layout_summary(n, row, np.ascontiguousarray(col, np.int32),
+                                   np.ascontiguousarray(dia, np.int32), s_orig, ll, lu,
+                                   ctypes.byref(bw))
+    if rc != 0:
+        raise ValueError("tfs_layout_summary failed")
+    return dict(s_orig=s_orig, level_lower=ll, level_upper=lu, bandwidth=bw.value)

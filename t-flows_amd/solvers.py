@@ -56,7 +56,8 @@ class _Info(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("nnz", ctypes.c_longlong),
                 ("levels_lower", ctypes.c_int), ("levels_upper", ctypes.c_int),
                 ("row_blocks", ctypes.c_int), ("device_bytes", ctypes.c_longlong),
-                ("inner_colours", ctypes.c_int)]
+                ("inner_colours", ctypes.c_int), ("spmv_rows", ctypes.c_int),
+                ("spmv_waves", ctypes.c_int)]
 
 
 def _lib():
@@ -92,6 +93,8 @@ def _lib():
                                            ctypes.c_double, ctypes.c_double, _DP, _DP]
     lib.tfs_prec_diag32_device.argtypes = [_VP, ctypes.c_int, _VP]
     lib.tfs_matrix_set_inner_order.argtypes = [_VP, ctypes.c_int]
+    lib.tfs_matrix_tune_spmv.argtypes = [_VP, _IP, _IP]
+    lib.tfs_matrix_set_spmv_shape.argtypes = [_VP, ctypes.c_int, ctypes.c_int]
     lib.tfs_matrix_set_sweep_schedule.argtypes = [_VP, ctypes.c_int]
     lib.tfs_timing_start.argtypes = [ctypes.c_int]
     lib.tfs_timing_stop.argtypes = [_DP, ctypes.POINTER(ctypes.c_longlong)]
@@ -171,7 +174,19 @@ class Matrix:
         _check(self._lib.tfs_matrix_get_info(self._h, ctypes.byref(i)), "tfs_matrix_get_info")
         return dict(n=i.n, nnz=i.nnz, levels_lower=i.levels_lower,
                     levels_upper=i.levels_upper, row_blocks=i.row_blocks,
-                    device_bytes=i.device_bytes, inner_colours=i.inner_colours)
+                    device_bytes=i.device_bytes, inner_colours=i.inner_colours,
+                    spmv_rows=i.spmv_rows, spmv_waves=i.spmv_waves)
+
+    def tune_spmv(self):
+        """Times the SpMV shapes and keeps the fastest; returns (rows, waves)."""
+        r, w = ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.tfs_matrix_tune_spmv(self._h, ctypes.byref(r), ctypes.byref(w)),
+               "tfs_matrix_tune_spmv")
+        return r.value, w.value
+
+    def set_spmv_shape(self, rows, waves):
+        _check(self._lib.tfs_matrix_set_spmv_shape(self._h, int(rows), int(waves)),
+               "tfs_matrix_set_spmv_shape")
 
     def set_inner_order(self, order):
         """TFS_INNER_REFERENCE or TFS_INNER_COLOURED for solve_mixed."""
@@ -334,7 +349,7 @@ class Batch:
                                                          _VP, ctypes.c_double, _VP, _VP, _VP,
                                                          _VP, _VP]
             lib.tfs_batch_get_info.argtypes = [_VP, _IP, _IP, ctypes.POINTER(ctypes.c_longlong),
-                                               _IP]
+                                               _IP, _IP, _IP]
             lib._tfs_batch_typed = True
         self._lib = lib
         self._keep = []
@@ -373,12 +388,13 @@ class Batch:
         _check(self._lib.tfs_batch_set_values(self._h, s, val), "tfs_batch_set_values")
 
     def info(self):
-        ns, nt, gl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ns, nt, gl, sr, sw = (ctypes.c_int() for _ in range(5))
         nz = ctypes.c_longlong()
         self._lib.tfs_batch_get_info(self._h, ctypes.byref(ns), ctypes.byref(nt),
-                                     ctypes.byref(nz), ctypes.byref(gl))
+                                     ctypes.byref(nz), ctypes.byref(gl), ctypes.byref(sr),
+                                     ctypes.byref(sw))
         return dict(nsys=ns.value, n_total=nt.value, nnz_total=nz.value,
-                    graph_launches=gl.value)
+                    graph_launches=gl.value, spmv_rows=sr.value, spmv_waves=sw.value)
 
     def _vecs(self, xs, rs):
         for k, (x, r) in enumerate(zip(xs, rs)):

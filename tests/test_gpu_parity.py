@@ -286,3 +286,35 @@ def test_config2_full_size(torch, S, oracle):
     true_r = dev(torch, c.b)
     A.residual(x, true_r)
     assert float(torch.linalg.norm(true_r)) < 10 * tol
+
+
+@pytest.mark.parametrize("shape", [(16, 2), (32, 8), (64, 4), (7, 2)])
+def test_spmv_shapes_bitwise(torch, S, oracle, shape):
+    """Every SpMV shape (rows per block x wavefronts per workgroup) forms each
+    row sum exactly as the reference does."""
+    c = get_case("config2_small")
+    A = matrix(S, c)
+    A.set_spmv_shape(*shape)
+    assert (A.info()["spmv_rows"], A.info()["spmv_waves"]) == shape
+    x = np.cos(np.arange(c.n) * 0.11)
+    y = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    A.spmv(dev(torch, x), y)
+    assert np.array_equal(host(y), oracle.spmv(c.row, c.col, c.dia, c.val, x))
+    r = dev(torch, c.b)
+    A.residual(dev(torch, x), r)
+    assert np.array_equal(host(r), oracle.residual(c.row, c.col, c.dia, c.val, x, c.b))
+
+
+def test_tuned_spmv_solve_parity(torch, S, oracle):
+    c = get_case("momentum10")
+    A = matrix(S, c)
+    rows, waves = A.tune_spmv()
+    print("tuned shape", rows, waves, flush=True)
+    assert rows in (16, 32, 64) and waves in (2, 4, 8)
+    tol = 1e-10 * float(np.linalg.norm(c.b))
+    x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    r = dev(torch, c.b)
+    it, ini, fin = S.Bicg(A, x, r, 2, 5000, tol)
+    ref = oracle.solve("bicg", c.row, c.col, c.dia, c.val, c.b, prec=2, niter=5000, tol=tol)
+    assert iters_close(it, ref["niter"]) and fin < tol
+    assert rel_l2(host(x), ref["x"]) <= 1e-8

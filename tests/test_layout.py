@@ -1,0 +1,70 @@
+"""The host-side layouts the device kernels rely on (CPU only): the reverse
+Cuthill-McKee S order, the level structure of the caller's lower and upper
+dependency DAGs (which the incomplete factor and both sweeps follow), and the
+greedy colouring of the FP32 inner solves."""
+import numpy as np
+import pytest
+
+from tflows_amd import mesh
+
+CASES = {
+    "box_6x5x4": dict(kind=mesh.POISSON, nx=6, ny=5, nz=4),
+    "box_perm": dict(kind=mesh.POISSON, nx=7, ny=6, nz=5, permute=True),
+    "momentum_perm": dict(kind=mesh.MOMENTUM, nx=6, beta=2.0, jitter=0.1, permute=True,
+                          pin_first=False, rhs_zero_mean=False),
+}
+
+
+def neighbours(c, i):
+    cols = c.col[c.row[i]:c.row[i + 1]]
+    return cols[cols != i]
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_colouring_is_proper_and_greedy(name):
+    c = mesh.make_case(**CASES[name])
+    k, perm, colour = mesh.layout_colour(c.row, c.col)
+    assert sorted(perm.tolist()) == list(range(c.n))
+    assert np.all(np.diff(colour[perm]) >= 0)            # grouped by colour
+    for i in range(c.n):
+        nb = neighbours(c, i)
+        assert np.all(colour[nb] != colour[i])           # proper
+        earlier = set(colour[nb[nb < i]].tolist())       # greedy: smallest free colour
+        assert colour[i] == min(set(range(k + 1)) - earlier)
+    assert k == colour.max() + 1
+
+
+def test_lexicographic_box_is_two_coloured():
+    c = mesh.make_case(**CASES["box_6x5x4"])
+    k, perm, colour = mesh.layout_colour(c.row, c.col)
+    assert k == 2
+    # red-black: colour = parity of i + j + k in lexicographic numbering
+    idx = np.arange(c.n)
+    i, j, kk = idx % 6, (idx // 6) % 5, idx // 30
+    assert np.array_equal(colour, (i + j + kk) % 2)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_levels_follow_the_callers_dags(name):
+    c = mesh.make_case(**CASES[name])
+    L = mesh.layout_summary(c.row, c.col, c.dia)
+    assert sorted(L["s_orig"].tolist()) == list(range(c.n))
+    for i in range(c.n):
+        nb = neighbours(c, i)
+        lo, up = nb[nb < i], nb[nb > i]
+        assert L["level_lower"][i] == (L["level_lower"][lo].max() + 1 if lo.size else 0)
+        assert L["level_upper"][i] == (L["level_upper"][up].max() + 1 if up.size else 0)
+
+
+def test_box_depth_and_rcm_bandwidth():
+    c = mesh.make_case(**CASES["box_6x5x4"])
+    L = mesh.layout_summary(c.row, c.col, c.dia)
+    assert L["level_lower"].max() + 1 == 6 + 5 + 4 - 2
+    assert L["level_upper"].max() + 1 == 6 + 5 + 4 - 2
+    # RCM of a box is no wider than its lexicographic numbering (nx*ny)
+    assert 0 < L["bandwidth"] <= 6 * 5
+    cp = mesh.make_case(**CASES["box_perm"])
+    Lp = mesh.layout_summary(cp.row, cp.col, cp.dia)
+    natural_perm_bw = max(abs(int(cp.col[k]) - i) for i in range(cp.n)
+                          for k in range(cp.row[i], cp.row[i + 1]))
+    assert Lp["bandwidth"] < natural_perm_bw
