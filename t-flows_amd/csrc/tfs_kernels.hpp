@@ -419,7 +419,10 @@ k_spmv(int nblk, const int *__restrict__ rblk, const int *__restrict__ row,
   const T alpha = MODE == SPMV_CGS_TAIL ? (T)st->alpha : T(0);
   double acc_red = 0.0;
   for (int b = blockIdx.x * WAVES + wv; b < nblk; b += gridDim.x * WAVES) {
-    const int r0 = rblk[b], r1 = rblk[b + 1];
+    // b is the same in every lane: the block's bounds come through the
+    // scalar unit
+    const int bu = __builtin_amdgcn_readfirstlane(b);
+    const int r0 = rblk[bu], r1 = rblk[bu + 1];
     const int e0 = row[r0], e1 = row[r1];
     T yr = T(0);
     int r = -1;
@@ -427,19 +430,23 @@ k_spmv(int nblk, const int *__restrict__ rblk, const int *__restrict__ row,
       // all of the block's col/val loads are issued before the first x
       // gather (kSpmvCap / kWave per lane), so each wave keeps up to 8
       // independent loads in flight instead of one dependent pair
+      // (indices past the block are clamped to its last entry: every load is
+      // unconditional, so none waits on another, and the clamped ones hit
+      // the cache line already being read)
       constexpr int kPer = kSpmvCap / kWave;
       int ci[kPer];
       T vi[kPer];
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
-        const int e = e0 + lane + u * kWave;
-        ci[u] = e < e1 ? col[e] : 0;
-        vi[u] = e < e1 ? val[e] : T(0);
+        const int e = min(e0 + lane + u * kWave, e1 - 1);
+        ci[u] = col[e];
+        vi[u] = val[e];
       }
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
         const int e = e0 + lane + u * kWave;
-        if (e < e1) lds[e - e0] = vi[u] * x[ci[u]];
+        const T pr = vi[u] * x[ci[u]];
+        if (e < e1) lds[e - e0] = pr;
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -652,15 +659,22 @@ k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, i
     // which its last dependency arrives: a chunk can hold two levels, so a
     // row may wait on another lane of its own wavefront, and a lane that left
     // the loop would only run its store once every lane of the wave had left.
+    // The first pass reads the dependencies with plain (cached) loads: within
+    // a sweep a location only ever changes from the sentinel to its final
+    // value, and every sentinel was stored by an earlier launch, so a plain
+    // load returns either the final value or a (possibly stale) sentinel.
+    // Only a sentinel sends the row to the device-coherent polls, which
+    // bypass the XCD's L2 and fetch whole lines from memory.
     int used = 0;  // entries folded into s so far, in order
     unsigned int spins = 0;
     bool done = false;
+    bool first = true;
     while (!done) {
       bool blocked = false;
 #pragma unroll
       for (int k = 0; k < kPrefetch; ++k) {
         if (!blocked && k >= used && k < len) {
-          const T v = coherent_load(&out[pc[k]]);
+          const T v = first ? out[pc[k]] : coherent_load(&out[pc[k]]);
           if (Sentinel<T>::is(v)) {
             blocked = true;
           } else {
@@ -670,7 +684,7 @@ k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, i
         }
       }
       for (int e = e0 + used; !blocked && e < e1; ++e) {  // entries beyond the prefetch
-        const T v = coherent_load(&out[Tr.col[e]]);
+        const T v = first ? out[Tr.col[e]] : coherent_load(&out[Tr.col[e]]);
         if (Sentinel<T>::is(v)) {
           blocked = true;
         } else {
@@ -679,6 +693,7 @@ k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, i
           used = e - e0 + 1;
         }
       }
+      first = false;
       if (!blocked) {
         coherent_store(&out[p], OP == OP_FACTOR ? T(1) / s : s * dp);
         done = true;

@@ -57,9 +57,15 @@ struct Request {
   bool cancelled = false;
 };
 
-// Device bytes of a resident case before it exists: values and indices in
-// three layouts, maps, factor and 7 FP64 work vectors (rounded up).
-long long estimate_bytes(int n, long long nnz) { return 44 * nnz + 180 * (long long)n + (1 << 20); }
+// Device bytes of a resident case before it exists (replaced by the exact
+// figure once created): per entry, col + source index in the S layout, the
+// caller-order and S-order FP64 values, and col + source + value in the L or
+// U triangle (off-diagonal entries); per row, row pointers, maps, diagonal
+// copies, the factor in two orders, two staging vectors and 7 work vectors;
+// plus the fixed per-matrix buffers.
+long long estimate_bytes(int n, long long nnz) {
+  return 24 * nnz + 16 * (nnz - n) + 160 * (long long)n + (64 << 10);
+}
 
 }  // namespace
 

@@ -48,3 +48,20 @@ def rel_l2(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def exact_solution(row, col, val, b):
+    """x = A^-1 b by sparse LU (scipy), the yardstick for solvers that reach
+    the tolerance along a different path than the reference (0-based CRS)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    n = len(row) - 1
+    A = sp.csr_matrix((np.asarray(val, np.float64), np.asarray(col), np.asarray(row)), shape=(n, n))
+    return spla.spsolve(A.tocsc(), np.asarray(b, np.float64))
+
+
+def refinement_bar(x_ref_path, x_exact):
+    """The accuracy a solve that reaches the tolerance by another path must
+    meet: within 10x of the reference path's own error against the exact
+    solution, and never worse than rel 1e-8 unless the reference itself is."""
+    return max(1e-8, 10.0 * rel_l2(x_ref_path, x_exact))

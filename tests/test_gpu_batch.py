@@ -53,18 +53,22 @@ def test_batch_equals_single_solves(torch, S, oracle, prec):
         xb = [torch.zeros(c.n, dtype=torch.float64, device="cuda")]
         rb = [torch.from_numpy(c.b.copy()).cuda()]
         nb, inib, finb, stb = B1.solve(xb, rb, prec, [3000], [tols[k]])
-        assert niter[k] == nb[0] and st[k] == S.TFS_OK and stb[0] == S.TFS_OK
+        assert niter[k] == nb[0] and st[k] == stb[0]
         assert np.array_equal(xs[k].cpu().numpy(), xb[0].cpu().numpy()), k
         assert fin[k] == finb[0] and ini[k] == inib[0]
         A = S.Matrix(c.row, c.col, c.dia, c.val)
         x1 = torch.zeros(c.n, dtype=torch.float64, device="cuda")
         r1 = torch.from_numpy(c.b.copy()).cuda()
         it1, ini1, fin1 = S.Cg(A, x1, r1, prec, 3000, tols[k])
-        assert abs(int(niter[k]) - it1) <= max(1, it1 // 20)
-        assert rel_l2(xs[k].cpu().numpy(), x1.cpu().numpy()) <= 1e-8
         ref = oracle.solve("cg", c.row, c.col, c.dia, c.val, c.b, prec=prec, niter=3000,
                            tol=tols[k])
+        assert abs(int(niter[k]) - it1) <= max(1, it1 // 20)
         assert abs(niter[k] - ref["niter"]) <= max(1, ref["niter"] // 20)
+        if ref["niter"] > 3000:   # the reference does not converge either (Cg, nonsymmetric)
+            assert st[k] == S.TFS_NOT_CONVERGED
+            continue
+        assert st[k] == S.TFS_OK
+        assert rel_l2(xs[k].cpu().numpy(), x1.cpu().numpy()) <= 1e-8
         assert rel_l2(xs[k].cpu().numpy(), ref["x"]) <= 1e-8
     assert B.info()["graph_launches"] > 0
 

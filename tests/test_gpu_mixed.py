@@ -2,15 +2,20 @@
 
 The refinement solver has no counterpart in the reference; what pins it is the
 FP64 answer: the refined solution must reach the same FP64 tolerance (the
-final residual is the true ||b - A x||, computed in FP64) and agree with the
-oracle's FP64 Cg solution to rel-L2 <= 1e-8 at tol 1e-10 * ||b||.  The FP32
+final residual is the true ||b - A x||, computed in FP64) and be as accurate
+as the oracle's FP64 Cg solution at the same tolerance: its rel-L2 error
+against the exact solution (sparse LU) is within 10x of the oracle's own
+error, and <= 1e-8 wherever the oracle's is (a solve that stops at
+||r|| < 1e-10 ||b|| along another path cannot agree with the oracle's iterate
+more closely than both agree with the solution: on the stretched Poisson case
+both are ~5e-8 from it).  The errors are printed for every case.  The FP32
 factor is the reference recurrence in FP32: it agrees with the FP64 oracle
 factor to FP32 rounding (rel 1e-5).
 """
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import exact_solution, refinement_bar, rel_l2
 from tflows_amd import cases, mesh
 
 pytestmark = pytest.mark.gpu
@@ -54,7 +59,11 @@ def test_refinement_reaches_fp64_solution(torch, S, oracle, name, prec, order):
     assert A.status == S.TFS_OK and fin < tol, (steps, inner, fin, tol)
     assert 1 <= steps <= 10
     assert abs(ini - np.linalg.norm(c.b)) <= 1e-12 * np.linalg.norm(c.b)
-    assert rel_l2(x.cpu().numpy(), ref["x"]) <= 1e-8
+    xs = exact_solution(c.row, c.col, c.val, c.b)
+    err, err_ref = rel_l2(x.cpu().numpy(), xs), rel_l2(ref["x"], xs)
+    print(f"  steps {steps} inner {inner}: rel-L2 vs exact {err:.2e} (oracle {err_ref:.2e}), "
+          f"vs oracle {rel_l2(x.cpu().numpy(), ref['x']):.2e}", flush=True)
+    assert err <= refinement_bar(ref["x"], xs)
     # the residual left in r1 is the true FP64 residual
     true_r = torch.from_numpy(c.b.copy()).cuda()
     A.residual(x, true_r)
@@ -104,11 +113,18 @@ def test_coloured_inner_order(torch, S):
         out[order] = x.cpu().numpy()
         print("order", order, "steps", steps, "inner", inner, flush=True)
     assert A.info()["inner_colours"] == 2
-    assert rel_l2(out[0], out[1]) <= 1e-9
-    # scaled values: the solution scales inversely
+    xs = exact_solution(c.row, c.col, c.val, c.b)
+    ref = S.Matrix(c.row, c.col, c.dia, c.val)
+    x64 = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+    r64 = torch.from_numpy(c.b.copy()).cuda()
+    S.Cg(ref, x64, r64, 2, 20000, tol)
+    bar = refinement_bar(x64.cpu().numpy(), xs)
+    for order in out:
+        assert rel_l2(out[order], xs) <= bar
+    # new values reach the colour-ordered copy: doubled values, half the solution
     A.set_values(2.0 * c.val)
     x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
     r = torch.from_numpy(c.b.copy()).cuda()
     steps, inner, ini, fin = S.solve_mixed(A, x, r, 2, 50, tol / 2)
     assert A.status == S.TFS_OK
-    assert rel_l2(2.0 * x.cpu().numpy(), out[1]) <= 1e-9
+    assert rel_l2(2.0 * x.cpu().numpy(), xs) <= bar
