@@ -115,28 +115,51 @@ def cpu_iteration_sample(case, solve, iters, gpu_niter):
                 seconds_per_solve=per_solve)
 
 
-def timed(step, steps, warmup, sync):
-    for _ in range(warmup):
+def timed(step, steps, warmup, sync, label=None):
+    """Runs warmup + steps calls of step(); with a label, logs every call
+    (long solves then keep printing)."""
+    for k in range(warmup):
+        ts = time.perf_counter()
         step()
+        if label:
+            sync()
+            log(f"  {label} warmup {k}: {time.perf_counter() - ts:.2f} s")
     sync()
     lat = []
     t0 = time.perf_counter()
     res = None
-    for _ in range(steps):
+    for k in range(steps):
         ts = time.perf_counter()
         res = step()
         lat.append(time.perf_counter() - ts)
+        if label:
+            log(f"  {label} step {k}: {lat[-1]:.2f} s")
     sync()
     return time.perf_counter() - t0, lat, res
 
 
-def spmv_roofline(A, x, y, n, nnz, reps=50, inloop_ms=None):
+def measured_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 counter summary
+    profiles/<name>.md (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy
+    kernel's reading): the line 'traffic_bytes_per_launch: N'.  None if that
+    summary is absent."""
+    f = ROOT / "profiles" / f"{name}.md"
+    if not f.exists():
+        return None
+    for ln in f.read_text().splitlines():
+        if ln.strip().startswith("traffic_bytes_per_launch:"):
+            return float(ln.split(":", 1)[1])
+    return None
+
+
+def spmv_roofline(A, x, y, n, nnz, reps=50, inloop_ms=None, traffic_profile=None):
     ms = A.spmv_time(x, y, reps=reps)
     algo = spmv_bytes(n, nnz)
     achieved = algo / (ms * 1e-3) / 1e9
     return {
         "kernel": "k_spmv (FP64 CSR SpMV)", "bound": "hbm", "achieved": achieved,
-        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+        "traffic": measured_traffic(traffic_profile) if traffic_profile else None,
         "algorithmic_bytes": algo, "launch_us": ms * 1e3,
         "launch_us_in_solves": None if inloop_ms is None else inloop_ms * 1e3,
     }
@@ -206,7 +229,8 @@ def single_solve(args, k):
     bd = breakdown(lib, step)
     y = torch.empty_like(b)
     roof = spmv_roofline(A, x, y, c.n, c.nnz,
-                         inloop_ms=spmv_ms.value / max(spmv_n.value, 1))
+                         inloop_ms=spmv_ms.value / max(spmv_n.value, 1),
+                         traffic_profile=f"k_spmv_workload{k}_counters")
     log(f"  spmv {roof['launch_us']:.1f} us back-to-back ({roof['achieved']:.0f} GB/s); "
         f"breakdown {bd}")
     # the solve again, for the solution compared with the CPU
@@ -258,16 +282,16 @@ def mixed_vs_fp64(args):
         A.values_changed()
         return solvers.Cg(A, x, r, solve["prec"], solve["cap"], tol)
 
-    def step32():
+    def step32(x_rtol=0.0):
         r.copy_(b)
         x.zero_()
         A.values_changed()
-        return solvers.solve_mixed(A, x, r, solve["prec"], 200, tol, args.inner_rtol)
+        return solvers.solve_mixed(A, x, r, solve["prec"], 200, tol, args.inner_rtol, x_rtol)
 
-    e64, lat64, res64 = timed(step64, args.steps, args.warmup, torch.cuda.synchronize)
+    e64, lat64, res64 = timed(step64, args.steps, args.warmup, torch.cuda.synchronize, "fp64")
     x64 = x.cpu().numpy()
     log(f"  fp64: {1e3 * e64 / args.steps:.1f} ms/solve, niter={res64[0]}, fin={res64[2]:.3e}")
-    e32, lat32, res32 = timed(step32, args.steps, args.warmup, torch.cuda.synchronize)
+    e32, lat32, res32 = timed(step32, args.steps, args.warmup, torch.cuda.synchronize, "mixed")
     x32 = x.cpu().numpy()
     steps_out, inner, _, fin32 = res32
     err = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
@@ -275,6 +299,12 @@ def mixed_vs_fp64(args):
     log(f"  mixed: {1e3 * e32 / args.steps:.1f} ms/solve, {steps_out} refinement steps, "
         f"{inner} inner iterations, fin={fin32:.3e}; speedup {speedup:.2f}x, rel-L2 vs FP64 "
         f"{err:.2e}")
+    # the same with the correction criterion (FP64-level accuracy)
+    x_rtol = solve["rtol"]
+    ex, _, resx = timed(lambda: step32(x_rtol), args.steps, 0, torch.cuda.synchronize, "mixed+x")
+    errx = float(np.linalg.norm(x.cpu().numpy() - x64) / np.linalg.norm(x64))
+    log(f"  mixed, ||e|| <= {x_rtol:g} ||x||: {1e3 * ex / args.steps:.1f} ms/solve, "
+        f"{resx[0]} steps, {resx[1]} inner; speedup {e64 / ex:.2f}x, rel-L2 vs FP64 {errx:.2e}")
     lib = solvers._lib()
     bd = breakdown(lib, step64)
     y = torch.empty_like(b)
@@ -296,7 +326,12 @@ def mixed_vs_fp64(args):
                             "breakdown_ms_per_solve": {k2: v["ms_per_solve"] for k2, v in bd.items()}},
                    "mixed": {"refinement_steps": steps_out, "inner_iterations": inner,
                              "fin_res": fin32, "inner_rtol": args.inner_rtol,
+                             "inner_order": "coloured" if A.info()["inner_colours"] else
+                             "reference", "inner_colours": A.info()["inner_colours"],
                              "rel_l2_vs_fp64": err},
+                   "mixed_x_rtol": {"x_rtol": x_rtol, "ms_per_solve": 1e3 * ex / args.steps,
+                                    "refinement_steps": resx[0], "inner_iterations": resx[1],
+                                    "speedup_vs_fp64": e64 / ex, "rel_l2_vs_fp64": errx},
                    "speedup_vs_fp64": speedup, "cpu_baseline": cpu})
     out["dtype"] = "f32 inner / f64 outer"
     out["config"].update(n_cells=c.n, nnz=c.nnz, rtol=solve["rtol"], tol=tol,
@@ -334,9 +369,10 @@ def batch(args):
         reset()
         return B.solve(xs, rs, 2, [20000] * len(sysl), tols)
 
-    e64, _, r64 = timed(step64, args.steps, args.warmup, torch.cuda.synchronize)
+    e64, _, r64 = timed(step64, args.steps, args.warmup, torch.cuda.synchronize, "batch fp64")
     x64 = [x.cpu().numpy() for x in xs]
-    e32, lat, r32 = timed(step_mixed, args.steps, args.warmup, torch.cuda.synchronize)
+    e32, lat, r32 = timed(step_mixed, args.steps, args.warmup, torch.cuda.synchronize,
+                          "batch mixed")
     errs = [float(np.linalg.norm(x.cpu().numpy() - y) / np.linalg.norm(y)) for x, y in zip(xs, x64)]
     steps_out, inner, _, fin, st = r32
     assert all(s == 0 for s in st), st.tolist()
@@ -488,7 +524,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--cpu-reps", type=int, default=3)
-    ap.add_argument("--cpu-iters", type=int, default=20,
+    ap.add_argument("--cpu-iters", type=int, default=10,
                     help="workload 3: iterations in the CPU sample")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="workloads 4-5: CPU seconds of the sample")

@@ -118,10 +118,16 @@ int tfs_solve_device(tfs_matrix *A, int method, int prec, double *d_x, double *d
  * niter: in = cap on refinement steps; out = steps taken (cap+1 if tol was not
  * reached, which also happens when FP32 stops reducing the defect);
  * inner_iters = FP32 PCG iterations over all steps; ini_res = ||b - A x0||;
- * fin_res = ||b - A x|| (the true FP64 residual, left in r1). */
+ * fin_res = ||b - A x|| (the true FP64 residual, left in r1).
+ * x_rtol > 0 adds a stopping condition on the solution: the last correction
+ * must also satisfy ||e|| <= x_rtol ||x||.  Each step contracts the error by
+ * about cond(A) * 2^-24, so on an ill-conditioned system ||r|| can pass tol
+ * while the error is still well above what FP64 CG leaves at the same tol;
+ * x_rtol = tol / ||b|| asks for the FP64 solver's accuracy (one or two more
+ * steps).  0 stops on the residual alone. */
 int tfs_solve_mixed_device(tfs_matrix *A, int prec, double *d_x, double *d_r1, int *niter,
-                           int *inner_iters, double tol, double inner_rtol, double *ini_res,
-                           double *fin_res);
+                           int *inner_iters, double tol, double inner_rtol, double x_rtol,
+                           double *ini_res, double *fin_res);
 /* ---- SpMV shape ------------------------------------------------------------------
  * Not in the reference.  The SpMV works on row blocks of at most 64 rows and
  * 512 entries, one wavefront each, 2, 4 or 8 wavefronts per workgroup
@@ -169,6 +175,9 @@ int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *const *d_x,
                                  double *const *d_r1, int cap, const double *tol,
                                  double inner_rtol, int *steps, int *inner_iters,
                                  double *ini_res, double *fin_res, int *status);
+/* Row order of the batch's FP32 inner solves, as tfs_matrix_set_inner_order
+ * (default TFS_INNER_COLOURED: every system's rows grouped by colour). */
+int tfs_batch_set_inner_order(tfs_batch *B, int order);
 /* spmv_rows / spmv_waves: the SpMV shape the batch's tuner picked at creation. */
 int tfs_batch_get_info(const tfs_batch *B, int *nsys, int *n_total, long long *nnz_total,
                        int *graph_launches, int *spmv_rows, int *spmv_waves);

@@ -40,7 +40,11 @@ constexpr int kChunk = 256;              // positions per persistent-sweep chunk
 constexpr int kPrefetch = 8;             // row entries held in registers
 constexpr unsigned kSpinLimit = 1u << 20;
 
-enum Epi { EPI_BNRM = 0, EPI_INIRES, EPI_RHO, EPI_ALPHA, EPI_NORM };
+enum Epi { EPI_BNRM = 0, EPI_INIRES, EPI_RHO, EPI_ALPHA, EPI_NORM, EPI_AUX0, EPI_AUX1 };
+
+// Reductions that run whatever the solve's state (the rest return at once
+// once the solve is done).
+__device__ __forceinline__ bool epi_always(int epi) { return epi == EPI_BNRM || epi >= EPI_AUX0; }
 
 // ---------------------------------------------------------------------------
 // Solve-state transitions (Cg.f90:48-58,75-90,108-122,148-153,167-181).
@@ -89,6 +93,10 @@ __device__ inline void epilogue(int epi, double tot, KState *st) {
       }
       break;
     }
+    case EPI_AUX0:
+    case EPI_AUX1:
+      st->aux[epi - EPI_AUX0] = sqrt(tot);
+      break;
     default:
       break;
   }
@@ -105,7 +113,7 @@ __device__ __forceinline__ void store_partial(double v, double *partials) {
 // state transition (one workgroup; visibility comes from the kernel boundary).
 __global__ void __launch_bounds__(kRedThreads)
 k_finish(int nparts, const double *__restrict__ partials, int epi, KState *st) {
-  if (epi != EPI_BNRM && solve_done(st)) return;
+  if (!epi_always(epi) && solve_done(st)) return;
   __shared__ double lds[kRedThreads / kWave];
   double v = 0.0;
   for (int k = threadIdx.x; k < nparts; k += kRedThreads) v += partials[k];
@@ -122,7 +130,7 @@ k_finish(int nparts, const double *__restrict__ partials, int epi, KState *st) {
 template <typename T>
 __global__ void __launch_bounds__(kRedThreads)
 k_sumsq(int n, const T *__restrict__ v, int epi, const KState *st, double *partials) {
-  if (epi != EPI_BNRM && solve_done(st)) return;
+  if (!epi_always(epi) && solve_done(st)) return;
   double s = 0.0;
   TFS_GRID_STRIDE(i, n) {
     const double a = v[i];
@@ -388,13 +396,6 @@ k_finish_seg(const int *__restrict__ tile_first, const double *__restrict__ part
       if (before + 1 == (unsigned int)all->cap) all->done = 1;
     }
   }
-}
-
-// Iterative refinement, batched: x += e on every row (a converged system's
-// correction is zero).
-__global__ void __launch_bounds__(kRedThreads)
-k_add_correction_all(int n, double *__restrict__ x, const float *__restrict__ e) {
-  TFS_GRID_STRIDE(i, n) x[i] = x[i] + (double)e[i];
 }
 
 // ---------------------------------------------------------------------------

@@ -57,14 +57,14 @@ struct Request {
   bool cancelled = false;
 };
 
-// Device bytes of a resident case before it exists (replaced by the exact
-// figure once created): per entry, col + source index in the S layout, the
+// Device bytes of a resident case before its first solve (then replaced by
+// the exact figure): per entry, col + source index in the S layout, the
 // caller-order and S-order FP64 values, and col + source + value in the L or
 // U triangle (off-diagonal entries); per row, row pointers, maps, diagonal
-// copies, the factor in two orders, two staging vectors and 7 work vectors;
-// plus the fixed per-matrix buffers.
+// copies, the factor in two orders, two staging vectors and up to 12 work
+// vectors (Cgs); plus the fixed per-matrix buffers.
 long long estimate_bytes(int n, long long nnz) {
-  return 24 * nnz + 16 * (nnz - n) + 160 * (long long)n + (64 << 10);
+  return 24 * nnz + 16 * (nnz - n) + 200 * (long long)n + (32 << 10);
 }
 
 }  // namespace
@@ -159,15 +159,6 @@ struct tfs_server {
                                &c->A);
         if (rc == 0) rc = tfs_matrix_set_sweep_schedule(c->A, TFS_SWEEP_DYNAMIC);
         if (rc == 0) rc = tfs_matrix_set_values(c->A, c->val.data());
-        tfs_matrix_info info;
-        if (rc == 0 && tfs_matrix_get_info(c->A, &info) == 0) {
-          // work vectors are allocated by the first solve: 7 FP64 vectors
-          const long long actual = info.device_bytes + 7LL * 8 * c->n;
-          std::lock_guard<std::mutex> lk(mu);
-          used += actual - c->bytes;
-          peak = std::max(peak, used);
-          c->bytes = actual;
-        }
       }
       if (rc == 0) {
         r->x.assign((size_t)c->n, 0.0);
@@ -178,6 +169,15 @@ struct tfs_server {
         r->niter = it;
         r->ini_res = ini;
         r->fin_res = fin;
+        // the exact device bytes, now that the solve has allocated its values
+        // and work vectors (the reservation was an estimate)
+        tfs_matrix_info info;
+        if (tfs_matrix_get_info(c->A, &info) == 0) {
+          std::lock_guard<std::mutex> lk(mu);
+          used += info.device_bytes - c->bytes;
+          peak = std::max(peak, used);
+          c->bytes = info.device_bytes;
+        }
       }
       r->status = rc;
       const bool free_case = r->release || (create && rc < 0 && rc != TFS_ERR_BREAKDOWN);

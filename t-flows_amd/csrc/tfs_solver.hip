@@ -648,11 +648,16 @@ int build_inner(tfs_matrix *A) {
 // FP32-inner PCG inside FP64 iterative refinement (defect correction).
 //   r = b - A x (FP64);  repeat: solve A e = r in FP32 (PCG with the factor
 //   recomputed in FP32 from FP32 values) to ||r_in|| < inner_rtol ||r||;
-//   x += e (FP64); r = b - A x (FP64)  -- until ||r|| < tol.
+//   x += e (FP64); r = b - A x (FP64)  -- until ||r|| < tol, and, if x_rtol
+//   > 0, until the last correction is also small: ||e|| <= x_rtol ||x||.
+// Each step contracts the error by about cond(A) * 2^-24 (the FP32 rounding
+// of A), so on an ill-conditioned system the residual can pass tol while
+// components of the error along the smallest eigenvectors remain; the
+// correction size bounds them (it is the previous iterate's error).
 // The outer residual, the update and every norm stay in FP64 on the device.
 int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *niter,
-                       int *inner_iters, double tol, double inner_rtol, double *ini_res,
-                       double *fin_res) {
+                       int *inner_iters, double tol, double inner_rtol, double x_rtol,
+                       double *ini_res, double *fin_res) {
   if (!A || !niter || !inner_iters || !ini_res || !fin_res) return TFS_ERR_INVALID;
   if (prec < 0 || prec > 2) prec = 0;
   const int n = A->n, cap = *niter;
@@ -704,17 +709,20 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
   KState *ho = A->h_st + 1;
   const double err0 = ho->bnrm2;
   double err = err0;
+  bool small_update = x_rtol <= 0.0;  // no criterion on the correction
   int outer = 0, status = TFS_OK, stalls = 0;
-  while (err >= tol) {
+  while (err >= tol || !small_update) {
     if (outer >= cap) {
       status = TFS_NOT_CONVERGED;
       break;
     }
     ++outer;
-    // inner solve, FP32: e = 0, r32 = r (in I's S order)
+    // inner solve, FP32: e = 0, r32 = r (in I's S order); below tol (only
+    // while the correction criterion is pending) the inner target is relative
+    const double inner_tol = err >= tol ? std::max(inner_rtol * err, 0.5 * tol) : inner_rtol * err;
     gather(n, D.r, to_s, F.r);
     hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.x, 0.0f);
-    TFS_TRY(init_state(I, 0, std::max(inner_rtol * err, 0.5 * tol), 100000, 0.0));
+    TFS_TRY(init_state(I, 0, inner_tol, 100000, 0.0));
     hipLaunchKernelGGL(k_sumsq<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.r,
                        (int)EPI_INIRES, si, I->partials);
     finish(g, I->partials, EPI_INIRES, si);
@@ -726,10 +734,18 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
       status = TFS_ERR_BREAKDOWN;
       break;
     }
-    // x += e; r = b - A x; ||r||
+    // x += e; r = b - A x; ||r|| (and ||e||, ||x|| for the correction criterion)
     hipLaunchKernelGGL(k_add_correction, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x,
                        F.x, to_s);
     TFS_TRY(init_state(A, 1, tol, cap, 0.0));
+    if (!small_update) {
+      hipLaunchKernelGGL(k_sumsq<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.x,
+                         (int)EPI_AUX0, so, A->partials);
+      finish(g, A->partials, EPI_AUX0, so);
+      hipLaunchKernelGGL(k_sumsq<double>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x,
+                         (int)EPI_AUX1, so, A->partials);
+      finish(g, A->partials, EPI_AUX1, so);
+    }
     outer_residual();
     TFS_HIP(hipGetLastError());
     TFS_TRY(read_state(A, 1));
@@ -739,11 +755,12 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
       status = TFS_ERR_BREAKDOWN;
       break;
     }
+    if (!small_update) small_update = ho->aux[0] <= x_rtol * ho->aux[1];
     stalls = e_new > 0.9 * err ? stalls + 1 : 0;
     err = e_new;
     if (stalls >= 3) {  // FP32 can no longer reduce the defect
-      status = TFS_NOT_CONVERGED;
-      break;
+      if (err >= tol) status = TFS_NOT_CONVERGED;
+      break;             // (below tol: the correction is at the rounding floor)
     }
   }
   scatter(n, D.x, A->s_orig, x);
@@ -1111,11 +1128,12 @@ extern "C" int tfs_solve(tfs_matrix *A, int method, int prec, double *x1, double
 
 extern "C" int tfs_solve_mixed_device(tfs_matrix *A, int prec, double *d_x, double *d_r1,
                                       int *niter, int *inner_iters, double tol,
-                                      double inner_rtol, double *ini_res, double *fin_res) {
+                                      double inner_rtol, double x_rtol, double *ini_res,
+                                      double *fin_res) {
   if (!A) return TFS_ERR_INVALID;
   StreamScope scope(A->stream);
-  return solve_mixed_device(A, prec, d_x, d_r1, niter, inner_iters, tol, inner_rtol, ini_res,
-                            fin_res);
+  return solve_mixed_device(A, prec, d_x, d_r1, niter, inner_iters, tol, inner_rtol, x_rtol,
+                            ini_res, fin_res);
 }
 
 // Building blocks on caller-order vectors: permuted into the S layout and back.
@@ -1293,6 +1311,12 @@ struct tfs_batch {
   int graph_prec[2] = {-1, -1};
   bool use_graph = true;
   int graph_launches = 0;
+  // FP32 inner solves of the refinement on a colour-ordered copy (each
+  // system's rows grouped by colour, systems in order), built on first use
+  int inner_order = TFS_INNER_COLOURED;
+  tfs_batch *inner = nullptr;
+  int *inner_emap = nullptr;    // inner's entry k <- entry of A->val
+  int *inner_from_s = nullptr;  // S position of A for inner's S position i
 };
 
 namespace {
@@ -1467,6 +1491,142 @@ void batch_init_states(tfs_batch *B, int base, const int *cap, const double *tol
 
 }  // namespace
 
+namespace {
+
+// The device side of a batch from its concatenated 0-based block-diagonal
+// structure (B->nsys, n_of, row_off, nnz_of, nnz_off set): matrix, tiles,
+// states, SpMV shape.  `Ly` receives the host layout.  On failure the
+// caller destroys B.
+int batch_build(tfs_batch *B, long long N, long long NNZ, const int *cr, const int *cc,
+                const int *cd, Layout &Ly) {
+  int rc = matrix_create((int)N, NNZ, cr, cc, cd, 0, &B->A, &Ly);
+  if (rc < 0) return rc;
+  const int nsys = B->nsys;
+  // every system occupies one contiguous S range (a connected block is laid
+  // out contiguously by RCM; several components of one system stay adjacent)
+  std::vector<int> tsys, tbeg, tend, tfirst;
+  for (int s = 0; s < nsys; ++s) {
+    int lo = (int)N, hi = -1;
+    for (int i = 0; i < B->n_of[s]; ++i) {
+      const int k = Ly.s_of_orig[B->row_off[s] + i];
+      lo = std::min(lo, k);
+      hi = std::max(hi, k);
+    }
+    if (hi - lo + 1 != B->n_of[s]) return TFS_ERR_INVALID;
+    tfirst.push_back((int)tsys.size());
+    for (int b = lo; b <= hi; b += kTileRows) {
+      tsys.push_back(s);
+      tbeg.push_back(b);
+      tend.push_back(std::min(hi + 1, b + kTileRows));
+    }
+  }
+  tfirst.push_back((int)tsys.size());
+  B->ntiles = (int)tsys.size();
+  tfs_matrix *A = B->A;
+  if ((rc = upload(A, &B->tile_sys, tsys)) < 0 || (rc = upload(A, &B->tile_beg, tbeg)) < 0 ||
+      (rc = upload(A, &B->tile_end, tend)) < 0 || (rc = upload(A, &B->tile_first, tfirst)) < 0 ||
+      (rc = batch_alloc(B, (void **)&B->d_st, sizeof(KState) * (2 * nsys + 2))) < 0 ||
+      (rc = batch_alloc(B, (void **)&B->partials, sizeof(double) * std::max(1, B->ntiles))) < 0)
+    return rc;
+  if (hipHostMalloc((void **)&B->h_st, sizeof(KState) * (2 * nsys + 2), hipHostMallocDefault) !=
+      hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess)
+    return TFS_ERR_NOMEM;
+  {
+    // the batch's SpMV only forms products (its reductions are per tile), so
+    // the tuned shape cannot change any result
+    StreamScope scope(A->stream);
+    if ((rc = tune_spmv(A)) < 0) return rc;
+  }
+  return 0;
+}
+
+}  // namespace
+
+namespace {
+
+// The colour-ordered copy of a batch (see build_inner for one matrix): the
+// block-diagonal S graph coloured greedily in S order, each system's rows
+// sorted by colour, the systems kept in their order, so that the copy is a
+// batch of the same systems whose sweeps have one level per colour.
+int build_batch_inner(tfs_batch *B) {
+  if (B->inner) return 0;
+  tfs_matrix *A = B->A;
+  const int n = A->n, ns = B->nsys;
+  const long long nnz = A->nnz;
+  std::vector<int> row(n + 1), col((size_t)nnz), src((size_t)nnz), orig(n);
+  TFS_HIP(hipMemcpyAsync(row.data(), A->s_row, sizeof(int) * (n + 1), hipMemcpyDeviceToHost,
+                         cur_stream()));
+  TFS_HIP(hipMemcpyAsync(col.data(), A->s_col, sizeof(int) * nnz, hipMemcpyDeviceToHost,
+                         cur_stream()));
+  TFS_HIP(hipMemcpyAsync(src.data(), A->s_src, sizeof(int) * nnz, hipMemcpyDeviceToHost,
+                         cur_stream()));
+  TFS_HIP(hipMemcpyAsync(orig.data(), A->s_orig, sizeof(int) * n, hipMemcpyDeviceToHost,
+                         cur_stream()));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
+  std::vector<int> perm, colour;
+  colour_order(n, row.data(), col.data(), perm, &colour);
+  // system of each S row (from its caller row), then (system, colour, S) order
+  std::vector<int> sys_of(n);
+  for (int k = 0; k < n; ++k) {
+    const int c = orig[k];
+    sys_of[k] = (int)(std::upper_bound(B->row_off.begin(), B->row_off.end(), c) -
+                      B->row_off.begin()) - 1;
+  }
+  std::vector<int> order(n);
+  for (int k = 0; k < n; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return sys_of[a] != sys_of[b] ? sys_of[a] < sys_of[b] : colour[a] < colour[b];
+  });
+  std::vector<int> pos(n);
+  for (int i = 0; i < n; ++i) pos[order[i]] = i;
+  std::vector<int> prow(n + 1, 0), pcol((size_t)nnz), pdia(n, -1), emap((size_t)nnz);
+  std::vector<std::pair<int, int>> ent;
+  long long q = 0;
+  for (int i = 0; i < n; ++i) {
+    const int k0 = order[i];
+    ent.clear();
+    for (int k = row[k0]; k < row[k0 + 1]; ++k) ent.emplace_back(pos[col[k]], src[k]);
+    std::sort(ent.begin(), ent.end());
+    for (const auto &e : ent) {
+      if (e.first == i) pdia[i] = (int)q;
+      pcol[(size_t)q] = e.first;
+      emap[(size_t)q] = e.second;
+      ++q;
+    }
+    prow[i + 1] = (int)q;
+  }
+  auto *C = new tfs_batch;
+  C->nsys = ns;
+  C->n_of = B->n_of;
+  C->row_off = B->row_off;
+  // a system's entries stay one contiguous range in the copy
+  long long eo = 0;
+  for (int s = 0; s < ns; ++s) {
+    C->nnz_of.push_back(B->nnz_of[s]);
+    C->nnz_off.push_back(eo);
+    eo += B->nnz_of[s];
+  }
+  Layout Ly;
+  int rc = batch_build(C, n, nnz, prow.data(), pcol.data(), pdia.data(), Ly);
+  if (rc < 0) {
+    tfs_batch_destroy(C);
+    return rc;
+  }
+  std::vector<int> from_s(n);
+  for (int i = 0; i < n; ++i) from_s[i] = order[Ly.s_orig[i]];
+  if ((rc = upload(A, &B->inner_emap, emap)) < 0 || (rc = upload(A, &B->inner_from_s, from_s)) < 0) {
+    tfs_batch_destroy(C);
+    return rc;
+  }
+  C->A->persistent = A->persistent;
+  C->A->dynamic = A->dynamic;
+  B->inner = C;
+  A->inner_ready = false;
+  return 0;
+}
+
+}  // namespace
+
 extern "C" int tfs_batch_create(int nsys, const int *n, const long long *nnz,
                                 const int *const *row, const int *const *col,
                                 const int *const *dia, int base, tfs_batch **out) {
@@ -1511,55 +1671,10 @@ extern "C" int tfs_batch_create(int nsys, const int *n, const long long *nnz,
   }
   cr[N] = (int)NNZ;
   Layout Ly;
-  int rc = matrix_create((int)N, NNZ, cr.data(), cc.data(), cd.data(), 0, &B->A, &Ly);
+  const int rc = batch_build(B, N, NNZ, cr.data(), cc.data(), cd.data(), Ly);
   if (rc < 0) {
-    delete B;
-    return rc;
-  }
-  // every system occupies one contiguous S range (a connected block is laid
-  // out contiguously by RCM; several components of one system stay adjacent)
-  std::vector<int> tsys, tbeg, tend, tfirst;
-  for (int s = 0; s < nsys; ++s) {
-    int lo = (int)N, hi = -1;
-    for (int i = 0; i < n[s]; ++i) {
-      const int k = Ly.s_of_orig[B->row_off[s] + i];
-      lo = std::min(lo, k);
-      hi = std::max(hi, k);
-    }
-    if (hi - lo + 1 != n[s]) {
-      tfs_batch_destroy(B);
-      return TFS_ERR_INVALID;
-    }
-    tfirst.push_back((int)tsys.size());
-    for (int b = lo; b <= hi; b += kTileRows) {
-      tsys.push_back(s);
-      tbeg.push_back(b);
-      tend.push_back(std::min(hi + 1, b + kTileRows));
-    }
-  }
-  tfirst.push_back((int)tsys.size());
-  B->ntiles = (int)tsys.size();
-  tfs_matrix *A = B->A;
-  if ((rc = upload(A, &B->tile_sys, tsys)) < 0 || (rc = upload(A, &B->tile_beg, tbeg)) < 0 ||
-      (rc = upload(A, &B->tile_end, tend)) < 0 || (rc = upload(A, &B->tile_first, tfirst)) < 0 ||
-      (rc = batch_alloc(B, (void **)&B->d_st, sizeof(KState) * (2 * nsys + 2))) < 0 ||
-      (rc = batch_alloc(B, (void **)&B->partials, sizeof(double) * std::max(1, B->ntiles))) < 0) {
     tfs_batch_destroy(B);
     return rc;
-  }
-  if (hipHostMalloc((void **)&B->h_st, sizeof(KState) * (2 * nsys + 2), hipHostMallocDefault) !=
-      hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) {
-    tfs_batch_destroy(B);
-    return TFS_ERR_NOMEM;
-  }
-  {
-    // the batch's SpMV only forms products (its reductions are per tile), so
-    // the tuned shape cannot change any result
-    StreamScope scope(A->stream);
-    if ((rc = tune_spmv(A)) < 0) {
-      tfs_batch_destroy(B);
-      return rc;
-    }
   }
   *out = B;
   return TFS_OK;
@@ -1567,6 +1682,7 @@ extern "C" int tfs_batch_create(int nsys, const int *n, const long long *nnz,
 
 extern "C" int tfs_batch_destroy(tfs_batch *B) {
   if (!B) return TFS_OK;
+  if (B->inner) tfs_batch_destroy(B->inner);
   if (B->A && B->A->stream) (void)hipStreamSynchronize(B->A->stream);
   for (auto &g : B->graph)
     if (g) (void)hipGraphExecDestroy(g);
@@ -1669,15 +1785,29 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
   inner_rtol = std::min(0.5, std::max(1e-6, inner_rtol));
   tfs_matrix *A = B->A;
   const int n = A->n, ns = B->nsys;
+  // the inner solves run on I: this batch, or its colour-ordered copy (prec 2)
+  const bool coloured = B->inner_order == TFS_INNER_COLOURED && prec == 2;
+  tfs_batch *I = B;
+  if (coloured) {
+    TFS_TRY(build_batch_inner(B));
+    I = B->inner;
+    if (!A->inner_ready) {
+      gather((int)A->nnz, A->val, B->inner_emap, I->A->val);
+      values_changed(I->A);
+      A->inner_ready = true;
+    }
+  }
+  const int *to_s = coloured ? B->inner_from_s : nullptr;  // I's S position -> B's
   TFS_TRY(ensure_vec64(A, 7));
-  TFS_TRY(ensure_vec32(A, 7));
+  TFS_TRY(ensure_vec32(I->A, 7));
   TFS_TRY(prepare_values<double>(A));
-  TFS_TRY(prec_form<float>(A, prec));
+  TFS_TRY(prec_form<float>(I->A, prec));
   const Vecs<double> D = vecs_of(A->vec64, n);
-  const Vecs<float> F = vecs_of(A->vec32, n);
-  KState *si = B->d_st, *so = B->d_st + ns, *alli = B->d_st + 2 * ns, *allo = alli + 1;
-  KState *hi = B->h_st, *ho = B->h_st + ns;
-  const Tiles tl = tiles_of(B);
+  const Vecs<float> F = vecs_of(I->A->vec32, n);
+  KState *so = B->d_st + ns, *allo = B->d_st + 2 * ns + 1;
+  KState *si = I->d_st, *alli = I->d_st + 2 * ns;
+  KState *hi = I->h_st, *ho = B->h_st + ns;
+  const Tiles tl = tiles_of(B), tli = tiles_of(I);
   const int g = vgrid(n);
   TFS_TRY(batch_in(B, d_x, A->stage_x, D.x));
   TFS_TRY(batch_in(B, d_r1, A->stage_r, D.z));  // b
@@ -1713,7 +1843,7 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
     std::vector<int> icap(ns, 100000);
     std::vector<double> itol(ns);
     for (int s = 0; s < ns; ++s) itol[s] = std::max(inner_rtol * err[s], 0.5 * tol[s]);
-    batch_init_states(B, 0, icap.data(), itol.data());
+    batch_init_states(I, 0, icap.data(), itol.data());
     int ndone = 0;
     for (int s = 0; s < ns; ++s)
       if (!active[s]) {
@@ -1721,23 +1851,23 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
         ++ndone;
       }
     hi[2 * ns].ctr[0] = (unsigned)ndone;
-    TFS_TRY(batch_write(B));
-    gather(n, D.r, (const int *)nullptr, F.r);
+    TFS_TRY(batch_write(I));
+    gather(n, D.r, to_s, F.r);
     hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.x, 0.0f);
-    hipLaunchKernelGGL(k_sumsq_seg<float>, dim3(B->ntiles), dim3(kRedThreads), 0, cur_stream(),
-                       tl, F.r, (int)EPI_INIRES, si, alli, B->partials);
-    finish_seg(B, EPI_INIRES, si, alli);
+    hipLaunchKernelGGL(k_sumsq_seg<float>, dim3(I->ntiles), dim3(kRedThreads), 0, cur_stream(),
+                       tli, F.r, (int)EPI_INIRES, si, alli, I->partials);
+    finish_seg(I, EPI_INIRES, si, alli);
     TFS_HIP(hipGetLastError());
-    TFS_TRY(batch_read(B));
-    TFS_TRY(batch_iterate<float>(B, prec, F, si, alli, 100000));
+    TFS_TRY(batch_read(I));
+    TFS_TRY(batch_iterate<float>(I, prec, F, si, alli, 100000));
     for (int s = 0; s < ns; ++s)
       if (active[s]) {
         ++outer[s];
         inner_iters[s] += std::min(hi[s].niter_out, 100000);
         if (hi[s].status == kStBreakdown) st_out[s] = TFS_ERR_BREAKDOWN;
       }
-    hipLaunchKernelGGL(k_add_correction_all, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x,
-                       F.x);
+    hipLaunchKernelGGL(k_add_correction, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x, F.x,
+                       to_s);
     TFS_TRY(outer_residual());
     for (int s = 0; s < ns; ++s) {
       if (!active[s]) continue;
@@ -1753,6 +1883,7 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
   TFS_HIP(hipGetLastError());
   TFS_HIP(hipStreamSynchronize(cur_stream()));
   TFS_TRY(check_sweep(A));
+  if (I != B) TFS_TRY(check_sweep(I->A));
   int worst = TFS_OK;
   for (int s = 0; s < ns; ++s) {
     ini_res[s] = err0[s];
@@ -1763,6 +1894,12 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
       worst = status[s];
   }
   return worst;
+}
+
+extern "C" int tfs_batch_set_inner_order(tfs_batch *B, int order) {
+  if (!B || (order != TFS_INNER_REFERENCE && order != TFS_INNER_COLOURED)) return TFS_ERR_INVALID;
+  B->inner_order = order;
+  return TFS_OK;
 }
 
 extern "C" int tfs_batch_get_info(const tfs_batch *B, int *nsys, int *n_total,

@@ -90,7 +90,8 @@ def _lib():
     lib.tfs_prec_diag_device.argtypes = [_VP, _VP]
     lib.tfs_spmv_time.argtypes = [_VP, _VP, _VP, ctypes.c_int, _DP]
     lib.tfs_solve_mixed_device.argtypes = [_VP, ctypes.c_int, _VP, _VP, _IP, _IP,
-                                           ctypes.c_double, ctypes.c_double, _DP, _DP]
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                           _DP, _DP]
     lib.tfs_prec_diag32_device.argtypes = [_VP, ctypes.c_int, _VP]
     lib.tfs_matrix_set_inner_order.argtypes = [_VP, ctypes.c_int]
     lib.tfs_matrix_tune_spmv.argtypes = [_VP, _IP, _IP]
@@ -293,11 +294,12 @@ def solve(method, A, x, r1, prec, niter, tol, ini_res=0.0):
     return _solve(_METHODS.get(method, method), A, x, r1, prec, niter, tol, ini_res)
 
 
-def solve_mixed(A, x, r1, prec, niter, tol, inner_rtol=1e-4):
+def solve_mixed(A, x, r1, prec, niter, tol, inner_rtol=1e-4, x_rtol=0.0):
     """FP32-inner PCG inside FP64 iterative refinement (GPU tensors only).
 
     Returns (steps, inner_iterations, ini_res, fin_res); fin_res is the true
-    FP64 residual ||b - A x||, which is also left in r1."""
+    FP64 residual ||b - A x||, which is also left in r1.  x_rtol > 0 also
+    requires the last correction to satisfy ||e|| <= x_rtol ||x||."""
     it = ctypes.c_int(int(niter))
     inner = ctypes.c_int(0)
     ir = ctypes.c_double(0.0)
@@ -305,7 +307,8 @@ def solve_mixed(A, x, r1, prec, niter, tol, inner_rtol=1e-4):
     _sync()
     rc = A._lib.tfs_solve_mixed_device(A.handle, prec, _dptr(x, A.n, "x"), _dptr(r1, A.n, "r1"),
                                        ctypes.byref(it), ctypes.byref(inner), float(tol),
-                                       float(inner_rtol), ctypes.byref(ir), ctypes.byref(fr))
+                                       float(inner_rtol), float(x_rtol), ctypes.byref(ir),
+                                       ctypes.byref(fr))
     A.status = _check(rc, "solve_mixed")
     return it.value, inner.value, ir.value, fr.value
 
@@ -350,6 +353,7 @@ class Batch:
                                                          _VP, _VP]
             lib.tfs_batch_get_info.argtypes = [_VP, _IP, _IP, ctypes.POINTER(ctypes.c_longlong),
                                                _IP, _IP, _IP]
+            lib.tfs_batch_set_inner_order.argtypes = [_VP, ctypes.c_int]
             lib._tfs_batch_typed = True
         self._lib = lib
         self._keep = []
@@ -386,6 +390,10 @@ class Batch:
     def set_values(self, s, val):
         val = np.ascontiguousarray(val, np.float64)
         _check(self._lib.tfs_batch_set_values(self._h, s, val), "tfs_batch_set_values")
+
+    def set_inner_order(self, order):
+        _check(self._lib.tfs_batch_set_inner_order(self._h, int(order)),
+               "tfs_batch_set_inner_order")
 
     def info(self):
         ns, nt, gl, sr, sw = (ctypes.c_int() for _ in range(5))
