@@ -111,10 +111,11 @@ int tfs_solve_device(tfs_matrix *A, int method, int prec, double *d_x, double *d
 
 /* ---- FP32-inner solve inside FP64 iterative refinement -------------------------
  * Not in the reference: values and inner Krylov vectors stream in FP32 while
- * the outer residual b - A x, the solution update and every norm stay FP64
- * on the device.  Defect correction:  r = b - A x;  repeat { solve A e = r by
- * FP32 PCG (the prec factor recomputed in FP32 from FP32 values) to
- * ||r_inner|| < inner_rtol ||r||;  x += e;  r = b - A x }  until ||r|| < tol.
+ * the outer residual b - A x, the solution update and every dot product stay
+ * FP64 on the device.  r = b - A x;  repeat { solve A z = r by FP32 PCG (the
+ * prec factor recomputed in FP32 from FP32 values) to ||r_inner|| <
+ * inner_rtol ||r||;  update x and r from z (see tfs_matrix_set_refinement) }
+ * until ||r|| < tol.
  * niter: in = cap on refinement steps; out = steps taken (cap+1 if tol was not
  * reached, which also happens when FP32 stops reducing the defect);
  * inner_iters = FP32 PCG iterations over all steps; ini_res = ||b - A x0||;
@@ -151,6 +152,16 @@ int tfs_matrix_set_spmv_shape(tfs_matrix *A, int rows_per_block, int waves_per_g
  *                        the refinement still converges to the FP64 tol. */
 enum { TFS_INNER_REFERENCE = 0, TFS_INNER_COLOURED = 1 };
 int tfs_matrix_set_inner_order(tfs_matrix *A, int order);
+/* Outer loop of the refinement:
+ *   TFS_REFINE_FCG     (default) flexible CG in FP64 with the FP32 inner solve
+ *                      as its (variable) preconditioner: x and r are updated
+ *                      along A-orthogonalised directions, so convergence does
+ *                      not need cond(A) * 2^-24 < 1;
+ *   TFS_REFINE_DEFECT  classical defect correction x += z, r = b - A x, which
+ *                      contracts by about cond(A) * 2^-24 per step.
+ * Both stop on the true FP64 residual. */
+enum { TFS_REFINE_FCG = 0, TFS_REFINE_DEFECT = 1 };
+int tfs_matrix_set_refinement(tfs_matrix *A, int outer);
 
 /* ---- batched solves ----------------------------------------------------------------
  * Not in the reference (its callers solve one system at a time): many

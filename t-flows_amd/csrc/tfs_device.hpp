@@ -20,7 +20,7 @@ struct KState {
   double rho, rho_old, alpha, beta, pq, err, bnrm2, tol, ini_res, fin_res;
   int iter, cap, done, niter_out, status, pad;
   unsigned int ctr[4];
-  double aux[2];  // norms kept for the host (no state transition)
+  double aux[6];  // dot products and norms kept for the host (no state transition)
 };
 
 // status values mirrored from the public header

@@ -40,7 +40,13 @@ constexpr int kChunk = 256;              // positions per persistent-sweep chunk
 constexpr int kPrefetch = 8;             // row entries held in registers
 constexpr unsigned kSpinLimit = 1u << 20;
 
-enum Epi { EPI_BNRM = 0, EPI_INIRES, EPI_RHO, EPI_ALPHA, EPI_NORM, EPI_AUX0, EPI_AUX1 };
+// EPI_AUX0..5 store the plain sum into aux[k] (dot products), EPI_NAUX0..5
+// its square root (norms).
+enum Epi {
+  EPI_BNRM = 0, EPI_INIRES, EPI_RHO, EPI_ALPHA, EPI_NORM,
+  EPI_AUX0, EPI_AUX1, EPI_AUX2, EPI_AUX3, EPI_AUX4, EPI_AUX5,
+  EPI_NAUX0, EPI_NAUX1, EPI_NAUX2, EPI_NAUX3, EPI_NAUX4, EPI_NAUX5
+};
 
 // Reductions that run whatever the solve's state (the rest return at once
 // once the solve is done).
@@ -93,9 +99,12 @@ __device__ inline void epilogue(int epi, double tot, KState *st) {
       }
       break;
     }
-    case EPI_AUX0:
-    case EPI_AUX1:
-      st->aux[epi - EPI_AUX0] = sqrt(tot);
+    case EPI_AUX0: case EPI_AUX1: case EPI_AUX2: case EPI_AUX3: case EPI_AUX4: case EPI_AUX5:
+      st->aux[epi - EPI_AUX0] = tot;
+      break;
+    case EPI_NAUX0: case EPI_NAUX1: case EPI_NAUX2: case EPI_NAUX3: case EPI_NAUX4:
+    case EPI_NAUX5:
+      st->aux[epi - EPI_NAUX0] = sqrt(tot);
       break;
     default:
       break;
@@ -135,6 +144,36 @@ k_sumsq(int n, const T *__restrict__ v, int epi, const KState *st, double *parti
   TFS_GRID_STRIDE(i, n) {
     const double a = v[i];
     s += a * a;
+  }
+  store_partial<kRedThreads>(s, partials);
+}
+
+// Plain dot product with per-workgroup partials (outer loop of the
+// refinement; the host reads the result).
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_dot(int n, const T *__restrict__ a, const T *__restrict__ b, double *partials) {
+  double s = 0.0;
+  TFS_GRID_STRIDE(i, n) s += (double)a[i] * (double)b[i];
+  store_partial<kRedThreads>(s, partials);
+}
+
+// p = z + beta p
+__global__ void __launch_bounds__(kRedThreads)
+k_xpby(int n, const double *__restrict__ z, double *__restrict__ p, double beta) {
+  TFS_GRID_STRIDE(i, n) p[i] = z[i] + beta * p[i];
+}
+
+// x += alpha p; r -= alpha q; partial ||r||^2
+__global__ void __launch_bounds__(kRedThreads)
+k_update_xr(int n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
+            const double *__restrict__ q, double alpha, double *partials) {
+  double s = 0.0;
+  TFS_GRID_STRIDE(i, n) {
+    x[i] = x[i] + alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    s += ri * ri;
   }
   store_partial<kRedThreads>(s, partials);
 }

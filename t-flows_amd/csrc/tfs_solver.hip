@@ -165,6 +165,7 @@ struct tfs_matrix {
   // FP32 inner solves of the refinement on a colour-ordered copy P A P^T,
   // P a greedy colouring of the S-order graph (built on first use)
   int inner_order = TFS_INNER_COLOURED;
+  int refine = TFS_REFINE_FCG;   // outer loop of the refinement
   tfs_matrix *inner = nullptr;
   int *inner_emap = nullptr;     // inner's caller-order entry k <- entry of val
   int *inner_from_s = nullptr;   // S position of this matrix for inner's S position i
@@ -645,16 +646,19 @@ int build_inner(tfs_matrix *A) {
   return 0;
 }
 
-// FP32-inner PCG inside FP64 iterative refinement (defect correction).
-//   r = b - A x (FP64);  repeat: solve A e = r in FP32 (PCG with the factor
-//   recomputed in FP32 from FP32 values) to ||r_in|| < inner_rtol ||r||;
-//   x += e (FP64); r = b - A x (FP64)  -- until ||r|| < tol, and, if x_rtol
-//   > 0, until the last correction is also small: ||e|| <= x_rtol ||x||.
-// Each step contracts the error by about cond(A) * 2^-24 (the FP32 rounding
-// of A), so on an ill-conditioned system the residual can pass tol while
-// components of the error along the smallest eigenvectors remain; the
-// correction size bounds them (it is the previous iterate's error).
-// The outer residual, the update and every norm stay in FP64 on the device.
+// FP32-inner PCG inside an FP64 outer loop (iterative refinement).
+//   The outer loop is flexible CG (Notay's FCG(1)) in FP64: r = b - A x;
+//   z = B(r), the FP32 PCG solve of A z = r (values, factor and vectors in
+//   FP32) to ||r_in|| < inner_rtol ||r||; p = z + beta p with
+//   beta = -(z, A p_prev) / (p_prev, A p_prev); alpha = (p, r) / (p, A p);
+//   x += alpha p; r -= alpha A p (FP64 SpMV) -- until ||r|| < tol, checked on
+//   the true residual b - A x.  With TFS_REFINE_DEFECT the outer loop is plain
+//   defect correction (x += z).  Defect correction contracts the error by
+//   about cond(A) * 2^-24 per step (the FP32 rounding of A), which stalls near
+//   cond(A) ~ 1e7; the Krylov outer loop only needs the inner solve to be a
+//   good preconditioner, whatever the conditioning.
+//   x_rtol > 0: the last update must also satisfy ||dx|| <= x_rtol ||x||.
+// The outer residual, updates and every dot product stay in FP64 on the device.
 int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *niter,
                        int *inner_iters, double tol, double inner_rtol, double x_rtol,
                        double *ini_res, double *fin_res) {
@@ -673,6 +677,7 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
   // the factor's order matters (prec 2).  The diagonal preconditioners do not
   // depend on the order.
   const bool coloured = A->inner_order == TFS_INNER_COLOURED && prec == 2;
+  const bool krylov = A->refine == TFS_REFINE_FCG;
   tfs_matrix *I = A;
   if (coloured) {
     TFS_TRY(build_inner(A));
@@ -684,32 +689,52 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
     }
   }
   const int *to_s = coloured ? A->inner_from_s : nullptr;  // I's S position -> A's
-  TFS_TRY(ensure_vec64(A, 7));
+  TFS_TRY(ensure_vec64(A, 8));
   TFS_TRY(ensure_vec32(I, 7));
   TFS_TRY(prepare_values<double>(A));
   TFS_TRY(prec_form<float>(I, prec));
-  const Vecs<double> D = vecs_of(A->vec64, n);   // x, r (residual), z (b)
-  const Vecs<float> F = vecs_of(I->vec32, n);    // inner solve: x = correction e
+  // FP64: x, r (residual), z (inner result), p, q = A p, r2 slot = b.  The
+  // sweep slots (yL, zU) are not touched: they stay armed for FP64 solves.
+  const Vecs<double> D = vecs_of(A->vec64, n);
+  double *bS = D.r2;
+  const Vecs<float> F = vecs_of(I->vec32, n);    // inner solve: x = correction
   KState *so = A->d_st + 1, *si = I->d_st;
+  KState *ho = A->h_st + 1;
   const int g = vgrid(n);
   gather(n, x, A->s_orig, D.x);
-  gather(n, r1, A->s_orig, D.z);  // b kept for the outer residuals
-  // outer residual r = b - A x and its norm (FP64); the norm lands in bnrm2
-  auto outer_residual = [&]() {
-    gather(n, D.z, (const int *)nullptr, D.r);
-    launch_spmv<double, SPMV_RESID>(A, D.x, D.r, nullptr, nullptr, nullptr, nullptr);
-    hipLaunchKernelGGL(k_sumsq<double>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.r,
-                       (int)EPI_BNRM, so, A->partials);
-    finish(g, A->partials, EPI_BNRM, so);
+  gather(n, r1, A->s_orig, bS);
+  auto dot = [&](const double *u, const double *v, int epi) {
+    hipLaunchKernelGGL(k_dot<double>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, u, v,
+                       A->partials);
+    finish(g, A->partials, epi, so);
   };
-  TFS_TRY(init_state(A, 1, tol, cap, 0.0));
-  outer_residual();
-  TFS_HIP(hipGetLastError());
-  TFS_TRY(read_state(A, 1));
-  KState *ho = A->h_st + 1;
+  auto sumsq = [&](const double *u, int epi) {
+    hipLaunchKernelGGL(k_sumsq<double>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, u, epi,
+                       so, A->partials);
+    finish(g, A->partials, epi, so);
+  };
+  // true residual r = b - A x and its norm (FP64) into bnrm2; with
+  // update_norms also ||z|| and ||x|| (defect correction's update criterion)
+  auto true_residual = [&](bool update_norms = false) -> int {
+    TFS_TRY(init_state(A, 1, tol, cap, 0.0));
+    if (update_norms) {
+      hipLaunchKernelGGL(k_sumsq<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.x,
+                         (int)EPI_NAUX4, so, A->partials);
+      finish(g, A->partials, EPI_NAUX4, so);
+      sumsq(D.x, EPI_NAUX5);
+    }
+    gather(n, bS, (const int *)nullptr, D.r);
+    launch_spmv<double, SPMV_RESID>(A, D.x, D.r, nullptr, nullptr, nullptr, nullptr);
+    sumsq(D.r, EPI_BNRM);
+    TFS_HIP(hipGetLastError());
+    return read_state(A, 1);
+  };
+  TFS_TRY(true_residual());
   const double err0 = ho->bnrm2;
   double err = err0;
-  bool small_update = x_rtol <= 0.0;  // no criterion on the correction
+  bool small_update = x_rtol <= 0.0;  // no criterion on the update
+  bool have_p = false;                // a previous direction to orthogonalise against
+  double pq_prev = 0.0;
   int outer = 0, status = TFS_OK, stalls = 0;
   while (err >= tol || !small_update) {
     if (outer >= cap) {
@@ -717,9 +742,9 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
       break;
     }
     ++outer;
-    // inner solve, FP32: e = 0, r32 = r (in I's S order); below tol (only
-    // while the correction criterion is pending) the inner target is relative
-    const double inner_tol = err >= tol ? std::max(inner_rtol * err, 0.5 * tol) : inner_rtol * err;
+    // inner solve, FP32: e = 0, r32 = r (in I's S order)
+    const double inner_tol =
+        krylov || err < tol ? inner_rtol * err : std::max(inner_rtol * err, 0.5 * tol);
     gather(n, D.r, to_s, F.r);
     hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.x, 0.0f);
     TFS_TRY(init_state(I, 0, inner_tol, 100000, 0.0));
@@ -734,34 +759,76 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
       status = TFS_ERR_BREAKDOWN;
       break;
     }
-    // x += e; r = b - A x; ||r|| (and ||e||, ||x|| for the correction criterion)
-    hipLaunchKernelGGL(k_add_correction, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x,
-                       F.x, to_s);
-    TFS_TRY(init_state(A, 1, tol, cap, 0.0));
-    if (!small_update) {
-      hipLaunchKernelGGL(k_sumsq<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.x,
-                         (int)EPI_AUX0, so, A->partials);
-      finish(g, A->partials, EPI_AUX0, so);
-      hipLaunchKernelGGL(k_sumsq<double>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x,
-                         (int)EPI_AUX1, so, A->partials);
-      finish(g, A->partials, EPI_AUX1, so);
+    double e_new = 0.0;
+    if (!krylov) {
+      // defect correction: x += z; r = b - A x
+      hipLaunchKernelGGL(k_add_correction, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x,
+                         F.x, to_s);
+      TFS_TRY(true_residual(!small_update));
+      e_new = ho->bnrm2;
+      if (!small_update) small_update = ho->aux[4] <= x_rtol * ho->aux[5];
+    } else {
+      // z (FP64, this matrix's S order) from the inner result
+      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.z, 0.0);
+      hipLaunchKernelGGL(k_add_correction, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.z,
+                         F.x, to_s);
+      if (have_p) {
+        dot(D.z, D.q, EPI_AUX0);  // (z, A p_prev)
+        TFS_HIP(hipGetLastError());
+        TFS_TRY(read_state(A, 1));
+        const double beta = -ho->aux[0] / pq_prev;
+        hipLaunchKernelGGL(k_xpby, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.z, D.p, beta);
+      } else {
+        gather(n, D.z, (const int *)nullptr, D.p);
+      }
+      launch_spmv<double, SPMV_DOT>(A, D.p, D.q, D.p, nullptr, nullptr, nullptr);  // q = A p
+      finish(spmv_parts(A), A->partials, EPI_AUX1, so);                           // (p, q)
+      dot(D.p, D.r, EPI_AUX2);                                                     // (p, r)
+      if (!small_update) {
+        sumsq(D.p, EPI_NAUX4);
+        sumsq(D.x, EPI_NAUX5);
+      }
+      TFS_HIP(hipGetLastError());
+      TFS_TRY(read_state(A, 1));
+      const double pq = ho->aux[1], alpha = ho->aux[2] / pq;
+      if (!(pq > 0.0) || !std::isfinite(alpha)) {
+        status = TFS_ERR_BREAKDOWN;
+        break;
+      }
+      if (!small_update) small_update = std::fabs(alpha) * ho->aux[4] <= x_rtol * ho->aux[5];
+      hipLaunchKernelGGL(k_update_xr, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x, D.r,
+                         D.p, D.q, alpha, A->partials);
+      finish(g, A->partials, EPI_NAUX3, so);
+      TFS_HIP(hipGetLastError());
+      TFS_TRY(read_state(A, 1));
+      e_new = ho->aux[3];
+      pq_prev = pq;
+      have_p = true;
+      if (e_new < tol) {
+        // confirm on the true residual; if the recurrence drifted, restart from it
+        TFS_TRY(true_residual());
+        e_new = ho->bnrm2;
+        if (e_new >= tol) have_p = false;
+      }
     }
-    outer_residual();
-    TFS_HIP(hipGetLastError());
-    TFS_TRY(read_state(A, 1));
-    const double e_new = ho->bnrm2;
     if (!std::isfinite(e_new)) {
       err = e_new;
       status = TFS_ERR_BREAKDOWN;
       break;
     }
-    if (!small_update) small_update = ho->aux[0] <= x_rtol * ho->aux[1];
-    stalls = e_new > 0.9 * err ? stalls + 1 : 0;
+    // no progress: defect correction less than 10 % per step, the Krylov
+    // loop (whose residual need not fall monotonically) no fall at all
+    stalls = e_new > (krylov ? 1.0 : 0.9) * err ? stalls + 1 : 0;
     err = e_new;
     if (stalls >= 3) {  // FP32 can no longer reduce the defect
       if (err >= tol) status = TFS_NOT_CONVERGED;
-      break;             // (below tol: the correction is at the rounding floor)
+      break;             // (below tol: the update is at the rounding floor)
     }
+  }
+  if (krylov && status == TFS_OK) {
+    // the returned residual is the true one (the recurrence's may differ)
+    TFS_TRY(true_residual());
+    err = ho->bnrm2;
   }
   scatter(n, D.x, A->s_orig, x);
   scatter(n, D.r, A->s_orig, r1);
@@ -930,6 +997,10 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
     if (std::strcmp(e, "levels") == 0) A->persistent = false;
     if (std::strcmp(e, "dynamic") == 0) A->dynamic = true;
   }
+  if (const char *e = std::getenv("TFS_SWEEP_GROUPS")) {  // cap on the persistent grid
+    const int g = std::atoi(e);
+    if (g > 0) A->grid_persistent = std::min(A->grid_persistent, g);
+  }
   A->bandwidth = Ly.bandwidth;
   // the uploads and fills above ran on the null stream; the matrix stream is
   // non-blocking, so finish them before any work is queued there (the null
@@ -1091,6 +1162,12 @@ extern "C" int tfs_matrix_set_spmv_shape(tfs_matrix *A, int rows_per_block, int 
   std::vector<int> s_row;
   TFS_TRY(download_s_row(A, &s_row));
   return set_spmv_shape(A, s_row, rows_per_block, waves_per_group);
+}
+
+extern "C" int tfs_matrix_set_refinement(tfs_matrix *A, int outer) {
+  if (!A || (outer != TFS_REFINE_FCG && outer != TFS_REFINE_DEFECT)) return TFS_ERR_INVALID;
+  A->refine = outer;
+  return TFS_OK;
 }
 
 extern "C" int tfs_matrix_set_inner_order(tfs_matrix *A, int order) {

@@ -37,6 +37,7 @@ TFS_ERR_NO_DEVICE = -5
 
 CG, BICG, CGS = 0, 1, 2
 TFS_INNER_REFERENCE, TFS_INNER_COLOURED = 0, 1
+TFS_REFINE_FCG, TFS_REFINE_DEFECT = 0, 1
 _METHODS = {"cg": CG, "bicg": BICG, "cgs": CGS}
 
 _I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -94,6 +95,7 @@ def _lib():
                                            _DP, _DP]
     lib.tfs_prec_diag32_device.argtypes = [_VP, ctypes.c_int, _VP]
     lib.tfs_matrix_set_inner_order.argtypes = [_VP, ctypes.c_int]
+    lib.tfs_matrix_set_refinement.argtypes = [_VP, ctypes.c_int]
     lib.tfs_matrix_tune_spmv.argtypes = [_VP, _IP, _IP]
     lib.tfs_matrix_set_spmv_shape.argtypes = [_VP, ctypes.c_int, ctypes.c_int]
     lib.tfs_matrix_set_sweep_schedule.argtypes = [_VP, ctypes.c_int]
@@ -188,6 +190,11 @@ class Matrix:
     def set_spmv_shape(self, rows, waves):
         _check(self._lib.tfs_matrix_set_spmv_shape(self._h, int(rows), int(waves)),
                "tfs_matrix_set_spmv_shape")
+
+    def set_refinement(self, outer):
+        """TFS_REFINE_FCG (default) or TFS_REFINE_DEFECT for solve_mixed."""
+        _check(self._lib.tfs_matrix_set_refinement(self._h, int(outer)),
+               "tfs_matrix_set_refinement")
 
     def set_inner_order(self, order):
         """TFS_INNER_REFERENCE or TFS_INNER_COLOURED for solve_mixed."""

@@ -45,23 +45,26 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("outer", ["fcg", "defect"])
 @pytest.mark.parametrize("criterion", ["residual", "correction"])
 @pytest.mark.parametrize("order", ["reference", "coloured"])
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("prec", [0, 1, 2])
-def test_refinement_reaches_fp64_solution(torch, S, oracle, name, prec, order, criterion):
+def test_refinement_reaches_fp64_solution(torch, S, oracle, name, prec, order, criterion, outer):
     c = mesh.make_case(**CASES[name])
-    print(f"case {name} prec={prec} order={order} criterion={criterion} n={c.n}", flush=True)
+    print(f"case {name} prec={prec} order={order} criterion={criterion} outer={outer} n={c.n}",
+          flush=True)
     tol = 1e-10 * float(np.linalg.norm(c.b))
     ref = oracle.solve("cg", c.row, c.col, c.dia, c.val, c.b, prec=prec, niter=20000, tol=tol)
     A = S.Matrix(c.row, c.col, c.dia, c.val)
     A.set_inner_order(S.TFS_INNER_COLOURED if order == "coloured" else S.TFS_INNER_REFERENCE)
+    A.set_refinement(S.TFS_REFINE_FCG if outer == "fcg" else S.TFS_REFINE_DEFECT)
     x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
     r = torch.from_numpy(c.b.copy()).cuda()
     x_rtol = 1e-10 if criterion == "correction" else 0.0
     steps, inner, ini, fin = S.solve_mixed(A, x, r, prec, 50, tol, inner_rtol=1e-4, x_rtol=x_rtol)
     assert A.status == S.TFS_OK and fin < tol, (steps, inner, fin, tol)
-    assert 1 <= steps <= 20
+    assert 1 <= steps <= 30
     assert abs(ini - np.linalg.norm(c.b)) <= 1e-12 * np.linalg.norm(c.b)
     xs = exact_solution(c.row, c.col, c.val, c.b)
     err, err_ref = rel_l2(x.cpu().numpy(), xs), rel_l2(ref["x"], xs)
