@@ -568,7 +568,7 @@ def main():
                     help="workloads 4-5: CPU seconds of the sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0, help="workload 3: mesh scale per axis")
-    ap.add_argument("--inner-rtol", type=float, default=1e-4)
+    ap.add_argument("--inner-rtol", type=float, default=1e-2)
     ap.add_argument("--refine", choices=["fcg", "defect"], default="fcg",
                     help="workload 3: outer loop of the refinement")
     ap.add_argument("--cases", type=int, default=24)

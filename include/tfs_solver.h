@@ -120,12 +120,13 @@ int tfs_solve_device(tfs_matrix *A, int method, int prec, double *d_x, double *d
  * reached, which also happens when FP32 stops reducing the defect);
  * inner_iters = FP32 PCG iterations over all steps; ini_res = ||b - A x0||;
  * fin_res = ||b - A x|| (the true FP64 residual, left in r1).
- * x_rtol > 0 adds a stopping condition on the solution: the last correction
- * must also satisfy ||e|| <= x_rtol ||x||.  Each step contracts the error by
- * about cond(A) * 2^-24, so on an ill-conditioned system ||r|| can pass tol
- * while the error is still well above what FP64 CG leaves at the same tol;
- * x_rtol = tol / ||b|| asks for the FP64 solver's accuracy (one or two more
- * steps).  0 stops on the residual alone. */
+ * x_rtol > 0 adds a stopping condition on the solution: the last update of
+ * x must also satisfy ||dx|| <= x_rtol ||x||.  With the flexible-CG outer
+ * loop (the default) this gives the FP64 solver's accuracy at the same tol
+ * (x_rtol = tol / ||b||); defect correction's update omits the error it
+ * cannot resolve in FP32 (along the smallest eigenvectors when
+ * cond(A) * 2^-24 is not small), so there it is a weak check.  0 stops on the
+ * residual alone. */
 int tfs_solve_mixed_device(tfs_matrix *A, int prec, double *d_x, double *d_r1, int *niter,
                            int *inner_iters, double tol, double inner_rtol, double x_rtol,
                            double *ini_res, double *fin_res);

@@ -39,6 +39,7 @@ constexpr int kSmallLevel = 2048;        // a level this small joins a walked ru
 constexpr int kChunk = 256;              // positions per persistent-sweep chunk
 constexpr int kPrefetch = 8;             // row entries held in registers
 constexpr unsigned kSpinLimit = 1u << 20;
+constexpr int kSweepGroups = 1024;       // cap on a persistent sweep's workgroups
 
 // EPI_AUX0..5 store the plain sum into aux[k] (dot products), EPI_NAUX0..5
 // its square root (norms).

@@ -301,7 +301,7 @@ def solve(method, A, x, r1, prec, niter, tol, ini_res=0.0):
     return _solve(_METHODS.get(method, method), A, x, r1, prec, niter, tol, ini_res)
 
 
-def solve_mixed(A, x, r1, prec, niter, tol, inner_rtol=1e-4, x_rtol=0.0):
+def solve_mixed(A, x, r1, prec, niter, tol, inner_rtol=1e-2, x_rtol=0.0):
     """FP32-inner PCG inside FP64 iterative refinement (GPU tensors only).
 
     Returns (steps, inner_iterations, ini_res, fin_res); fin_res is the true
@@ -435,7 +435,7 @@ class Batch:
                "tfs_batch_solve_device")
         return niter, ini, fin, st
 
-    def solve_mixed(self, xs, rs, prec, cap, tols, inner_rtol=1e-4):
+    def solve_mixed(self, xs, rs, prec, cap, tols, inner_rtol=1e-2):
         """FP32-inner refinement per system; arrays (steps, inner, ini, fin, status)."""
         px, pr = self._vecs(xs, rs)
         tols = np.ascontiguousarray(tols, np.float64)

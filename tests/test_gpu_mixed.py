@@ -8,9 +8,11 @@ the exact solution (sparse LU) is printed for every case and bounded twice:
     contracts the error by about cond(A) * 2^-24, so on the aspect-100 Poisson
     case the residual passes 1e-10 ||b|| while the error is ~5e-8, where the
     oracle's FP64 Cg iterate is at ~1e-10;
-  * with the correction criterion x_rtol = 1e-10, as accurate as the oracle's
-    FP64 Cg at the same tolerance: within 10x of the oracle's own error and
-    <= 1e-8 wherever the oracle's is.
+  * with the flexible-CG outer loop (the default) and the update criterion
+    x_rtol = 1e-10, as accurate as the oracle's FP64 Cg at the same
+    tolerance: within 10x of the oracle's own error and <= 1e-8 wherever the
+    oracle's is (defect correction's update does not contain the error it
+    cannot resolve, so there the bound stays that of the residual).
 The FP32 factor is the reference recurrence in FP32: it agrees with the FP64
 oracle factor to FP32 rounding (rel 1e-5).
 """
@@ -70,9 +72,12 @@ def test_refinement_reaches_fp64_solution(torch, S, oracle, name, prec, order, c
     err, err_ref = rel_l2(x.cpu().numpy(), xs), rel_l2(ref["x"], xs)
     print(f"  steps {steps} inner {inner}: rel-L2 vs exact {err:.2e} (oracle {err_ref:.2e}), "
           f"vs oracle {rel_l2(x.cpu().numpy(), ref['x']):.2e}", flush=True)
-    if criterion == "correction":
+    if criterion == "correction" and outer == "fcg":
         assert err <= refinement_bar(ref["x"], xs)
     else:
+        # defect correction: its correction misses what FP32 cannot resolve
+        # (the error along the smallest eigenvectors), so the update criterion
+        # does not bound the error; the stated bound is that of the residual
         assert err <= max(1e-6, refinement_bar(ref["x"], xs))
     # the residual left in r1 is the true FP64 residual
     true_r = torch.from_numpy(c.b.copy()).cuda()
