@@ -173,24 +173,26 @@ def sweep_bytes(n, ntri):
     return 12 * ntri + 4 * (n + 1) + (4 + 8 + 8 + 8 + 8) * n
 
 
-def sweep_roofline(c, bd, traffic_profile=None):
-    """The preconditioner sweeps from the per-class HIP-event breakdown:
-    average launch (forward and backward alternate) against the average of
-    the two sweeps' algorithmic bytes."""
+def sweep_roofline(A, c, bd, traffic_profile=None):
+    """The preconditioner sweeps: average launch of forward and backward
+    sweeps run back to back (HIP events on the matrix's stream), against the
+    average of the two sweeps' algorithmic bytes; the in-solve per-launch
+    figure of the breakdown is kept beside it."""
     import numpy as np
     rows = np.repeat(np.arange(c.n), np.diff(c.row - c.base))
     cols = c.col - c.base
     ntri_l, ntri_u = int(np.sum(cols < rows)), int(np.sum(cols > rows))
     algo = 0.5 * (sweep_bytes(c.n, ntri_l) + sweep_bytes(c.n, ntri_u))
     sw = bd["sweeps"]
-    launch_us = 1e3 * sw["ms_per_solve"] / max(sw["launches_per_solve"], 1)
+    launch_us = 1e3 * A.sweep_time(reps=20)
+    inloop_us = 1e3 * sw["ms_per_solve"] / max(sw["launches_per_solve"], 1)
     achieved = algo / (launch_us * 1e-6) / 1e9
     return {
         "kernel": "k_sweep_syncfree (FP64 forward/backward incomplete-Cholesky sweeps)",
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": measured_traffic(traffic_profile) if traffic_profile else None,
-        "algorithmic_bytes": algo, "launch_us": launch_us,
+        "algorithmic_bytes": algo, "launch_us": launch_us, "launch_us_in_solves": inloop_us,
         "note": "dependency-latency bound (DAG depth), not bandwidth bound",
     }
 
@@ -261,7 +263,7 @@ def single_solve(args, k):
     roof = spmv_roofline(A, x, y, c.n, c.nnz,
                          inloop_ms=spmv_ms.value / max(spmv_n.value, 1),
                          traffic_profile=f"k_spmv_workload{k}_counters")
-    sroof = sweep_roofline(c, bd, traffic_profile=f"k_sweep_workload{k}_counters")
+    sroof = sweep_roofline(A, c, bd, traffic_profile=f"k_sweep_workload{k}_counters")
     log(f"  spmv {roof['launch_us']:.1f} us back-to-back ({roof['achieved']:.0f} GB/s); "
         f"sweep {sroof['launch_us']:.1f} us ({sroof['achieved']:.0f} GB/s); breakdown {bd}")
     # the dominant kernel of the solve is the one the roofline line reports
@@ -577,7 +579,7 @@ def main():
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--copy-mib", type=int, default=0,
-                    help="only run a plain device copy of this many MiB, 5 times (the "
+                    help="first run a plain device copy of this many MiB, 5 times (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -586,14 +588,15 @@ def main():
         sys.exit(2)
     import torch  # noqa: F401  (loads the HIP runtime the library shares)
     if args.copy_mib:
+        # a plain copy of known size, for scaling the HBM counters of the
+        # same profiled run (runs before the workload)
         src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(5):
             dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"calibration": "device copy", "bytes_read": src.numel() * 4,
-                          "bytes_written": src.numel() * 4, "launches": 5}), flush=True)
-        return
+        log(f"copy calibration: 5 launches of {src.numel() * 4} bytes read and written")
+        del src, dst
     load_package()
     if args.workload in (1, 2):
         out = single_solve(args, args.workload)

@@ -76,6 +76,8 @@ int tfs_matrix_set_values(tfs_matrix *A, const double *val);
 int tfs_matrix_set_values_device(tfs_matrix *A, const double *d_val);
 /* The caller rewrote the matrix's own device values in place
  * (tfs_matrix_values_device); the cached factor is dropped. */
+/* Copies the caller-order values (nnz doubles) back to host memory. */
+int tfs_matrix_get_values(tfs_matrix *A, double *val);
 int tfs_matrix_values_changed(tfs_matrix *A);
 /* Device address of the matrix's FP64 values (nnz entries). */
 double *tfs_matrix_values_device(tfs_matrix *A);
@@ -194,6 +196,32 @@ int tfs_batch_set_inner_order(tfs_batch *B, int order);
 int tfs_batch_get_info(const tfs_batch *B, int *nsys, int *n_total, long long *nnz_total,
                        int *graph_launches, int *spmv_rows, int *spmv_waves);
 
+/* ---- device-side assembly of A % val from face coefficients ------------------
+ * Replaces the face loops of Pressure_Matrix_Fractional.f90:19-48 and
+ * NewUVW.f90:349-384,500-506,586-590 (scatter through Matrix_Mod/Topology.f90
+ * pos/dia), so values built on the device need no upload:
+ *   val := 0;  for each face s, in order: interior (c2 > 0):
+ *     val(pos(1,s)) -= a12(s);  val(dia(c1)) += a12(s);
+ *     val(pos(2,s)) -= a21(s);  val(dia(c2)) += a21(s);
+ *   boundary (c2 <= 0, and bnd_on(s) != 0 if bnd_on is given):
+ *     val(dia(c1)) += a12(s);
+ *   for each cell: val(dia(c)) += cell(c);  sav(c) = val(dia(c));
+ *                  val(dia(c)) /= urf.
+ * The result is bitwise that of the sequential loop (each diagonal is summed
+ * over its faces in face order).  faces_c = the reference's faces_c(1:2,s),
+ * column-major (c1, c2 per face), pos = pos(1:2,s), dia as the topology built
+ * them, all in `base`.  a21 NULL means a21 = a12; bnd_on, cell and sav may be
+ * NULL; urf = 1 leaves the diagonal unscaled.  Device arrays: a12/a21 per
+ * face, bnd_on (bytes) per face, cell/sav per cell.  The matrix's values are
+ * replaced and marked changed, as after tfs_matrix_set_values_device. */
+typedef struct tfs_assembly tfs_assembly;
+int tfs_assembly_create(tfs_matrix *A, long long n_faces, const int *faces_c, const int *pos,
+                        const int *dia, int base, tfs_assembly **out);
+int tfs_assembly_destroy(tfs_assembly *S);
+int tfs_assemble_device(tfs_assembly *S, const double *d_a12, const double *d_a21,
+                        const unsigned char *d_bnd_on, const double *d_cell, double urf,
+                        double *d_sav);
+
 /* ---- serving: continuous batching of solves on one GPU ------------------------
  * Not in the reference.  A case (one system) is registered with its structure
  * and values; its GPU copy is made when its first solve is admitted and stays
@@ -247,6 +275,9 @@ int tfs_prec_diag32_device(tfs_matrix *A, int prec, double *d_out);
  * Runs `reps` back-to-back SpMV launches y := A x on the library's stream and
  * returns the mean launch duration in ms from HIP events on that stream. */
 int tfs_spmv_time(tfs_matrix *A, const double *d_x, double *d_y, int reps, double *ms);
+/* Average time of one FP64 incomplete-Cholesky sweep (forward and backward
+ * alternating, `reps` pairs back to back, factor formed first). */
+int tfs_sweep_time(tfs_matrix *A, int reps, double *ms);
 
 /* Kernel classes for event timing during real solves. */
 enum {

@@ -58,6 +58,9 @@ class Oracle:
         lib.or_topology_nnz.restype = ctypes.c_int
         lib.or_topology_nnz.argtypes = [ctypes.c_int, ctypes.c_int, _I32P]
         lib.or_topology.argtypes = [ctypes.c_int, ctypes.c_int, _I32P] + [_I32P] * 5
+        lib.or_assemble.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, _I32P, ctypes.c_int,
+                                    _F64P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_double, _F64P, ctypes.c_void_p]
         self.lib = lib
 
     def _mat(self, row, col, dia, val):
@@ -103,6 +106,22 @@ class Oracle:
         fn(ctypes.byref(m), x, r, prec, ctypes.byref(it), tol, ctypes.byref(ir), ctypes.byref(fr))
         return dict(x=x, r=r, niter=it.value, ini_res=ir.value, fin_res=fr.value)
 
+    def assemble(self, n_cells, faces_c, pos, dia, nnz, a12, a21=None, bnd_on=None, cell=None,
+                 urf=1.0, with_sav=False):
+        """Face-loop assembly; faces_c/pos/dia 1-based.  Returns val (and sav)."""
+        faces_c, pos, dia = _c32(faces_c), _c32(pos), _c32(dia)
+        nf = faces_c.size // 2
+        keep = [None if v is None else _c64(v) for v in (a21, cell)]
+        bo = None if bnd_on is None else np.ascontiguousarray(bnd_on, np.uint8)
+        val = np.zeros(nnz, np.float64)
+        sav = np.zeros(n_cells, np.float64) if with_sav else None
+        self.lib.or_assemble(n_cells, nf, faces_c, pos, dia, nnz, _c64(a12),
+                             None if keep[0] is None else keep[0].ctypes.data,
+                             None if bo is None else bo.ctypes.data,
+                             None if keep[1] is None else keep[1].ctypes.data, float(urf), val,
+                             None if sav is None else sav.ctypes.data)
+        return (val, sav) if with_sav else val
+
     def topology(self, n_cells, faces_c):
         faces_c = _c32(faces_c)
         nf = faces_c.size // 2
@@ -138,6 +157,9 @@ class Reference:
         lib.tfs_ref_topology.restype = ctypes.c_int
         lib.tfs_ref_topology.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _I32P,
                                          ctypes.c_int] + [_I32P] * 5
+        lib.tfs_ref_pressure_matrix.restype = ctypes.c_int
+        lib.tfs_ref_pressure_matrix.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _I32P,
+                                                ctypes.c_double, _F64P, ctypes.c_int, _F64P]
         self.lib = lib
         self.n = 0
 
@@ -184,3 +206,15 @@ class Reference:
         pos = np.zeros(2 * nf, np.int32)
         nnz = self.lib.tfs_ref_topology(n_cells, n_bnd, nf, faces_c, cap, row, col, dia, mir, pos)
         return dict(row=row, col=col[:nnz], dia=dia, mir=mir[:nnz], pos=pos)
+
+    def pressure_matrix(self, n_cells, faces_c, dt, scoef, n_bnd=None):
+        """Pressure_Matrix_Fractional on the topology of faces_c: A % val (0-based array)."""
+        faces_c = _c32(faces_c)
+        nf = faces_c.size // 2
+        if n_bnd is None:
+            n_bnd = int(max(0, -faces_c[1::2].min())) if nf else 0
+        cap = n_cells + 2 * nf
+        val = np.zeros(cap, np.float64)
+        nnz = self.lib.tfs_ref_pressure_matrix(n_cells, n_bnd, nf, faces_c, float(dt),
+                                               _c64(scoef), cap, val)
+        return val[:nnz]

@@ -27,6 +27,9 @@
  *   or_cgs                : dense solve on tiny nonsymmetric systems; bitwise
  *                           against the compiled reference.
  *   or_topology           : bitwise against the reference Matrix_Mod_Topology.
+ *   or_assemble           : bitwise against the reference Pressure_Matrix_Fractional;
+ *                           exact (integer-valued) coefficients against a dense
+ *                           sum of per-face 2x2 element matrices.
  */
 #include "tfs_oracle.h"
 
@@ -359,4 +362,32 @@ void or_topology(int n_cells, int n_faces, const int *faces_c, int *row,
     }
   }
   free(stencw);
+}
+
+/* Pressure_Matrix_Fractional.f90:19-21 (val := 0), :26-48 (face loop; the
+ * BUFFER branch is the boundary add here), NewUVW.f90:372-384 (the same loop
+ * with a12 != a21 and wall/inflow boundaries adding to the diagonal),
+ * :500-506 (inertia cell term), :586-590 (sav, then division by URF). */
+void or_assemble(int n_cells, int n_faces, const int *faces_c, const int *pos, const int *dia,
+                 int nnz, const double *a12, const double *a21, const unsigned char *bnd_on,
+                 const double *cell, double urf, double *val, double *sav) {
+  for (int k = 0; k < nnz; ++k) val[k] = 0.0;
+  for (int s = 0; s < n_faces; ++s) {
+    const int c1 = faces_c[2 * s], c2 = faces_c[2 * s + 1];
+    const double A12 = a12[s], A21 = a21 ? a21[s] : a12[s];
+    if (c2 > 0) {
+      val[pos[2 * s] - 1] = val[pos[2 * s] - 1] - A12;
+      val[dia[c1 - 1] - 1] = val[dia[c1 - 1] - 1] + A12;
+      val[pos[2 * s + 1] - 1] = val[pos[2 * s + 1] - 1] - A21;
+      val[dia[c2 - 1] - 1] = val[dia[c2 - 1] - 1] + A21;
+    } else if (!bnd_on || bnd_on[s]) {
+      val[dia[c1 - 1] - 1] = val[dia[c1 - 1] - 1] + A12;
+    }
+  }
+  for (int c = 0; c < n_cells; ++c) {
+    double *d = &val[dia[c] - 1];
+    if (cell) *d = *d + cell[c];
+    if (sav) sav[c] = *d;
+    if (urf != 1.0) *d = *d / urf;
+  }
 }

@@ -64,6 +64,17 @@ int or_topology_nnz(int n_cells, int n_faces, const int *faces_c);
 void or_topology(int n_cells, int n_faces, const int *faces_c, int *row,
                  int *col, int *dia, int *mir, int *pos);
 
+/* The face-loop assembly of A % val (Sources/Process/Pressure_Matrix_Fractional.f90:19-48;
+ * Sources/Process/NewUVW.f90:372-384 faces, :500-506 inertia, :586-590
+ * under-relaxation).  faces_c / pos / dia 1-based as the reference holds them
+ * (pos = Matrix_Mod_Topology's pos(1:2, s)); a21 NULL means a21 = a12; a
+ * boundary face adds a12 to its cell's diagonal when bnd_on is NULL or
+ * bnd_on[s] != 0; cell NULL adds nothing; sav NULL is not written.
+ * val[nnz] and sav[n] are 0-based outputs. */
+void or_assemble(int n_cells, int n_faces, const int *faces_c, const int *pos, const int *dia,
+                 int nnz, const double *a12, const double *a21, const unsigned char *bnd_on,
+                 const double *cell, double urf, double *val, double *sav);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,6 +26,7 @@
 #include "tfs_kernels.hpp"
 #include "tfs_layout.hpp"
 #include "tfs_solver.h"
+#include "tfs_internal.hpp"
 
 using namespace tfs;
 
@@ -1127,6 +1128,16 @@ extern "C" int tfs_matrix_set_values_device(tfs_matrix *A, const double *d_val) 
   return TFS_OK;
 }
 
+extern "C" int tfs_matrix_get_values(tfs_matrix *A, double *val) {
+  if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
+  if (A->nnz > 0 && !val) return TFS_ERR_INVALID;
+  TFS_HIP(hipMemcpyAsync(val, A->val, sizeof(double) * (size_t)A->nnz, hipMemcpyDeviceToHost,
+                         cur_stream()));
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
+  return TFS_OK;
+}
+
 extern "C" int tfs_matrix_values_changed(tfs_matrix *A) {
   if (!A) return TFS_ERR_INVALID;
   values_changed(A);
@@ -1134,6 +1145,8 @@ extern "C" int tfs_matrix_values_changed(tfs_matrix *A) {
 }
 
 extern "C" double *tfs_matrix_values_device(tfs_matrix *A) { return A ? A->val : nullptr; }
+
+hipStream_t tfs::matrix_stream(const tfs_matrix *A) { return A ? A->stream : nullptr; }
 
 extern "C" int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info) {
   if (!A || !info) return TFS_ERR_INVALID;
@@ -1341,6 +1354,32 @@ extern "C" int tfs_spmv_time(tfs_matrix *A, const double *d_x, double *d_y, int 
   float t = 0.f;
   TFS_HIP(hipEventElapsedTime(&t, A->ev0, A->ev1));
   *ms = (double)t / reps;
+  return TFS_OK;
+}
+
+extern "C" int tfs_sweep_time(tfs_matrix *A, int reps, double *ms) {
+  if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
+  if (!ms || reps < 1) return TFS_ERR_INVALID;
+  if (A->n == 0) {
+    *ms = 0.0;
+    return TFS_OK;
+  }
+  TFS_TRY(ensure_vec64(A, 7));
+  TFS_TRY(prec_form<double>(A, 2));
+  const Vecs<double> V = vecs_of(A->vec64, A->n);
+  hipLaunchKernelGGL(k_fill<double>, dim3(vgrid(A->n)), dim3(kRedThreads), 0, cur_stream(), A->n,
+                     V.z, 1.0);
+  dic_apply<double>(A, V.z, V.yL, V.zU, nullptr);  // warm
+  TFS_HIP(hipEventRecord(A->ev0, cur_stream()));
+  for (int k = 0; k < reps; ++k) dic_apply<double>(A, V.z, V.yL, V.zU, nullptr);
+  TFS_HIP(hipEventRecord(A->ev1, cur_stream()));
+  TFS_HIP(hipGetLastError());
+  TFS_HIP(hipEventSynchronize(A->ev1));
+  float t = 0.f;
+  TFS_HIP(hipEventElapsedTime(&t, A->ev0, A->ev1));
+  TFS_TRY(check_sweep(A));
+  *ms = (double)t / (2.0 * reps);
   return TFS_OK;
 }
 

@@ -90,11 +90,13 @@ def _lib():
     lib.tfs_prec_solve_device.argtypes = [_VP, ctypes.c_int, _VP, _VP]
     lib.tfs_prec_diag_device.argtypes = [_VP, _VP]
     lib.tfs_spmv_time.argtypes = [_VP, _VP, _VP, ctypes.c_int, _DP]
+    lib.tfs_sweep_time.argtypes = [_VP, ctypes.c_int, _DP]
     lib.tfs_solve_mixed_device.argtypes = [_VP, ctypes.c_int, _VP, _VP, _IP, _IP,
                                            ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                            _DP, _DP]
     lib.tfs_prec_diag32_device.argtypes = [_VP, ctypes.c_int, _VP]
     lib.tfs_matrix_set_inner_order.argtypes = [_VP, ctypes.c_int]
+    lib.tfs_matrix_get_values.argtypes = [_VP, _F64P]
     lib.tfs_matrix_set_refinement.argtypes = [_VP, ctypes.c_int]
     lib.tfs_matrix_tune_spmv.argtypes = [_VP, _IP, _IP]
     lib.tfs_matrix_set_spmv_shape.argtypes = [_VP, ctypes.c_int, ctypes.c_int]
@@ -212,6 +214,12 @@ class Matrix:
             rc = self._lib.tfs_matrix_set_values(self._h, val)
         _check(rc, "tfs_matrix_set_values")
 
+    def get_values(self):
+        """The caller-order values (host copy)."""
+        v = np.empty(self.nnz, np.float64)
+        _check(self._lib.tfs_matrix_get_values(self._h, v), "tfs_matrix_get_values")
+        return v
+
     def values_changed(self):
         _check(self._lib.tfs_matrix_values_changed(self._h), "tfs_matrix_values_changed")
 
@@ -252,6 +260,12 @@ class Matrix:
         _check(self._lib.tfs_prec_diag32_device(self._h, prec, _dptr(out, self.n, "out")),
                "tfs_prec_diag32_device")
         return out
+
+    def sweep_time(self, reps=20):
+        """ms per FP64 incomplete-Cholesky sweep launch, back to back."""
+        ms = ctypes.c_double()
+        _check(self._lib.tfs_sweep_time(self._h, int(reps), ctypes.byref(ms)), "tfs_sweep_time")
+        return ms.value
 
     def spmv_time(self, x, y, reps=20):
         _sync()
@@ -531,3 +545,55 @@ class Server:
         self._lib.tfs_server_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
                                    ctypes.byref(d))
         return dict(hbm_in_use=a.value, hbm_peak=b.value, resident=c.value, completed=d.value)
+
+
+class Assembly:
+    """Device-side face-loop assembly of a Matrix's values (tfs_assembly_*):
+    faces_c / pos / dia as the topology builds them (1-based by default);
+    coefficients are GPU tensors (a12, a21 per face, float64; bnd_on per face,
+    uint8; cell, sav per cell, float64)."""
+
+    def __init__(self, A, faces_c, pos, dia, base=1):
+        lib = A._lib
+        if not getattr(lib, "_tfs_asm_typed", False):
+            lib.tfs_assembly_create.argtypes = [_VP, ctypes.c_longlong, _I32P, _I32P, _I32P,
+                                                ctypes.c_int, ctypes.POINTER(_VP)]
+            lib.tfs_assembly_destroy.argtypes = [_VP]
+            lib.tfs_assemble_device.argtypes = [_VP, _VP, _VP, _VP, _VP, ctypes.c_double, _VP]
+            lib._tfs_asm_typed = True
+        self._lib = lib
+        self.A = A
+        faces_c = np.ascontiguousarray(faces_c, np.int32)
+        self.nf = faces_c.size // 2
+        h = _VP()
+        _check(lib.tfs_assembly_create(A.handle, self.nf, faces_c,
+                                       np.ascontiguousarray(pos, np.int32),
+                                       np.ascontiguousarray(dia, np.int32), base, ctypes.byref(h)),
+               "tfs_assembly_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.tfs_assembly_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def assemble(self, a12, a21=None, bnd_on=None, cell=None, urf=1.0, sav=None):
+        def ptr(t, n, what):
+            if t is None:
+                return None
+            if not _is_device_tensor(t) or t.numel() != n or not t.is_contiguous():
+                raise ValueError(f"{what}: contiguous GPU tensor of {n} elements expected")
+            return t.data_ptr()
+        n = self.A.n
+        _sync()
+        _check(self._lib.tfs_assemble_device(self._h, ptr(a12, self.nf, "a12"),
+                                             ptr(a21, self.nf, "a21"),
+                                             ptr(bnd_on, self.nf, "bnd_on"),
+                                             ptr(cell, n, "cell"), float(urf),
+                                             ptr(sav, n, "sav")), "tfs_assemble_device")

@@ -8,7 +8,7 @@ Inputs are not stored: they are regenerated from the recorded case parameters
 that a changed generator is detected.  Outputs stored per solve: x, r (the
 final residual the reference leaves in b), niter, ini_res, fin_res.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [key prefix]
 """
 import hashlib
 import json
@@ -79,8 +79,17 @@ def main():
     c = mesh.make_case(with_faces=True, **kwt)
     t = R.topology(c.n, c.faces)
     out["topology_6x5x4"] = dict(case=kwt, n=c.n, **t)
+    # pressure matrix of the same mesh (Pressure_Matrix_Fractional), seeded
+    # face coefficients Scoef and dt as inputs
+    scoef = np.random.default_rng(20261202).uniform(0.1, 2.0, c.faces.size // 2)
+    dt = 0.0137
+    out["pressure_matrix_6x5x4"] = dict(case=kwt, n=c.n, dt=dt, scoef=scoef,
+                                        val=R.pressure_matrix(c.n, c.faces, dt, scoef))
 
+    only = sys.argv[1] if len(sys.argv) > 1 else ""
     for key, rec in out.items():
+        if not key.startswith(only):
+            continue
         arrays = {k: v for k, v in rec.items() if isinstance(v, np.ndarray)}
         meta = {k: v for k, v in rec.items() if not isinstance(v, np.ndarray)}
         np.savez_compressed(HERE / f"{key}.npz", meta=np.array(json.dumps(meta)), **arrays)
