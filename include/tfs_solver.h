@@ -192,6 +192,9 @@ int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *const *d_x,
 /* Row order of the batch's FP32 inner solves, as tfs_matrix_set_inner_order
  * (default TFS_INNER_COLOURED: every system's rows grouped by colour). */
 int tfs_batch_set_inner_order(tfs_batch *B, int order);
+/* Outer loop of the batch's refinement, as tfs_matrix_set_refinement
+ * (default TFS_REFINE_FCG: flexible CG with per-system scalars). */
+int tfs_batch_set_refinement(tfs_batch *B, int outer);
 /* spmv_rows / spmv_waves: the SpMV shape the batch's tuner picked at creation. */
 int tfs_batch_get_info(const tfs_batch *B, int *nsys, int *n_total, long long *nnz_total,
                        int *graph_launches, int *spmv_rows, int *spmv_waves);

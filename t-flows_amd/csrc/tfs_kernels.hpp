@@ -36,7 +36,8 @@ constexpr int kMaxGrid = 2048;           // bound of every partial-sum grid
 constexpr int kLevelThreads = 256;       // threads per block, one level launch
 constexpr int kSmallThreads = 1024;      // one workgroup walking small levels
 constexpr int kSmallLevel = 2048;        // a level this small joins a walked run
-constexpr int kChunk = 256;              // positions per persistent-sweep chunk
+constexpr int kChunk = 256;              // threads per persistent-sweep workgroup
+constexpr int kSweepRows = 1;            // rows per thread (a chunk is kChunk * kSweepRows)
 constexpr int kPrefetch = 8;             // row entries held in registers
 constexpr unsigned kSpinLimit = 1u << 20;
 constexpr int kSweepGroups = 1024;       // cap on a persistent sweep's workgroups
@@ -639,7 +640,7 @@ __device__ __forceinline__ void coherent_store(T *p, T v) {
 // device-coherent loads and folding them into its sum strictly in the
 // caller's entry order -- so the result is bitwise the level-scheduled one.
 // Flag and data share one location, so no fence or counter is needed.
-// Workgroup w takes chunks w, w+G, ... (kChunk positions each, in level
+// Workgroup w takes chunks w, w+G, ... (kChunk*kSweepRows positions each, in level
 // order); positions depend only on lower positions, and the cooperative launch
 // keeps all G workgroups resident, so the lowest unfinished row always belongs
 // to a running workgroup whose dependencies are complete.  Every wait is
@@ -661,7 +662,9 @@ k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, i
                  unsigned int *ctr) {
   if (st && solve_done(st)) return;
   T *out = OP == OP_FACTOR ? d : x;
-  const int nchunks = (n + kChunk - 1) / kChunk;
+  constexpr int R = kSweepRows;
+  constexpr int kSpan = kChunk * R;  // positions per chunk
+  const int nchunks = (n + kSpan - 1) / kSpan;
   __shared__ int s_chunk;
   for (int c = DYN ? -1 : (int)blockIdx.x;; c += DYN ? 0 : (int)gridDim.x) {
     if (DYN) {
@@ -671,83 +674,110 @@ k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, i
       c = s_chunk;
     }
     if (c >= nchunks) break;
-    const int p = c * kChunk + (int)threadIdx.x;
-    if (p >= n) continue;
-    const int e0 = Tr.ptr[p], e1 = Tr.ptr[p + 1], len = e1 - e0;
-    int pc[kPrefetch];
-    T pv[kPrefetch];
+    // R rows per thread, kChunk apart (consecutive threads, consecutive rows):
+    // their loads are issued together and their waits overlap
+    int p[R], e0[R], len[R], used[R];
+    bool done[R];
+    T s[R], dp[R];
+    int pc[R][kPrefetch];
+    T pv[R][kPrefetch];
 #pragma unroll
-    for (int k = 0; k < kPrefetch; ++k) {
-      if (k < len) {
-        pc[k] = Tr.col[e0 + k];
-        pv[k] = Tr.val[e0 + k];
+    for (int j = 0; j < R; ++j) {
+      p[j] = c * kSpan + j * kChunk + (int)threadIdx.x;
+      done[j] = p[j] >= n;
+      e0[j] = len[j] = used[j] = 0;
+      dp[j] = s[j] = T(0);
+      if (!done[j]) {
+        e0[j] = Tr.ptr[p[j]];
+        len[j] = Tr.ptr[p[j] + 1] - e0[j];
       }
     }
-    T s, dp = T(0);
-    if (OP == OP_FACTOR) {
-      s = Tr.diag[p];
-    } else {
-      dp = d[p];
-      const int src = Tr.in_idx[p];
-      s = b[src];
-      if (OP == OP_FWD) arm[p] = Sentinel<T>::value();
-      if (OP == OP_BWD) {
-        b[src] = Sentinel<T>::value();  // this sweep is the only reader of b[src]
-        s = s / (dp + tiny<T>());
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+#pragma unroll
+      for (int k = 0; k < kPrefetch; ++k) {
+        if (k < len[j]) {
+          pc[j][k] = Tr.col[e0[j] + k];
+          pv[j][k] = Tr.val[e0[j] + k];
+        }
+      }
+      if (done[j]) continue;
+      if (OP == OP_FACTOR) {
+        s[j] = Tr.diag[p[j]];
+      } else {
+        dp[j] = d[p[j]];
+        const int src = Tr.in_idx[p[j]];
+        s[j] = b[src];
+        if (OP == OP_FWD) arm[p[j]] = Sentinel<T>::value();
+        if (OP == OP_BWD) {
+          b[src] = Sentinel<T>::value();  // this sweep is the only reader of b[src]
+          s[j] = s[j] / (dp[j] + tiny<T>());
+        }
       }
     }
     // A row's value is stored INSIDE the polling loop, in the iteration in
     // which its last dependency arrives: a chunk can hold two levels, so a
-    // row may wait on another lane of its own wavefront, and a lane that left
-    // the loop would only run its store once every lane of the wave had left.
+    // row may wait on another lane of its own wavefront (or on another row of
+    // its own thread), and a lane that left the loop would only run its store
+    // once every lane of the wave had left.
     // The first pass reads the dependencies with plain (cached) loads: within
     // a sweep a location only ever changes from the sentinel to its final
     // value, and every sentinel was stored by an earlier launch, so a plain
     // load returns either the final value or a (possibly stale) sentinel.
     // Only a sentinel sends the row to the device-coherent polls, which
     // bypass the XCD's L2 and fetch whole lines from memory.
-    int used = 0;  // entries folded into s so far, in order
     unsigned int spins = 0;
-    bool done = false;
     bool first = true;
-    while (!done) {
-      bool blocked = false;
+    for (;;) {
+      bool all_done = true;
 #pragma unroll
-      for (int k = 0; k < kPrefetch; ++k) {
-        if (!blocked && k >= used && k < len) {
-          const T v = first ? out[pc[k]] : coherent_load(&out[pc[k]]);
+      for (int j = 0; j < R; ++j) {
+        if (done[j]) continue;
+        bool blocked = false;
+#pragma unroll
+        for (int k = 0; k < kPrefetch; ++k) {
+          if (!blocked && k >= used[j] && k < len[j]) {
+            const T v = first ? out[pc[j][k]] : coherent_load(&out[pc[j][k]]);
+            if (Sentinel<T>::is(v)) {
+              blocked = true;
+            } else {
+              s[j] = OP == OP_FACTOR ? s[j] - v * pv[j][k] * pv[j][k] : s[j] - pv[j][k] * v;
+              used[j] = k + 1;
+            }
+          }
+        }
+        for (int e = e0[j] + used[j]; !blocked && e < e0[j] + len[j]; ++e) {  // beyond the prefetch
+          const T v = first ? out[Tr.col[e]] : coherent_load(&out[Tr.col[e]]);
           if (Sentinel<T>::is(v)) {
             blocked = true;
           } else {
-            s = OP == OP_FACTOR ? s - v * pv[k] * pv[k] : s - pv[k] * v;
-            used = k + 1;
+            const T a = Tr.val[e];
+            s[j] = OP == OP_FACTOR ? s[j] - v * a * a : s[j] - a * v;
+            used[j] = e - e0[j] + 1;
           }
         }
-      }
-      for (int e = e0 + used; !blocked && e < e1; ++e) {  // entries beyond the prefetch
-        const T v = first ? out[Tr.col[e]] : coherent_load(&out[Tr.col[e]]);
-        if (Sentinel<T>::is(v)) {
-          blocked = true;
+        if (!blocked) {
+          coherent_store(&out[p[j]], OP == OP_FACTOR ? T(1) / s[j] : s[j] * dp[j]);
+          done[j] = true;
         } else {
-          const T a = Tr.val[e];
-          s = OP == OP_FACTOR ? s - v * a * a : s - a * v;
-          used = e - e0 + 1;
+          all_done = false;
         }
       }
       first = false;
-      if (!blocked) {
-        coherent_store(&out[p], OP == OP_FACTOR ? T(1) / s : s * dp);
-        done = true;
-      } else if (++spins > kSpinLimit ||
-                 ((spins & 255u) == 0 && coherent_load(err) != 0)) {
+      if (all_done) break;
+      if (++spins > kSpinLimit || ((spins & 255u) == 0 && coherent_load(err) != 0)) {
         // out of patience, or another row already gave up: record the error
-        // and store a value so that this row's dependants finish too
+        // and store values so that the dependants finish too
         coherent_store(err, 1);
-        coherent_store(&out[p], OP == OP_FACTOR ? T(1) / s : s * dp);
-        done = true;
-      } else {
-        __builtin_amdgcn_s_sleep(2);
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+          if (!done[j]) {
+            coherent_store(&out[p[j]], OP == OP_FACTOR ? T(1) / s[j] : s[j] * dp[j]);
+            done[j] = true;
+          }
+        break;
       }
+      __builtin_amdgcn_s_sleep(2);
     }
   }
   if (DYN && threadIdx.x == 0) {

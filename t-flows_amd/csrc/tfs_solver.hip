@@ -158,6 +158,8 @@ struct tfs_matrix {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool persistent = true;      // one launch per sweep (else one per level)
   bool dynamic = false;        // persistent sweeps take chunks from a counter
+  bool sweep_fixed = false;    // schedule set by the caller or the environment
+  bool sweep_tuned = false;    // schedule picked by timing (first factor)
   unsigned int *sweep_ctr = nullptr;  // the dynamic schedule's counters
   int grid_persistent = 0;
   long long bytes = 0;
@@ -335,6 +337,10 @@ void values_changed(tfs_matrix *A) {
 }
 
 template <typename T>
+int tune_sweeps(tfs_matrix *A);
+int check_sweep(tfs_matrix *A);
+
+template <typename T>
 int prec_form(tfs_matrix *A, int prec) {
   if (prec < 0 || prec > 2) prec = 0;  // the reference treats any other value as none
   TFS_TRY(prepare_values<T>(A));
@@ -348,6 +354,8 @@ int prec_form(tfs_matrix *A, int prec) {
     TFS_HIP(hipGetLastError());
   }
   V.factor_prec = prec;
+  if (A->n > 0 && prec == 2 && !A->sweep_tuned && !A->sweep_fixed && A->persistent)
+    TFS_TRY(tune_sweeps<T>(A));
   return 0;
 }
 
@@ -421,6 +429,47 @@ int ensure_vectors(tfs_matrix *A, T **vec, int *have, int count) {
 }
 int ensure_vec64(tfs_matrix *A, int count) { return ensure_vectors(A, &A->vec64, &A->n_vec64, count); }
 int ensure_vec32(tfs_matrix *A, int count) { return ensure_vectors(A, &A->vec32, &A->n_vec32, count); }
+
+template <typename T> T **vec_slots(tfs_matrix *A);
+template <> double **vec_slots<double>(tfs_matrix *A) { return &A->vec64; }
+template <> float **vec_slots<float>(tfs_matrix *A) { return &A->vec32; }
+template <typename T> int *vec_count(tfs_matrix *A);
+template <> int *vec_count<double>(tfs_matrix *A) { return &A->n_vec64; }
+template <> int *vec_count<float>(tfs_matrix *A) { return &A->n_vec32; }
+
+// Both sweep schedules give the same bits; on the first factor of a matrix
+// the persistent synchronisation-free sweeps and the per-level launches are
+// timed on it (two forward/backward pairs each) and the faster is kept.  Deep
+// DAGs of small matrices favour the levels (whose small levels run in one
+// workgroup), wide ones the persistent kernel.
+template <typename T>
+int tune_sweeps(tfs_matrix *A) {
+  A->sweep_tuned = true;
+  TFS_TRY(ensure_vectors(A, vec_slots<T>(A), vec_count<T>(A), 7));
+  const Vecs<T> V = vecs_of(*vec_slots<T>(A), A->n);
+  hipLaunchKernelGGL(k_fill<T>, dim3(vgrid(A->n)), dim3(kRedThreads), 0, cur_stream(), A->n, V.z,
+                     T(1));
+  float t[2] = {0.f, 0.f};
+  for (int mode = 0; mode < 2; ++mode) {  // 0 persistent (current), 1 levels
+    A->persistent = mode == 0;
+    dic_apply<T>(A, V.z, V.yL, V.zU, nullptr);  // warm
+    TFS_HIP(hipEventRecord(A->ev0, cur_stream()));
+    for (int k = 0; k < 2; ++k) dic_apply<T>(A, V.z, V.yL, V.zU, nullptr);
+    TFS_HIP(hipEventRecord(A->ev1, cur_stream()));
+    TFS_HIP(hipGetLastError());
+    TFS_HIP(hipEventSynchronize(A->ev1));
+    TFS_HIP(hipEventElapsedTime(&t[mode], A->ev0, A->ev1));
+  }
+  A->persistent = t[0] <= t[1];
+  if (A->persistent) {
+    // the level launches left values where the persistent sweeps expect
+    // sentinels: re-arm the forward sweep's output in both precisions
+    if (A->vec64) arm(A->n, A->vec64 + 2 * (size_t)A->n);
+    if (A->vec32) arm(A->n, A->vec32 + 2 * (size_t)A->n);
+  }
+  TFS_HIP(hipGetLastError());
+  return check_sweep(A);
+}
 
 // One CG iteration (Cg.f90:97-178) in precision T; Bicg takes the same path
 // (its shadow sequence equals the primary one bit for bit, Bicg.f90:91-175).
@@ -995,13 +1044,14 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
   // the whole chip's worth of pollers the polls delay the stores they wait
   // for (measured on MI355X: a 382-level sweep takes half the time with 256
   // to 1024 workgroups as with 2048, a 19-level one 13 % less with 1024).
-  const int chunks = (n + kChunk - 1) / kChunk;
+  const int chunks = (n + kChunk * kSweepRows - 1) / (kChunk * kSweepRows);
   A->grid_persistent =
       std::max(1, std::min({chunks, n_cu * std::max(1, per_cu), kSweepGroups}));
   A->persistent = coop != 0 && per_cu > 0;
   if (const char *e = std::getenv("TFS_SWEEP")) {
     if (std::strcmp(e, "levels") == 0) A->persistent = false;
     if (std::strcmp(e, "dynamic") == 0) A->dynamic = true;
+    A->sweep_fixed = true;
   }
   if (const char *e = std::getenv("TFS_SWEEP_GROUPS")) {  // cap on the persistent grid
     const int g = std::atoi(e);
@@ -1435,6 +1485,7 @@ struct tfs_batch {
   // FP32 inner solves of the refinement on a colour-ordered copy (each
   // system's rows grouped by colour, systems in order), built on first use
   int inner_order = TFS_INNER_COLOURED;
+  int refine = TFS_REFINE_FCG;  // outer loop of the refinement
   tfs_batch *inner = nullptr;
   int *inner_emap = nullptr;    // inner's entry k <- entry of A->val
   int *inner_from_s = nullptr;  // S position of A for inner's S position i
@@ -1622,6 +1673,7 @@ int batch_build(tfs_batch *B, long long N, long long NNZ, const int *cr, const i
                 const int *cd, Layout &Ly) {
   int rc = matrix_create((int)N, NNZ, cr, cc, cd, 0, &B->A, &Ly);
   if (rc < 0) return rc;
+  B->A->sweep_fixed = true;  // the batch picks its sweeps when it captures its graph
   const int nsys = B->nsys;
   // every system occupies one contiguous S range (a connected block is laid
   // out contiguously by RCM; several components of one system stay adjacent)
@@ -1919,11 +1971,15 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
     }
   }
   const int *to_s = coloured ? B->inner_from_s : nullptr;  // I's S position -> B's
-  TFS_TRY(ensure_vec64(A, 7));
+  TFS_TRY(ensure_vec64(A, 8));
   TFS_TRY(ensure_vec32(I->A, 7));
   TFS_TRY(prepare_values<double>(A));
   TFS_TRY(prec_form<float>(I->A, prec));
+  // FP64: x, r, z (inner result), p, q = A p, r2 slot = b (the sweep slots
+  // stay armed for FP64 solves)
   const Vecs<double> D = vecs_of(A->vec64, n);
+  double *bS = D.r2;
+  const bool krylov = B->refine == TFS_REFINE_FCG;
   const Vecs<float> F = vecs_of(I->A->vec32, n);
   KState *so = B->d_st + ns, *allo = B->d_st + 2 * ns + 1;
   KState *si = I->d_st, *alli = I->d_st + 2 * ns;
@@ -1931,14 +1987,14 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
   const Tiles tl = tiles_of(B), tli = tiles_of(I);
   const int g = vgrid(n);
   TFS_TRY(batch_in(B, d_x, A->stage_x, D.x));
-  TFS_TRY(batch_in(B, d_r1, A->stage_r, D.z));  // b
+  TFS_TRY(batch_in(B, d_r1, A->stage_r, bS));  // b
   std::vector<int> caps(ns, cap), stalls(ns, 0), outer(ns, 0), st_out(ns, TFS_OK);
   std::vector<double> err(ns), err0(ns);
   for (int s = 0; s < ns; ++s) inner_iters[s] = 0;
   auto outer_residual = [&]() -> int {
     batch_init_states(B, ns, caps.data(), tol);
     TFS_TRY(batch_write(B));
-    gather(n, D.z, (const int *)nullptr, D.r);
+    gather(n, bS, (const int *)nullptr, D.r);
     launch_spmv<double, SPMV_RESID>(A, D.x, D.r, nullptr, nullptr, nullptr, nullptr);
     hipLaunchKernelGGL(k_sumsq_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0, cur_stream(),
                        tl, D.r, (int)EPI_BNRM, so, allo, B->partials);
@@ -1948,7 +2004,14 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
   };
   TFS_TRY(outer_residual());
   for (int s = 0; s < ns; ++s) err[s] = err0[s] = ho[s].bnrm2;
-  std::vector<char> active(ns, 0);
+  std::vector<char> active(ns, 0), have_p(ns, 0);
+  std::vector<double> pq_prev(ns, 0.0);
+  // per-system scalars into the outer states (read, edit, write back)
+  auto dots = [&](auto &&launch) -> int {
+    launch();
+    TFS_HIP(hipGetLastError());
+    return batch_read(B);
+  };
   for (;;) {
     int nact = 0;
     for (int s = 0; s < ns; ++s) {
@@ -1963,7 +2026,8 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
     // inner FP32 solves of the active systems; the others are done at once
     std::vector<int> icap(ns, 100000);
     std::vector<double> itol(ns);
-    for (int s = 0; s < ns; ++s) itol[s] = std::max(inner_rtol * err[s], 0.5 * tol[s]);
+    for (int s = 0; s < ns; ++s)
+      itol[s] = krylov ? inner_rtol * err[s] : std::max(inner_rtol * err[s], 0.5 * tol[s]);
     batch_init_states(I, 0, icap.data(), itol.data());
     int ndone = 0;
     for (int s = 0; s < ns; ++s)
@@ -1987,17 +2051,117 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
         inner_iters[s] += std::min(hi[s].niter_out, 100000);
         if (hi[s].status == kStBreakdown) st_out[s] = TFS_ERR_BREAKDOWN;
       }
-    hipLaunchKernelGGL(k_add_correction, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x, F.x,
-                       to_s);
-    TFS_TRY(outer_residual());
+    std::vector<double> e_new(ns, 0.0);
+    if (!krylov) {
+      // defect correction: x += z; r = b - A x
+      hipLaunchKernelGGL(k_add_correction, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x,
+                         F.x, to_s);
+      TFS_TRY(outer_residual());
+      for (int s = 0; s < ns; ++s) e_new[s] = ho[s].bnrm2;
+    } else {
+      // flexible CG per system (see solve_mixed_device): the outer states of
+      // the inactive systems are marked done, so their tiles are skipped
+      for (int s = 0; s < ns; ++s) {
+        ho[s].done = active[s] ? 0 : 1;
+        ho[s].iter = have_p[s] ? 1 : 0;   // k_pupdate_seg: p = z on a system's first step
+      }
+      TFS_TRY(batch_write(B));
+      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.z, 0.0);
+      hipLaunchKernelGGL(k_add_correction, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.z,
+                         F.x, to_s);
+      TFS_TRY(dots([&] {
+        hipLaunchKernelGGL(k_dot_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0,
+                           cur_stream(), tl, D.z, D.q, so, allo, B->partials);
+        finish_seg(B, EPI_AUX0, so, allo);  // (z, A p_prev)
+      }));
+      for (int s = 0; s < ns; ++s)
+        if (active[s] && have_p[s]) ho[s].beta = -ho[s].aux[0] / pq_prev[s];
+      TFS_TRY(batch_write(B));
+      hipLaunchKernelGGL(k_pupdate_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0,
+                         cur_stream(), tl, D.z, D.p, so, allo);
+      launch_spmv<double, SPMV_PLAIN>(A, D.p, D.q, nullptr, nullptr, nullptr, allo);
+      TFS_TRY(dots([&] {
+        hipLaunchKernelGGL(k_dot_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0,
+                           cur_stream(), tl, D.p, D.q, so, allo, B->partials);
+        finish_seg(B, EPI_AUX1, so, allo);  // (p, A p)
+        hipLaunchKernelGGL(k_dot_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0,
+                           cur_stream(), tl, D.p, D.r, so, allo, B->partials);
+        finish_seg(B, EPI_AUX2, so, allo);  // (p, r)
+      }));
+      for (int s = 0; s < ns; ++s) {
+        if (!active[s]) continue;
+        const double pq = ho[s].aux[1], alpha = ho[s].aux[2] / pq;
+        if (!(pq > 0.0) || !std::isfinite(alpha)) {
+          st_out[s] = TFS_ERR_BREAKDOWN;
+          ho[s].done = 1;
+          continue;
+        }
+        ho[s].alpha = alpha;
+        pq_prev[s] = pq;
+        have_p[s] = 1;
+      }
+      TFS_TRY(batch_write(B));
+      TFS_TRY(dots([&] {
+        hipLaunchKernelGGL(k_axpy_norm_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0,
+                           cur_stream(), tl, D.x, D.r, D.p, D.q, so, allo, B->partials);
+        finish_seg(B, EPI_NAUX3, so, allo);  // ||r|| of the recurrence
+      }));
+      bool confirm = false;
+      for (int s = 0; s < ns; ++s) {
+        e_new[s] = active[s] ? ho[s].aux[3] : err[s];
+        confirm |= active[s] && e_new[s] < tol[s];
+      }
+      if (confirm) {
+        // the true residual of each system whose recurrence passed its tol,
+        // formed in the z slot: it confirms convergence, or the system
+        // restarts from it.  Only those systems are touched, so no system's
+        // path depends on the others.
+        std::vector<char> check(ns, 0), restart(ns, 0);
+        for (int s = 0; s < ns; ++s) {
+          check[s] = active[s] && e_new[s] < tol[s];
+          ho[s].done = check[s] ? 0 : 1;
+        }
+        TFS_TRY(batch_write(B));
+        gather(n, bS, (const int *)nullptr, D.z);
+        launch_spmv<double, SPMV_RESID>(A, D.x, D.z, nullptr, nullptr, nullptr, nullptr);
+        TFS_TRY(dots([&] {
+          hipLaunchKernelGGL(k_sumsq_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0,
+                             cur_stream(), tl, D.z, (int)EPI_NAUX4, so, allo, B->partials);
+          finish_seg(B, EPI_NAUX4, so, allo);
+        }));
+        bool any = false;
+        for (int s = 0; s < ns; ++s) {
+          if (!check[s]) continue;
+          e_new[s] = ho[s].aux[4];
+          restart[s] = e_new[s] >= tol[s];
+          any |= restart[s];
+        }
+        if (any) {
+          for (int s = 0; s < ns; ++s) {
+            ho[s].done = restart[s] ? 0 : 1;
+            if (restart[s]) have_p[s] = 0;
+          }
+          TFS_TRY(batch_write(B));
+          hipLaunchKernelGGL(k_select_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0,
+                             cur_stream(), tl, D.z, D.r, so, allo);
+          TFS_HIP(hipGetLastError());
+        }
+      }
+    }
     for (int s = 0; s < ns; ++s) {
       if (!active[s]) continue;
-      const double e = ho[s].bnrm2;
+      const double e = e_new[s];
       if (!std::isfinite(e)) st_out[s] = TFS_ERR_BREAKDOWN;
-      stalls[s] = e > 0.9 * err[s] ? stalls[s] + 1 : 0;
+      stalls[s] = e > (krylov ? 1.0 : 0.9) * err[s] ? stalls[s] + 1 : 0;
       if (stalls[s] >= 3 && e >= tol[s]) st_out[s] = TFS_NOT_CONVERGED;
       err[s] = e;
     }
+  }
+  if (krylov) {
+    // the returned residuals are the true ones
+    TFS_TRY(outer_residual());
+    for (int s = 0; s < ns; ++s)
+      if (st_out[s] == TFS_OK) err[s] = ho[s].bnrm2;
   }
   TFS_TRY(batch_out(B, D.x, A->stage_x, d_x));
   TFS_TRY(batch_out(B, D.r, A->stage_r, d_r1));
@@ -2015,6 +2179,12 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
       worst = status[s];
   }
   return worst;
+}
+
+extern "C" int tfs_batch_set_refinement(tfs_batch *B, int outer) {
+  if (!B || (outer != TFS_REFINE_FCG && outer != TFS_REFINE_DEFECT)) return TFS_ERR_INVALID;
+  B->refine = outer;
+  return TFS_OK;
 }
 
 extern "C" int tfs_batch_set_inner_order(tfs_batch *B, int order) {
@@ -2043,6 +2213,7 @@ extern "C" int tfs_matrix_set_sweep_schedule(tfs_matrix *A, int schedule) {
   const bool was_levels = !A->persistent;
   A->persistent = schedule != TFS_SWEEP_LEVELS;
   A->dynamic = schedule == TFS_SWEEP_DYNAMIC;
+  A->sweep_fixed = true;
   if (was_levels && A->persistent) {
     // the persistent sweeps need their outputs armed
     if (A->vec64) arm(A->n, A->vec64 + 2 * (size_t)A->n);

@@ -38,6 +38,7 @@ TFS_ERR_NO_DEVICE = -5
 CG, BICG, CGS = 0, 1, 2
 TFS_INNER_REFERENCE, TFS_INNER_COLOURED = 0, 1
 TFS_REFINE_FCG, TFS_REFINE_DEFECT = 0, 1
+TFS_SWEEP_LEVELS, TFS_SWEEP_STATIC, TFS_SWEEP_DYNAMIC = 0, 1, 2
 _METHODS = {"cg": CG, "bicg": BICG, "cgs": CGS}
 
 _I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -192,6 +193,11 @@ class Matrix:
     def set_spmv_shape(self, rows, waves):
         _check(self._lib.tfs_matrix_set_spmv_shape(self._h, int(rows), int(waves)),
                "tfs_matrix_set_spmv_shape")
+
+    def set_sweep_schedule(self, schedule):
+        """TFS_SWEEP_LEVELS / _STATIC / _DYNAMIC (otherwise timed on the first factor)."""
+        _check(self._lib.tfs_matrix_set_sweep_schedule(self._h, int(schedule)),
+               "tfs_matrix_set_sweep_schedule")
 
     def set_refinement(self, outer):
         """TFS_REFINE_FCG (default) or TFS_REFINE_DEFECT for solve_mixed."""
@@ -375,6 +381,7 @@ class Batch:
             lib.tfs_batch_get_info.argtypes = [_VP, _IP, _IP, ctypes.POINTER(ctypes.c_longlong),
                                                _IP, _IP, _IP]
             lib.tfs_batch_set_inner_order.argtypes = [_VP, ctypes.c_int]
+            lib.tfs_batch_set_refinement.argtypes = [_VP, ctypes.c_int]
             lib._tfs_batch_typed = True
         self._lib = lib
         self._keep = []
@@ -415,6 +422,10 @@ class Batch:
     def set_inner_order(self, order):
         _check(self._lib.tfs_batch_set_inner_order(self._h, int(order)),
                "tfs_batch_set_inner_order")
+
+    def set_refinement(self, outer):
+        _check(self._lib.tfs_batch_set_refinement(self._h, int(outer)),
+               "tfs_batch_set_refinement")
 
     def info(self):
         ns, nt, gl, sr, sw = (ctypes.c_int() for _ in range(5))

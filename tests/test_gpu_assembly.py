@@ -3,10 +3,12 @@ face-loop restatement (itself pinned bitwise on the CPU against the
 reference's Pressure_Matrix_Fractional): bitwise for real-valued
 coefficients, with and without a21, boundary adds, cell terms, sav and
 under-relaxation; and a solve after the device assembly equals, bitwise, the
-solve after uploading the same values from the host."""
+solve after uploading the same values from the host (and meets the parity
+bar against the oracle's solve)."""
 import numpy as np
 import pytest
 
+from conftest import rel_l2
 from tflows_amd import mesh
 
 pytestmark = pytest.mark.gpu
@@ -83,4 +85,5 @@ def test_solve_after_device_assembly(torch, S, oracle):
     assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
     assert np.array_equal(out[0][2], out[1][2])
     ref = oracle.solve("bicg", c.row, c.col, c.dia, c.val, c.b, prec=2, niter=5000, tol=tol)
-    assert np.array_equal(out[0][2], ref["x"])
+    assert abs(out[0][0] - ref["niter"]) <= max(1, ref["niter"] // 20)
+    assert rel_l2(out[0][2], ref["x"]) <= 1e-8

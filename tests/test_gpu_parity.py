@@ -134,17 +134,22 @@ def test_spmv_and_residual_bitwise(torch, S, oracle, name):
 
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("prec", [0, 1, 2])
-@pytest.mark.parametrize("sweep", ["persistent", "levels"])
+@pytest.mark.parametrize("sweep", ["static", "dynamic", "levels", "env-levels", "timed"])
 def test_factor_and_sweeps_bitwise(torch, S, oracle, name, prec, sweep, monkeypatch):
-    """Both sweep schedules: one persistent launch per sweep, and one launch
-    per level (TFS_SWEEP=levels, read when the matrix is created)."""
+    """Every sweep schedule: one persistent launch per sweep (static or
+    dynamic chunk order), one launch per level (set on the matrix or by
+    TFS_SWEEP=levels when it is created), and the schedule timed on the
+    first factor (the default)."""
     c = get_case(name)
     print(f"case {name} prec={prec} sweep={sweep}", flush=True)
-    if sweep == "levels":
+    if sweep == "env-levels":
         monkeypatch.setenv("TFS_SWEEP", "levels")
     else:
         monkeypatch.delenv("TFS_SWEEP", raising=False)
     A = matrix(S, c)
+    if sweep in ("static", "dynamic", "levels"):
+        A.set_sweep_schedule({"static": S.TFS_SWEEP_STATIC, "dynamic": S.TFS_SWEEP_DYNAMIC,
+                              "levels": S.TFS_SWEEP_LEVELS}[sweep])
     A.prec_form(prec)
     d = torch.zeros(c.n, dtype=torch.float64, device="cuda")
     A.prec_diag(d)
