@@ -446,6 +446,16 @@ k_finish_seg(const int *__restrict__ tile_first, const double *__restrict__ part
 // partial ||rs||^2 (Cgs.f90:182-221, all row-local once alpha is known).
 enum SpmvMode { SPMV_PLAIN = 0, SPMV_RESID, SPMV_DOT, SPMV_CGS_TAIL };
 
+// Workgroup id renumbered XCD-major: hardware deals workgroup i to XCD i % 8;
+// the result r is a bijection of [0, n) under which XCD x holds one
+// contiguous range of r (the first n % 8 XCDs one more than the others).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_major(int i, int n) {
+  const int x = i % kXcds, k = i / kXcds;
+  const int per = n / kXcds, tall = n % kXcds;  // XCDs 0..tall-1 hold per + 1
+  return x < tall ? x * (per + 1) + k : tall * (per + 1) + (x - tall) * per + k;
+}
+
 // WAVES: wavefronts per workgroup (2, 4 or 8; the row blocks, at most 64 rows
 // each, are a runtime layout) -- the two shapes the SpMV tuner picks.
 template <typename T, int MODE, int WAVES>
@@ -460,7 +470,15 @@ k_spmv(int nblk, const int *__restrict__ rblk, const int *__restrict__ row,
   T *lds = prod[wv];
   const T alpha = MODE == SPMV_CGS_TAIL ? (T)st->alpha : T(0);
   double acc_red = 0.0;
-  for (int b = blockIdx.x * WAVES + wv; b < nblk; b += gridDim.x * WAVES) {
+  // XCD-aware work split: workgroups are dealt round-robin to the 8 XCDs,
+  // each with its own L2; renumber them so that every XCD owns one contiguous
+  // run of workgroups, and give every workgroup one contiguous run of row
+  // blocks -- neighbouring blocks, which gather overlapping ranges of x,
+  // then share an L2
+  const int wg = xcd_major(blockIdx.x, gridDim.x);
+  const int per = (nblk + gridDim.x - 1) / gridDim.x;
+  const int b_end = min(nblk, (wg + 1) * per);
+  for (int b = wg * per + wv; b < b_end; b += WAVES) {
     // b is the same in every lane: the block's bounds come through the
     // scalar unit
     const int bu = __builtin_amdgcn_readfirstlane(b);

@@ -1051,7 +1051,7 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
   if (const char *e = std::getenv("TFS_SWEEP")) {
     if (std::strcmp(e, "levels") == 0) A->persistent = false;
     if (std::strcmp(e, "dynamic") == 0) A->dynamic = true;
-    A->sweep_fixed = true;
+    A->sweep_fixed = true;  // "static" (or anything else): the persistent static schedule
   }
   if (const char *e = std::getenv("TFS_SWEEP_GROUPS")) {  // cap on the persistent grid
     const int g = std::atoi(e);
