@@ -308,6 +308,8 @@ def mixed_vs_fp64(args):
     tol = solve["rtol"] * float(np.linalg.norm(c.b))
     A = solvers.Matrix(c.row, c.col, c.dia, c.val)
     A.set_refinement(solvers.TFS_REFINE_FCG if args.refine == "fcg" else solvers.TFS_REFINE_DEFECT)
+    if args.inner_block:
+        A.set_inner_block(args.inner_block)
     info = A.info()
     b = torch.from_numpy(c.b).cuda()
     x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
@@ -390,6 +392,9 @@ def batch(args):
     ns = [c.n for c in sysl]
     log(f"  cells {min(ns)}..{max(ns)}, total {sum(ns)}")
     B = solvers.Batch([(c.row, c.col, c.dia, c.val) for c in sysl])
+    B.set_refinement(solvers.TFS_REFINE_FCG if args.refine == "fcg" else solvers.TFS_REFINE_DEFECT)
+    if args.inner_block:
+        B.set_inner_block(args.inner_block)
     tols = [1e-10 * float(np.linalg.norm(c.b)) for c in sysl]
     bs = [torch.from_numpy(c.b).cuda() for c in sysl]
     xs = [torch.zeros(c.n, dtype=torch.float64, device="cuda") for c in sysl]
@@ -572,7 +577,9 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="workload 3: mesh scale per axis")
     ap.add_argument("--inner-rtol", type=float, default=1e-2)
     ap.add_argument("--refine", choices=["fcg", "defect"], default="fcg",
-                    help="workload 3: outer loop of the refinement")
+                    help="workloads 3-4: outer loop of the refinement")
+    ap.add_argument("--inner-block", type=int, default=0,
+                    help="workloads 3-4: rows per colour block of the inner order (0: default)")
     ap.add_argument("--cases", type=int, default=24)
     ap.add_argument("--visits", type=int, default=10)
     ap.add_argument("--rate", type=float, default=200.0)

@@ -93,6 +93,9 @@ double tfs_case_max_aspect(const tfs_case *c);
  * the number of colours.  The FP32 inner solves of the refinement run in
  * this order (S-order rows as input there). */
 int tfs_layout_colour(int n, const int *row, const int *col, int *perm, int *colour);
+/* The same with runs of `block` consecutive rows coloured as units. */
+int tfs_layout_block_colour(int n, const int *row, const int *col, int block, int *perm,
+                            int *colour);
 /* The layouts the solver builds for a matrix (0-based CRS): s_orig = caller
  * row at each S (reverse Cuthill-McKee) position; level of each caller row in
  * the lower and upper dependency DAGs of the caller's numbering; bandwidth of

@@ -155,6 +155,11 @@ int tfs_matrix_set_spmv_shape(tfs_matrix *A, int rows_per_block, int waves_per_g
  *                        the refinement still converges to the FP64 tol. */
 enum { TFS_INNER_REFERENCE = 0, TFS_INNER_COLOURED = 1 };
 int tfs_matrix_set_inner_order(tfs_matrix *A, int order);
+/* Block multi-colouring of the inner order: runs of `rows` consecutive rows
+ * (of the RCM order) are coloured as units, keeping their couplings inside
+ * the factor (a stronger preconditioner than 1-row colouring, with at most
+ * `rows` sweep levels per colour).  1 = plain colouring. */
+int tfs_matrix_set_inner_block(tfs_matrix *A, int rows);
 /* Outer loop of the refinement:
  *   TFS_REFINE_FCG     (default) flexible CG in FP64 with the FP32 inner solve
  *                      as its (variable) preconditioner: x and r are updated
@@ -192,6 +197,7 @@ int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *const *d_x,
 /* Row order of the batch's FP32 inner solves, as tfs_matrix_set_inner_order
  * (default TFS_INNER_COLOURED: every system's rows grouped by colour). */
 int tfs_batch_set_inner_order(tfs_batch *B, int order);
+int tfs_batch_set_inner_block(tfs_batch *B, int rows);
 /* Outer loop of the batch's refinement, as tfs_matrix_set_refinement
  * (default TFS_REFINE_FCG: flexible CG with per-system scalars). */
 int tfs_batch_set_refinement(tfs_batch *B, int outer);

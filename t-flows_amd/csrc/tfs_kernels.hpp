@@ -41,6 +41,7 @@ constexpr int kSweepRows = 1;            // rows per thread (a chunk is kChunk *
 constexpr int kPrefetch = 8;             // row entries held in registers
 constexpr unsigned kSpinLimit = 1u << 20;
 constexpr int kSweepGroups = 1024;       // cap on a persistent sweep's workgroups
+constexpr int kInnerBlock = 1;           // rows per colour block of the FP32 inner order
 
 // EPI_AUX0..5 store the plain sum into aux[k] (dot products), EPI_NAUX0..5
 // its square root (norms).

@@ -168,7 +168,7 @@ void build_sweep(int n, const int *row, const int *col, const int *dia, bool low
 }  // namespace
 
 int colour_order(int n, const int *row, const int *col, std::vector<int> &perm,
-                 std::vector<int> *colour_out) {
+                 std::vector<int> *colour_out, int block) {
   // symmetrised adjacency: the pattern may be unsymmetric
   std::vector<int> deg(n + 1, 0);
   for (int i = 0; i < n; ++i)
@@ -185,21 +185,29 @@ int colour_order(int n, const int *row, const int *col, std::vector<int> &perm,
         adj[fill[i]++] = col[k];
         adj[fill[col[k]]++] = i;
       }
-  std::vector<int> colour(n, -1), seen;  // seen[c] == i: colour c taken by a neighbour of i
+  // blocks of `block` consecutive rows are coloured as units: a block takes
+  // the smallest colour that no neighbouring block has
+  block = std::max(1, block);
+  const int nb = (n + block - 1) / block;
+  std::vector<int> bcol(nb, -1), seen;  // seen[c] == B: colour c taken by a neighbour of B
   int ncol = 0;
-  for (int i = 0; i < n; ++i) {
-    for (int k = deg[i]; k < deg[i + 1]; ++k) {
-      const int c = colour[adj[k]];
-      if (c >= 0) seen[c] = i;
-    }
+  for (int B = 0; B < nb; ++B) {
+    const int i0 = B * block, i1 = std::min(n, i0 + block);
+    for (int i = i0; i < i1; ++i)
+      for (int k = deg[i]; k < deg[i + 1]; ++k) {
+        const int Bj = adj[k] / block;
+        if (Bj != B && bcol[Bj] >= 0) seen[bcol[Bj]] = B;
+      }
     int c = 0;
-    while (c < ncol && seen[c] == i) ++c;
+    while (c < ncol && seen[c] == B) ++c;
     if (c == ncol) {
       ++ncol;
       seen.push_back(-1);
     }
-    colour[i] = c;
+    bcol[B] = c;
   }
+  std::vector<int> colour(n);
+  for (int i = 0; i < n; ++i) colour[i] = bcol[i / block];
   std::vector<int> start(ncol + 1, 0);
   for (int i = 0; i < n; ++i) start[colour[i] + 1]++;
   for (int c = 0; c < ncol; ++c) start[c + 1] += start[c];

@@ -56,9 +56,11 @@ struct Layout {
 // `perm` lists the vertices by colour, index order within a colour (new
 // position -> vertex); returns the number of colours.  Rows of one colour
 // are independent, so an incomplete factor in this order has one level per
-// colour.
+// colour.  With block > 1, runs of `block` consecutive vertices are coloured
+// as units (block multi-colouring): a block keeps its internal couplings, in
+// order, and an incomplete factor has at most `block` levels per colour.
 int colour_order(int n, const int *row, const int *col, std::vector<int> &perm,
-                 std::vector<int> *colour = nullptr);
+                 std::vector<int> *colour = nullptr, int block = 1);
 
 // row/col/dia are 0-based, validated (diagonal present, columns ascending).
 void build_layout(int n, const int *row, const int *col, const int *dia, int spmv_rows,

@@ -6,6 +6,18 @@
 #include "tfs_layout.hpp"
 #include "tfs_mesh.h"
 
+extern "C" int tfs_layout_block_colour(int n, const int *row, const int *col, int block,
+                                       int *perm, int *colour) {
+  if (n < 0 || block < 1 || (n > 0 && (!row || !col || !perm || !colour))) return -1;
+  std::vector<int> p, c;
+  const int ncol = tfs::colour_order(n, row, col, p, &c, block);
+  for (int i = 0; i < n; ++i) {
+    perm[i] = p[i];
+    colour[i] = c[i];
+  }
+  return ncol;
+}
+
 extern "C" int tfs_layout_colour(int n, const int *row, const int *col, int *perm, int *colour) {
   if (n < 0 || (n > 0 && (!row || !col || !perm || !colour))) return -1;
   std::vector<int> p, c;

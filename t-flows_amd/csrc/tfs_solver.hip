@@ -168,6 +168,7 @@ struct tfs_matrix {
   // FP32 inner solves of the refinement on a colour-ordered copy P A P^T,
   // P a greedy colouring of the S-order graph (built on first use)
   int inner_order = TFS_INNER_COLOURED;
+  int inner_block = kInnerBlock;  // rows coloured as one unit in the inner order
   int refine = TFS_REFINE_FCG;   // outer loop of the refinement
   tfs_matrix *inner = nullptr;
   int *inner_emap = nullptr;     // inner's caller-order entry k <- entry of val
@@ -657,7 +658,7 @@ int build_inner(tfs_matrix *A) {
                          cur_stream()));
   TFS_HIP(hipStreamSynchronize(cur_stream()));
   std::vector<int> perm;
-  const int ncol = colour_order(n, row.data(), col.data(), perm);
+  const int ncol = colour_order(n, row.data(), col.data(), perm, nullptr, A->inner_block);
   std::vector<int> pos(n);
   for (int i = 0; i < n; ++i) pos[perm[i]] = i;
   std::vector<int> prow(n + 1, 0), pcol((size_t)nnz), pdia(n, -1), emap((size_t)nnz);
@@ -1232,6 +1233,23 @@ extern "C" int tfs_matrix_set_spmv_shape(tfs_matrix *A, int rows_per_block, int 
   return set_spmv_shape(A, s_row, rows_per_block, waves_per_group);
 }
 
+extern "C" int tfs_matrix_set_inner_block(tfs_matrix *A, int rows) {
+  if (!A || rows < 1 || rows > 4096) return TFS_ERR_INVALID;
+  if (rows == A->inner_block) return TFS_OK;
+  StreamScope scope(A->stream);
+  A->inner_block = rows;
+  if (A->inner) {  // rebuilt on the next refinement solve
+    A->bytes -= A->inner->bytes;
+    tfs_matrix_destroy(A->inner);
+    A->inner = nullptr;
+    TFS_TRY(dev_free(A, A->inner_emap, sizeof(int) * (size_t)A->nnz));
+    TFS_TRY(dev_free(A, A->inner_from_s, sizeof(int) * (size_t)A->n));
+    A->inner_emap = A->inner_from_s = nullptr;
+    A->inner_colours = 0;
+  }
+  return TFS_OK;
+}
+
 extern "C" int tfs_matrix_set_refinement(tfs_matrix *A, int outer) {
   if (!A || (outer != TFS_REFINE_FCG && outer != TFS_REFINE_DEFECT)) return TFS_ERR_INVALID;
   A->refine = outer;
@@ -1485,6 +1503,7 @@ struct tfs_batch {
   // FP32 inner solves of the refinement on a colour-ordered copy (each
   // system's rows grouped by colour, systems in order), built on first use
   int inner_order = TFS_INNER_COLOURED;
+  int inner_block = kInnerBlock;
   int refine = TFS_REFINE_FCG;  // outer loop of the refinement
   tfs_batch *inner = nullptr;
   int *inner_emap = nullptr;    // inner's entry k <- entry of A->val
@@ -1737,7 +1756,7 @@ int build_batch_inner(tfs_batch *B) {
                          cur_stream()));
   TFS_HIP(hipStreamSynchronize(cur_stream()));
   std::vector<int> perm, colour;
-  colour_order(n, row.data(), col.data(), perm, &colour);
+  colour_order(n, row.data(), col.data(), perm, &colour, B->inner_block);
   // system of each S row (from its caller row), then (system, colour, S) order
   std::vector<int> sys_of(n);
   for (int k = 0; k < n; ++k) {
@@ -2179,6 +2198,21 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
       worst = status[s];
   }
   return worst;
+}
+
+extern "C" int tfs_batch_set_inner_block(tfs_batch *B, int rows) {
+  if (!B || rows < 1 || rows > 4096) return TFS_ERR_INVALID;
+  if (rows == B->inner_block) return TFS_OK;
+  B->inner_block = rows;
+  if (B->inner) {  // rebuilt on the next refinement solve
+    StreamScope scope(B->A->stream);
+    tfs_batch_destroy(B->inner);
+    B->inner = nullptr;
+    TFS_TRY(dev_free(B->A, B->inner_emap, sizeof(int) * (size_t)B->A->nnz));
+    TFS_TRY(dev_free(B->A, B->inner_from_s, sizeof(int) * (size_t)B->A->n));
+    B->inner_emap = B->inner_from_s = nullptr;
+  }
+  return TFS_OK;
 }
 
 extern "C" int tfs_batch_set_refinement(tfs_batch *B, int outer) {

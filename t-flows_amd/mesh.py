@@ -73,6 +73,9 @@ def _lib():
         lib.tfs_case_max_aspect.argtypes = [ctypes.c_void_p]
         lib.tfs_layout_colour.restype = ctypes.c_int
         lib.tfs_layout_colour.argtypes = [ctypes.c_int, _I32P, _I32P, _I32P, _I32P]
+        lib.tfs_layout_block_colour.restype = ctypes.c_int
+        lib.tfs_layout_block_colour.argtypes = [ctypes.c_int, _I32P, _I32P, ctypes.c_int, _I32P,
+                                                _I32P]
         lib.tfs_layout_summary.restype = ctypes.c_int
         lib.tfs_layout_summary.argtypes = [ctypes.c_int, _I32P, _I32P, _I32P, _I32P, _I32P,
                                            _I32P, ctypes.POINTER(ctypes.c_int)]
@@ -171,13 +174,17 @@ def topology(n_cells, faces_c, base=0, with_pos=False):
     return (row, col, dia, pos) if with_pos else (row, col, dia)
 
 
-def layout_colour(row, col):
-    """Greedy colouring the FP32 inner solves use: (ncolours, perm, colour)."""
+def layout_colour(row, col, block=1):
+    """Greedy (block) colouring the FP32 inner solves use: (ncolours, perm, colour)."""
     row = np.ascontiguousarray(row, np.int32)
     n = len(row) - 1
     perm = np.empty(n, np.int32)
     colour = np.empty(n, np.int32)
-    k = _lib().tfs_layout_colour(n, row, np.ascontiguousarray(col, np.int32), perm, colour)
+    col = np.ascontiguousarray(col, np.int32)
+    if block == 1:
+        k = _lib().tfs_layout_colour(n, row, col, perm, colour)
+    else:
+        k = _lib().tfs_layout_block_colour(n, row, col, int(block), perm, colour)
     if k < 0:
         raise ValueError("tfs_layout_colour failed")
     return k, perm, colour

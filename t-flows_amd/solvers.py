@@ -99,6 +99,7 @@ def _lib():
     lib.tfs_matrix_set_inner_order.argtypes = [_VP, ctypes.c_int]
     lib.tfs_matrix_get_values.argtypes = [_VP, _F64P]
     lib.tfs_matrix_set_refinement.argtypes = [_VP, ctypes.c_int]
+    lib.tfs_matrix_set_inner_block.argtypes = [_VP, ctypes.c_int]
     lib.tfs_matrix_tune_spmv.argtypes = [_VP, _IP, _IP]
     lib.tfs_matrix_set_spmv_shape.argtypes = [_VP, ctypes.c_int, ctypes.c_int]
     lib.tfs_matrix_set_sweep_schedule.argtypes = [_VP, ctypes.c_int]
@@ -203,6 +204,11 @@ class Matrix:
         """TFS_REFINE_FCG (default) or TFS_REFINE_DEFECT for solve_mixed."""
         _check(self._lib.tfs_matrix_set_refinement(self._h, int(outer)),
                "tfs_matrix_set_refinement")
+
+    def set_inner_block(self, rows):
+        """Rows per colour block of the inner order (1 = plain colouring)."""
+        _check(self._lib.tfs_matrix_set_inner_block(self._h, int(rows)),
+               "tfs_matrix_set_inner_block")
 
     def set_inner_order(self, order):
         """TFS_INNER_REFERENCE or TFS_INNER_COLOURED for solve_mixed."""
@@ -382,6 +388,7 @@ class Batch:
                                                _IP, _IP, _IP]
             lib.tfs_batch_set_inner_order.argtypes = [_VP, ctypes.c_int]
             lib.tfs_batch_set_refinement.argtypes = [_VP, ctypes.c_int]
+            lib.tfs_batch_set_inner_block.argtypes = [_VP, ctypes.c_int]
             lib._tfs_batch_typed = True
         self._lib = lib
         self._keep = []
@@ -426,6 +433,10 @@ class Batch:
     def set_refinement(self, outer):
         _check(self._lib.tfs_batch_set_refinement(self._h, int(outer)),
                "tfs_batch_set_refinement")
+
+    def set_inner_block(self, rows):
+        _check(self._lib.tfs_batch_set_inner_block(self._h, int(rows)),
+               "tfs_batch_set_inner_block")
 
     def info(self):
         ns, nt, gl, sr, sw = (ctypes.c_int() for _ in range(5))

@@ -143,3 +143,26 @@ def test_coloured_inner_order(torch, S):
     steps, inner, ini, fin = S.solve_mixed(A, x, r, 2, 50, tol / 2, x_rtol=1e-10)
     assert A.status == S.TFS_OK
     assert rel_l2(2.0 * x.cpu().numpy(), xs) <= bar
+
+
+@pytest.mark.parametrize("block", [4, 16])
+def test_block_coloured_inner_order(torch, S, oracle, block):
+    """Block multi-coloured inner order: the refined answer meets the same bar
+    against the exact solution, and changing the block size rebuilds the copy."""
+    c = mesh.make_case(**CASES["poisson_stretched20"])
+    tol = 1e-10 * float(np.linalg.norm(c.b))
+    ref = oracle.solve("cg", c.row, c.col, c.dia, c.val, c.b, prec=2, niter=20000, tol=tol)
+    xs = exact_solution(c.row, c.col, c.val, c.b)
+    A = S.Matrix(c.row, c.col, c.dia, c.val)
+    inners = []
+    for blk in (1, block):
+        A.set_inner_block(blk)
+        x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
+        r = torch.from_numpy(c.b.copy()).cuda()
+        steps, inner, ini, fin = S.solve_mixed(A, x, r, 2, 50, tol, x_rtol=1e-10)
+        assert A.status == S.TFS_OK and fin < tol
+        assert rel_l2(x.cpu().numpy(), xs) <= refinement_bar(ref["x"], xs)
+        inners.append(inner)
+        print("block", blk, "colours", A.info()["inner_colours"], "steps", steps, "inner", inner,
+              flush=True)
+    assert A.info()["inner_colours"] >= 2

@@ -68,3 +68,30 @@ def test_box_depth_and_rcm_bandwidth():
     natural_perm_bw = max(abs(int(cp.col[k]) - i) for i in range(cp.n)
                           for k in range(cp.row[i], cp.row[i + 1]))
     assert Lp["bandwidth"] < natural_perm_bw
+
+
+@pytest.mark.parametrize("block", [2, 8, 32])
+@pytest.mark.parametrize("name", list(CASES))
+def test_block_colouring(name, block):
+    """Blocks of `block` consecutive rows are coloured as units: adjacent
+    blocks differ, every block takes the smallest colour its earlier
+    neighbours leave free, and the order lists colours, then blocks, then
+    rows, each ascending."""
+    c = mesh.make_case(**CASES[name])
+    k, perm, colour = mesh.layout_colour(c.row, c.col, block)
+    assert sorted(perm.tolist()) == list(range(c.n))
+    blk = np.arange(c.n) // block
+    nb = blk[-1] + 1
+    bcol = np.array([colour[b * block] for b in range(nb)])
+    assert np.array_equal(colour, bcol[blk])
+    for b in range(nb):
+        rows = np.arange(b * block, min(c.n, (b + 1) * block))
+        nbrs = set()
+        for i in rows:
+            nbrs |= set((neighbours(c, i) // block).tolist())
+        nbrs.discard(b)
+        assert all(bcol[o] != bcol[b] for o in nbrs)
+        earlier = {bcol[o] for o in nbrs if o < b}
+        assert bcol[b] == min(set(range(k + 1)) - earlier)
+    key = colour[perm] * c.n + perm
+    assert np.all(np.diff(key) > 0)
