@@ -135,7 +135,7 @@ int tfs_solve_mixed_device(tfs_matrix *A, int prec, double *d_x, double *d_r1, i
 /* ---- SpMV shape ------------------------------------------------------------------
  * Not in the reference.  The SpMV works on row blocks of at most 64 rows and
  * 512 entries, one wavefront each, 2, 4 or 8 wavefronts per workgroup
- * (default 64 rows, 4 wavefronts).  Row sums are the same for every shape
+ * (default 64 rows, 8 wavefronts).  Row sums are the same for every shape
  * (each row is summed left to right); the dot products fused into the SpMV
  * of the Krylov iterations are summed per workgroup, so their last-bit
  * rounding follows the shape.  tfs_matrix_tune_spmv times 16/32/64 rows x

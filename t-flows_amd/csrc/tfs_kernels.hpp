@@ -27,7 +27,7 @@
 
 namespace tfs {
 
-constexpr int kSpmvWaves = 4;            // waves per SpMV workgroup
+constexpr int kSpmvWaves = 8;            // waves per SpMV workgroup (default shape)
 constexpr int kSpmvThreads = kSpmvWaves * kWave;
 constexpr int kSpmvRows = 64;            // rows per SpMV row block (one per lane)
 constexpr int kSpmvCap = 512;            // entries per row block staged in LDS
