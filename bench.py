@@ -153,7 +153,12 @@ def measured_traffic(name):
 
 
 def spmv_roofline(A, x, y, n, nnz, reps=50, inloop_ms=None, traffic_profile=None):
-    ms = A.spmv_time(x, y, reps=reps)
+    """The SpMV against HBM: each launch timed alone after a 640 MiB copy has
+    flushed the 256 MB memory-side cache (the 128^3 matrix, 217 MB, would
+    otherwise partly stay there between back-to-back launches); the warm
+    back-to-back figure is kept beside it."""
+    ms = A.spmv_time_cold(x, y, reps=10)
+    warm = A.spmv_time(x, y, reps=reps)
     algo = spmv_bytes(n, nnz)
     achieved = algo / (ms * 1e-3) / 1e9
     return {
@@ -161,6 +166,7 @@ def spmv_roofline(A, x, y, n, nnz, reps=50, inloop_ms=None, traffic_profile=None
         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
         "traffic": measured_traffic(traffic_profile) if traffic_profile else None,
         "algorithmic_bytes": algo, "launch_us": ms * 1e3,
+        "launch_us_warm": warm * 1e3, "achieved_warm": algo / (warm * 1e-3) / 1e9,
         "launch_us_in_solves": None if inloop_ms is None else inloop_ms * 1e3,
     }
 

@@ -284,6 +284,9 @@ int tfs_prec_diag32_device(tfs_matrix *A, int prec, double *d_out);
  * Runs `reps` back-to-back SpMV launches y := A x on the library's stream and
  * returns the mean launch duration in ms from HIP events on that stream. */
 int tfs_spmv_time(tfs_matrix *A, const double *d_x, double *d_y, int reps, double *ms);
+/* The same with the 256 MB memory-side cache flushed before every launch
+ * (640 MiB copied), each launch timed alone: the HBM-bound figure. */
+int tfs_spmv_time_cold(tfs_matrix *A, const double *d_x, double *d_y, int reps, double *ms);
 /* Average time of one FP64 incomplete-Cholesky sweep (forward and backward
  * alternating, `reps` pairs back to back, factor formed first). */
 int tfs_sweep_time(tfs_matrix *A, int reps, double *ms);

@@ -472,14 +472,17 @@ k_spmv(int nblk, const int *__restrict__ rblk, const int *__restrict__ row,
   const T alpha = MODE == SPMV_CGS_TAIL ? (T)st->alpha : T(0);
   double acc_red = 0.0;
   // XCD-aware work split: workgroups are dealt round-robin to the 8 XCDs,
-  // each with its own L2; renumber them so that every XCD owns one contiguous
-  // run of workgroups, and give every workgroup one contiguous run of row
-  // blocks -- neighbouring blocks, which gather overlapping ranges of x,
-  // then share an L2
-  const int wg = xcd_major(blockIdx.x, gridDim.x);
-  const int per = (nblk + gridDim.x - 1) / gridDim.x;
-  const int b_end = min(nblk, (wg + 1) * per);
-  for (int b = wg * per + wv; b < b_end; b += WAVES) {
+  // each with its own L2.  Every XCD takes one contiguous range of row blocks
+  // (in proportion to its workgroups), and its workgroups walk that range
+  // side by side (workgroup j of the XCD takes blocks j, j + m, ... of it):
+  // the blocks in flight on one XCD are neighbours, whose x gathers overlap,
+  // so the x window they need stays in that XCD's L2
+  const int xcd = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+  const int m = ((int)gridDim.x - xcd + kXcds - 1) / kXcds;  // workgroups on this XCD
+  const int first = xcd_major(blockIdx.x, gridDim.x) - j;    // the XCD's first renumbered id
+  const int b_beg = (int)((long long)nblk * first / gridDim.x);
+  const int b_end = (int)((long long)nblk * (first + m) / gridDim.x);
+  for (int b = b_beg + j * WAVES + wv; b < b_end; b += m * WAVES) {
     // b is the same in every lane: the block's bounds come through the
     // scalar unit
     const int bu = __builtin_amdgcn_readfirstlane(b);

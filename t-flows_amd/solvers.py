@@ -92,6 +92,7 @@ def _lib():
     lib.tfs_prec_diag_device.argtypes = [_VP, _VP]
     lib.tfs_spmv_time.argtypes = [_VP, _VP, _VP, ctypes.c_int, _DP]
     lib.tfs_sweep_time.argtypes = [_VP, ctypes.c_int, _DP]
+    lib.tfs_spmv_time_cold.argtypes = [_VP, _VP, _VP, ctypes.c_int, _DP]
     lib.tfs_solve_mixed_device.argtypes = [_VP, ctypes.c_int, _VP, _VP, _IP, _IP,
                                            ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                            _DP, _DP]
@@ -277,6 +278,16 @@ class Matrix:
         """ms per FP64 incomplete-Cholesky sweep launch, back to back."""
         ms = ctypes.c_double()
         _check(self._lib.tfs_sweep_time(self._h, int(reps), ctypes.byref(ms)), "tfs_sweep_time")
+        return ms.value
+
+    def spmv_time_cold(self, x, y, reps=10):
+        """ms per FP64 SpMV launch with the memory-side cache flushed before each."""
+        _dptr(x, self.n, "x")
+        _dptr(y, self.n, "y")
+        ms = ctypes.c_double()
+        _sync()
+        _check(self._lib.tfs_spmv_time_cold(self._h, x.data_ptr(), y.data_ptr(), int(reps),
+                                            ctypes.byref(ms)), "tfs_spmv_time_cold")
         return ms.value
 
     def spmv_time(self, x, y, reps=20):
