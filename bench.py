@@ -37,6 +37,8 @@ import time
 
 ROOT = pathlib.Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0
+WORKLOAD_TAG = {1: "poisson32", 2: "momentum128", 3: "poisson256_aspect100", 4: "batch64",
+                5: "serving"}
 METRIC = "Poisson solves/s & SpMV HBM GB/s vs plain FP64 at 1e-10 resid; p50/p99 latency"
 
 
@@ -153,21 +155,24 @@ def measured_traffic(name):
 
 
 def spmv_roofline(A, x, y, n, nnz, reps=50, inloop_ms=None, traffic_profile=None):
-    """The SpMV against HBM: each launch timed alone after a 640 MiB copy has
-    flushed the 256 MB memory-side cache (the 128^3 matrix, 217 MB, would
-    otherwise partly stay there between back-to-back launches); the warm
-    back-to-back figure is kept beside it."""
-    ms = A.spmv_time_cold(x, y, reps=10)
+    """The SpMV against HBM.  With inloop_ms (the average launch timed with
+    HIP events inside the timed solves) that is the figure; otherwise each
+    launch is timed alone after a 640 MiB read has flushed the 256 MB
+    memory-side cache.  The flushed and the warm back-to-back figures are
+    kept beside it."""
+    cold = A.spmv_time_cold(x, y, reps=10)
     warm = A.spmv_time(x, y, reps=reps)
     algo = spmv_bytes(n, nnz)
+    ms = inloop_ms if inloop_ms is not None else cold
     achieved = algo / (ms * 1e-3) / 1e9
     return {
         "kernel": "k_spmv (FP64 CSR SpMV)", "bound": "hbm", "achieved": achieved,
         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
         "traffic": measured_traffic(traffic_profile) if traffic_profile else None,
         "algorithmic_bytes": algo, "launch_us": ms * 1e3,
+        "timing": "in the timed solves" if inloop_ms is not None else "cache flushed, alone",
+        "launch_us_flushed": cold * 1e3, "achieved_flushed": algo / (cold * 1e-3) / 1e9,
         "launch_us_warm": warm * 1e3, "achieved_warm": algo / (warm * 1e-3) / 1e9,
-        "launch_us_in_solves": None if inloop_ms is None else inloop_ms * 1e3,
     }
 
 
@@ -179,26 +184,25 @@ def sweep_bytes(n, ntri):
     return 12 * ntri + 4 * (n + 1) + (4 + 8 + 8 + 8 + 8) * n
 
 
-def sweep_roofline(A, c, bd, traffic_profile=None):
-    """The preconditioner sweeps: average launch of forward and backward
-    sweeps run back to back (HIP events on the matrix's stream), against the
-    average of the two sweeps' algorithmic bytes; the in-solve per-launch
-    figure of the breakdown is kept beside it."""
+def sweep_roofline(A, c, inloop_us, traffic_profile=None):
+    """The preconditioner sweeps: the average launch of the forward and
+    backward sweeps timed with HIP events inside the timed solves, against
+    the average of the two sweeps' algorithmic bytes; the back-to-back
+    figure is kept beside it."""
     import numpy as np
     rows = np.repeat(np.arange(c.n), np.diff(c.row - c.base))
     cols = c.col - c.base
     ntri_l, ntri_u = int(np.sum(cols < rows)), int(np.sum(cols > rows))
     algo = 0.5 * (sweep_bytes(c.n, ntri_l) + sweep_bytes(c.n, ntri_u))
-    sw = bd["sweeps"]
-    launch_us = 1e3 * A.sweep_time(reps=20)
-    inloop_us = 1e3 * sw["ms_per_solve"] / max(sw["launches_per_solve"], 1)
-    achieved = algo / (launch_us * 1e-6) / 1e9
+    b2b_us = 1e3 * A.sweep_time(reps=20)
+    achieved = algo / (inloop_us * 1e-6) / 1e9
     return {
         "kernel": "k_sweep_syncfree (FP64 forward/backward incomplete-Cholesky sweeps)",
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": measured_traffic(traffic_profile) if traffic_profile else None,
-        "algorithmic_bytes": algo, "launch_us": launch_us, "launch_us_in_solves": inloop_us,
+        "algorithmic_bytes": algo, "launch_us": inloop_us, "timing": "in the timed solves",
+        "launch_us_back_to_back": b2b_us,
         "note": "dependency-latency bound (DAG depth), not bandwidth bound",
     }
 
@@ -231,7 +235,7 @@ def single_solve(args, k):
     import numpy as np
     import torch
     from tflows_amd import cases, mesh, solvers
-    kw = cases.config_kwargs(k)
+    kw = cases.workload_kwargs(k)
     solve = cases.SOLVE[k]
     log(f"generating workload {k}: {kw}")
     t0 = time.perf_counter()
@@ -256,24 +260,30 @@ def single_solve(args, k):
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    lib.tfs_timing_start(0)  # HIP events around every SpMV launch of the timed region
+    # per-class kernel times of one solve; the dominant class is then timed
+    # with HIP events around each of its launches through the timed region
+    bd = breakdown(lib, step)
+    dominant_sweeps = bd["sweeps"]["ms_per_solve"] > bd["spmv"]["ms_per_solve"]
+    lib.tfs_timing_start(1 if dominant_sweeps else 0)
     elapsed, lat, res = timed(step, args.steps, 0, torch.cuda.synchronize)
-    spmv_ms, spmv_n = ctypes.c_double(), ctypes.c_longlong()
-    lib.tfs_timing_stop(ctypes.byref(spmv_ms), ctypes.byref(spmv_n))
+    dom_ms, dom_n = ctypes.c_double(), ctypes.c_longlong()
+    lib.tfs_timing_stop(ctypes.byref(dom_ms), ctypes.byref(dom_n))
+    dom_launch_ms = dom_ms.value / max(dom_n.value, 1)
     niter, ini_res, fin_res = res
     value = args.steps / elapsed
     log(f"  {value:.2f} solves/s, {1e3 * elapsed / args.steps:.3f} ms/solve, niter={niter}, "
         f"fin_res={fin_res:.3e} (tol {tol:.3e})")
-    bd = breakdown(lib, step)
     y = torch.empty_like(b)
+    spmv_in_solve = bd["spmv"]["ms_per_solve"] / max(bd["spmv"]["launches_per_solve"], 1)
+    sweep_in_solve = bd["sweeps"]["ms_per_solve"] / max(bd["sweeps"]["launches_per_solve"], 1)
     roof = spmv_roofline(A, x, y, c.n, c.nnz,
-                         inloop_ms=spmv_ms.value / max(spmv_n.value, 1),
-                         traffic_profile=f"k_spmv_workload{k}_counters")
-    sroof = sweep_roofline(A, c, bd, traffic_profile=f"k_sweep_workload{k}_counters")
-    log(f"  spmv {roof['launch_us']:.1f} us back-to-back ({roof['achieved']:.0f} GB/s); "
-        f"sweep {sroof['launch_us']:.1f} us ({sroof['achieved']:.0f} GB/s); breakdown {bd}")
-    # the dominant kernel of the solve is the one the roofline line reports
-    dominant_sweeps = bd["sweeps"]["ms_per_solve"] > bd["spmv"]["ms_per_solve"]
+                         inloop_ms=spmv_in_solve if dominant_sweeps else dom_launch_ms,
+                         traffic_profile=f"k_spmv_{WORKLOAD_TAG[k]}_counters")
+    sroof = sweep_roofline(A, c, 1e3 * (dom_launch_ms if dominant_sweeps else sweep_in_solve),
+                           traffic_profile=f"k_sweep_{WORKLOAD_TAG[k]}_counters")
+    log(f"  spmv {roof['launch_us']:.1f} us in solves ({roof['achieved']:.0f} GB/s), "
+        f"{roof['launch_us_flushed']:.1f} us flushed; sweep {sroof['launch_us']:.1f} us in "
+        f"solves ({sroof['achieved']:.0f} GB/s); breakdown {bd}")
     # the solve again, for the solution compared with the CPU
     step()
     x_gpu = x.cpu().numpy()
@@ -283,8 +293,8 @@ def single_solve(args, k):
         cpu, err = cpu_full_solves(c, solve, tol, args.cpu_reps, x_gpu)
         log(f"  cpu {cpu['value']:.3f} solves/s ({cpu['kind']}), rel-L2 {err:.2e}")
     lat_ms = np.array(lat) * 1e3
-    name = ("config 2: momentum 128^3 hex (tanh beta 2, 10% jitter, permuted), Bicg + DIC "
-            "(prec 2) to rel 1e-10" if k == 2 else "config 1: Poisson 32^3, Cg + DIC to rel 1e-8")
+    name = ("momentum 128^3 hex (tanh beta 2, 10% jitter, permuted), Bicg + DIC "
+            "(prec 2) to rel 1e-10" if k == 2 else "Poisson 32^3, Cg + DIC to rel 1e-8")
     out = line(args, value, elapsed, name, {
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
                        "p99": float(np.percentile(lat_ms, 99))},
@@ -305,7 +315,7 @@ def mixed_vs_fp64(args):
     import numpy as np
     import torch
     from tflows_amd import cases, mesh, solvers
-    kw = cases.config_kwargs(3, args.scale)
+    kw = cases.workload_kwargs(3, args.scale)
     solve = cases.SOLVE[3]
     log(f"generating workload 3: {kw}")
     t0 = time.perf_counter()
@@ -362,7 +372,7 @@ def mixed_vs_fp64(args):
         log(f"  cpu {cpu['value']:.4f} solves/s ({cpu['kind']})")
     lat = np.array(lat32) * 1e3
     out = line(args, args.steps / e32, e32,
-               f"config 3: Poisson {kw['nx']}^3 stretched to aspect 100, FP32-inner PCG "
+               f"Poisson {kw['nx']}^3 stretched to aspect 100, FP32-inner PCG "
                f"(DIC) in FP64 refinement ({args.refine} outer loop) to rel 1e-10, plain FP64 "
                f"Cg beside it", {
                    "latency_ms": {"p50": float(np.percentile(lat, 50)),
@@ -454,7 +464,7 @@ def batch(args):
                           f"{spent:.1f} s), scaled per cell to all {sum(ns)} cells, 1 thread on "
                           f"{cpu_name()}")
     out = line(args, len(sysl) * args.steps / e32, e32,
-               "config 4: 64 Poisson systems, cells per axis log-uniform [16,96], beta [1,3], "
+               "64 Poisson systems, cells per axis log-uniform [16,96], beta [1,3], "
                "one block-diagonal batch, FP32+FP64 refinement (DIC) to rel 1e-10", {
                    "latency_ms": {"p50": float(np.percentile(np.array(lat) * 1e3, 50)),
                                   "p99": float(np.percentile(np.array(lat) * 1e3, 99))},
@@ -551,7 +561,7 @@ def serving(args):
                           f"scaled per cell to the stream's {total_cells} cells, 1 thread on "
                           f"{cpu_name()}")
     out = line(args, value, makespan,
-               f"config 5: serving, Poisson arrivals {args.rate:g} req/s, {len(cs)} cases "
+               f"serving, Poisson arrivals {args.rate:g} req/s, {len(cs)} cases "
                f"log-uniform [1e4,4e6] cells, 70% pressure Cg / 30% momentum Bicg (DIC), "
                f"{args.visits} visits each, rel 1e-10, HBM budget {args.budget_gb:g} GB", {
                    "latency_ms": {"p50": float(np.percentile(lat, 50)),

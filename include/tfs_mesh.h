@@ -98,8 +98,10 @@ int tfs_layout_block_colour(int n, const int *row, const int *col, int block, in
                             int *colour);
 /* The layouts the solver builds for a matrix (0-based CRS): s_orig = caller
  * row at each S (reverse Cuthill-McKee) position; level of each caller row in
- * the lower and upper dependency DAGs of the caller's numbering; bandwidth of
- * the pattern in S order. */
+ * the lower dependency DAG of the caller's numbering (forward sweep) and in
+ * the backward sweep's order (the lower levels counted from the deepest, so
+ * every upper dependency of a row sits at a smaller level); bandwidth of the
+ * pattern in S order. */
 int tfs_layout_summary(int n, const int *row, const int *col, const int *dia, int *s_orig,
                        int *level_lower, int *level_upper, int *bandwidth);
 

@@ -8,6 +8,6 @@ Modules:
               through libtfs_solver.so (HIP, gfx950); plus SpMV, the FP32-inner
               refinement solver, the batched solver and the serving loop.
   mesh     -- synthetic hex-mesh cases and the CRS topology builder.
-  cases    -- the measured configurations (sizes, seeds).
+  cases    -- the measured workloads (sizes, seeds).
   build    -- in-tree build of the native libraries.
 """

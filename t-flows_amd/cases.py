@@ -49,7 +49,7 @@ def beta_for_aspect(n, aspect):
     return 0.5 * (lo + hi)
 
 
-def config_kwargs(k, scale=1.0):
+def workload_kwargs(k, scale=1.0):
     """make_case keyword arguments of workload k (1..3); `scale` shrinks the
     mesh per axis for parity runs at oracle-friendly sizes."""
     if k == 1:

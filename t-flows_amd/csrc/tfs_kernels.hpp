@@ -447,16 +447,6 @@ k_finish_seg(const int *__restrict__ tile_first, const double *__restrict__ part
 // partial ||rs||^2 (Cgs.f90:182-221, all row-local once alpha is known).
 enum SpmvMode { SPMV_PLAIN = 0, SPMV_RESID, SPMV_DOT, SPMV_CGS_TAIL };
 
-// Workgroup id renumbered XCD-major: hardware deals workgroup i to XCD i % 8;
-// the result r is a bijection of [0, n) under which XCD x holds one
-// contiguous range of r (the first n % 8 XCDs one more than the others).
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_major(int i, int n) {
-  const int x = i % kXcds, k = i / kXcds;
-  const int per = n / kXcds, tall = n % kXcds;  // XCDs 0..tall-1 hold per + 1
-  return x < tall ? x * (per + 1) + k : tall * (per + 1) + (x - tall) * per + k;
-}
-
 // WAVES: wavefronts per workgroup (2, 4 or 8; the row blocks, at most 64 rows
 // each, are a runtime layout) -- the two shapes the SpMV tuner picks.
 template <typename T, int MODE, int WAVES>
@@ -471,18 +461,12 @@ k_spmv(int nblk, const int *__restrict__ rblk, const int *__restrict__ row,
   T *lds = prod[wv];
   const T alpha = MODE == SPMV_CGS_TAIL ? (T)st->alpha : T(0);
   double acc_red = 0.0;
-  // XCD-aware work split: workgroups are dealt round-robin to the 8 XCDs,
-  // each with its own L2.  Every XCD takes one contiguous range of row blocks
-  // (in proportion to its workgroups), and its workgroups walk that range
-  // side by side (workgroup j of the XCD takes blocks j, j + m, ... of it):
-  // the blocks in flight on one XCD are neighbours, whose x gathers overlap,
-  // so the x window they need stays in that XCD's L2
-  const int xcd = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
-  const int m = ((int)gridDim.x - xcd + kXcds - 1) / kXcds;  // workgroups on this XCD
-  const int first = xcd_major(blockIdx.x, gridDim.x) - j;    // the XCD's first renumbered id
-  const int b_beg = (int)((long long)nblk * first / gridDim.x);
-  const int b_end = (int)((long long)nblk * (first + m) / gridDim.x);
-  for (int b = b_beg + j * WAVES + wv; b < b_end; b += m * WAVES) {
+  // Wavefront w of workgroup g takes row blocks g*WAVES + w, then every
+  // gridDim*WAVES-th.  Measured against splits that give each XCD (each L2)
+  // one contiguous range of blocks, this plain interleave moved more bytes
+  // per second both from HBM (cache-flushed launches, 128^3 and 256^3) and
+  // inside the solves.
+  for (int b = blockIdx.x * WAVES + wv; b < nblk; b += gridDim.x * WAVES) {
     // b is the same in every lane: the block's bounds come through the
     // scalar unit
     const int bu = __builtin_amdgcn_readfirstlane(b);

@@ -7,6 +7,9 @@
 namespace tfs {
 namespace {
 
+// The upper layout is the lower one backwards (see build_sweep).
+constexpr bool kMirrorUpper = true;
+
 // Breadth-first search from `start` over nodes whose mark differs from
 // `stamp` (marking them); neighbours are taken in ascending degree, ties by
 // number (Cuthill-McKee).  Appends the visit order to `out`, returns the
@@ -99,11 +102,24 @@ std::vector<int> rcm(int n, const int *row, const int *col) {
   return order;  // order[k] = caller row placed at position k
 }
 
+// With `mirror` (the lower layout, for the upper triangle) the levels are
+// the lower levels counted from the deepest one, ties in reverse S order: the
+// upper order is then the lower order backwards.  That is a valid order for
+// the backward sweep (a row's upper dependencies have larger caller numbers,
+// hence deeper lower levels, hence earlier mirrored positions), has the same
+// number of levels as its own level order, and lets the backward sweep read
+// the forward sweep's output and re-arm it in contiguous runs.
 void build_sweep(int n, const int *row, const int *col, const int *dia, bool lower,
-                 const std::vector<int> &s_of_orig, int small_level, SweepLayout &S) {
+                 const std::vector<int> &s_of_orig, int small_level, SweepLayout &S,
+                 const SweepLayout *mirror = nullptr) {
   std::vector<int> lev(n, 0);
   int nlev = 0;
-  if (lower) {
+  if (mirror) {
+    nlev = mirror->nlev;
+    for (int l = 0; l < nlev; ++l)
+      for (int p = mirror->lev_ptr[l]; p < mirror->lev_ptr[l + 1]; ++p)
+        lev[mirror->orig[p]] = nlev - 1 - l;
+  } else if (lower) {
     for (int i = 0; i < n; ++i) {
       int l = 0;
       for (int j = row[i]; j < dia[i]; ++j) l = std::max(l, lev[col[j]] + 1);
@@ -122,14 +138,15 @@ void build_sweep(int n, const int *row, const int *col, const int *dia, bool low
   S.lev_ptr.assign(nlev + 1, 0);
   for (int i = 0; i < n; ++i) S.lev_ptr[lev[i] + 1]++;
   for (int l = 0; l < nlev; ++l) S.lev_ptr[l + 1] += S.lev_ptr[l];
-  // order: by level, ties in S order (walk caller rows in S order)
+  // order: by level, ties in S order (walk caller rows in S order), or in
+  // reverse S order for a mirrored layout
   std::vector<int> s_orig(n);
   for (int i = 0; i < n; ++i) s_orig[s_of_orig[i]] = i;
   std::vector<int> fill(S.lev_ptr.begin(), S.lev_ptr.end() - 1);
   S.orig.assign(n, 0);
   S.of_orig.assign(n, 0);
   for (int k = 0; k < n; ++k) {
-    const int i = s_orig[k];
+    const int i = s_orig[mirror ? n - 1 - k : k];
     const int p = fill[lev[i]]++;
     S.orig[p] = i;
     S.of_orig[i] = p;
@@ -253,7 +270,8 @@ void build_layout(int n, const int *row, const int *col, const int *dia, int spm
     r = e;
   }
   build_sweep(n, row, col, dia, true, out.s_of_orig, small_level, out.L);
-  build_sweep(n, row, col, dia, false, out.s_of_orig, small_level, out.U);
+  build_sweep(n, row, col, dia, false, out.s_of_orig, small_level, out.U,
+              kMirrorUpper ? &out.L : nullptr);
   out.l_to_s.assign(n, 0);
   out.u_to_l.assign(n, 0);
   out.s_to_u.assign(n, 0);

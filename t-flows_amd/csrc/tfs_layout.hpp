@@ -12,7 +12,9 @@
 //      dependency DAG of the CALLER's numbering (which defines the
 //      incomplete factor), ties in S order; only the strictly-lower entries
 //      are stored, contiguously, level after level.
-//   U  (backward sweep): the same for the upper DAG and strictly-upper entries.
+//   U  (backward sweep): the L order backwards, strictly-upper entries.  A
+//      row's upper dependencies sit at deeper lower levels, so they come
+//      first; the backward sweep reads the forward sweep's output in runs.
 #pragma once
 
 #include <vector>

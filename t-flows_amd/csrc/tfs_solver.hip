@@ -1425,6 +1425,24 @@ extern "C" int tfs_spmv_time(tfs_matrix *A, const double *d_x, double *d_y, int 
   return TFS_OK;
 }
 
+namespace {
+
+// Reads m words and stores nothing unless their xor is a value the caller
+// never places there: a read-only sweep over a buffer larger than the
+// memory-side cache leaves it holding clean lines of that buffer only, so a
+// kernel timed next finds none of its own data cached and has no dirty
+// lines to write back.
+__global__ void __launch_bounds__(kRedThreads)
+k_flush_read(long long m, const unsigned long long *__restrict__ buf, unsigned long long *out) {
+  unsigned long long acc = 0;
+  for (long long i = blockIdx.x * (long long)kRedThreads + threadIdx.x; i < m;
+       i += (long long)gridDim.x * kRedThreads)
+    acc ^= buf[i];
+  if (acc == 0x5a5a5a5a5a5a5a5aULL) *out = acc;
+}
+
+}  // namespace
+
 extern "C" int tfs_spmv_time_cold(tfs_matrix *A, const double *d_x, double *d_y, int reps,
                                   double *ms) {
   if (!A) return TFS_ERR_INVALID;
@@ -1435,17 +1453,17 @@ extern "C" int tfs_spmv_time_cold(tfs_matrix *A, const double *d_x, double *d_y,
     return TFS_OK;
   }
   TFS_TRY(prepare_values<double>(A));
-  // 2 x 320 MiB, copied one onto the other before every launch: more than
-  // the 256 MB memory-side cache, so the SpMV finds none of its data there
-  const size_t half = (size_t)320 << 20;
-  char *flush = nullptr;
-  TFS_HIP(hipMalloc((void **)&flush, 2 * half));
+  // 640 MiB read before every launch: more than the 256 MB memory-side
+  // cache, so the SpMV finds none of its data there
+  const long long words = (640LL << 20) / 8;
+  unsigned long long *flush = nullptr;
+  TFS_HIP(hipMalloc((void **)&flush, (size_t)words * 8 + 8));
   float total = 0.f;
   int rc = 0;
   for (int k = 0; k < reps && rc == 0; ++k) {
-    if (hipMemcpyAsync(flush + half, flush, half, hipMemcpyDeviceToDevice, cur_stream()) !=
-            hipSuccess ||
-        hipEventRecord(A->ev0, cur_stream()) != hipSuccess) {
+    hipLaunchKernelGGL(k_flush_read, dim3(4096), dim3(kRedThreads), 0, cur_stream(), words, flush,
+                       flush + words);
+    if (hipGetLastError() != hipSuccess || hipEventRecord(A->ev0, cur_stream()) != hipSuccess) {
       rc = TFS_ERR_DEVICE;
       break;
     }

@@ -65,13 +65,13 @@ def main():
             out[f"{name}_prec{prec}"] = dict(case=kw, sha256=digest(c), prec=prec, z=z, d=d)
         rr = R.residual(c.b, c.b)
         out[f"{name}_residual"] = dict(case=kw, sha256=digest(c), r=rr)
-    # the 32^3 Poisson configuration: CG, DIC, relative 1e-8, cap 1e4
-    kw1 = cases.config_kwargs(1)
+    # the 32^3 Poisson workload: CG, DIC, relative 1e-8, cap 1e4
+    kw1 = cases.workload_kwargs(1)
     c = mesh.make_case(**kw1)
     R.set_matrix(c.row, c.col, c.dia, c.val)
     tol = 1e-8 * float(np.linalg.norm(c.b))
     r = R.solve("cg", c.b, prec=2, niter=10000, tol=tol)
-    out["config1_cg_p2"] = dict(case=kw1, sha256=digest(c), method="cg", prec=2, tol=tol,
+    out["poisson32_cg_p2"] = dict(case=kw1, sha256=digest(c), method="cg", prec=2, tol=tol,
                                 cap=10000, niter=r["niter"], ini_res=r["ini_res"],
                                 fin_res=r["fin_res"], x=r["x"], r=r["r"])
     # topology of a permuted jittered mesh (Matrix_Mod_Topology)

@@ -42,8 +42,8 @@ CASES = {
                          permute=True),
     "momentum10": dict(kind=mesh.MOMENTUM, nx=10, beta=2.0, jitter=0.1, permute=True,
                        pin_first=False, rhs_zero_mean=False),
-    "config1": cases.config_kwargs(1),
-    "config2_small": cases.config_kwargs(2, scale=0.25),
+    "poisson32": cases.workload_kwargs(1),
+    "momentum32": cases.workload_kwargs(2, scale=0.25),
 }
 
 _case_cache = {}
@@ -160,7 +160,7 @@ def test_factor_and_sweeps_bitwise(torch, S, oracle, name, prec, sweep, monkeypa
     assert np.array_equal(host(z), oracle.prec_solve(c.row, c.col, c.dia, c.val, d_or, prec, c.b))
 
 
-@pytest.mark.parametrize("name", ["poisson12", "poisson_perm", "momentum10", "config2_small"])
+@pytest.mark.parametrize("name", ["poisson12", "poisson_perm", "momentum10", "momentum32"])
 @pytest.mark.parametrize("method", ["cg", "bicg", "cgs"])
 @pytest.mark.parametrize("prec", [0, 1, 2])
 def test_solve_parity(torch, S, oracle, name, method, prec):
@@ -185,11 +185,11 @@ def test_solve_parity(torch, S, oracle, name, method, prec):
         assert niter == cap + 1 and A.status == S.TFS_NOT_CONVERGED
 
 
-def test_config1_against_reference_golden(torch, S):
+def test_poisson32_against_reference_golden(torch, S):
     """32^3 Poisson, Cg prec 2 to 1e-8: the reference's own outputs."""
-    z = np.load(GOLDEN / "config1_cg_p2.npz", allow_pickle=False)
+    z = np.load(GOLDEN / "poisson32_cg_p2.npz", allow_pickle=False)
     meta = json.loads(str(z["meta"]))
-    c = get_case("config1")
+    c = get_case("poisson32")
     A = matrix(S, c)
     x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
     r = dev(torch, c.b)
@@ -273,11 +273,11 @@ def test_values_update_invalidates_factor(torch, S, oracle):
     assert np.array_equal(host(d), oracle.prec_form(c.row, c.col, c.dia, val2, 2))
 
 
-def test_config2_full_size(torch, S, oracle):
+def test_momentum128_full_size(torch, S, oracle):
     """128^3 momentum (permuted, jittered, stretched), Bicg prec 2 to 1e-10:
     GPU against the oracle at full size, plus the true residual b - A x."""
-    c = mesh.make_case(**cases.config_kwargs(2))
-    print(f"config2 n={c.n} nnz={c.nnz}", flush=True)
+    c = mesh.make_case(**cases.workload_kwargs(2))
+    print(f"momentum 128^3 n={c.n} nnz={c.nnz}", flush=True)
     tol = 1e-10 * float(np.linalg.norm(c.b))
     ref = oracle.solve("bicg", c.row, c.col, c.dia, c.val, c.b, prec=2, niter=10000, tol=tol)
     A = matrix(S, c)
@@ -297,7 +297,7 @@ def test_config2_full_size(torch, S, oracle):
 def test_spmv_shapes_bitwise(torch, S, oracle, shape):
     """Every SpMV shape (rows per block x wavefronts per workgroup) forms each
     row sum exactly as the reference does."""
-    c = get_case("config2_small")
+    c = get_case("momentum32")
     A = matrix(S, c)
     A.set_spmv_shape(*shape)
     assert (A.info()["spmv_rows"], A.info()["spmv_waves"]) == shape

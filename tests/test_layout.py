@@ -53,7 +53,12 @@ def test_levels_follow_the_callers_dags(name):
         nb = neighbours(c, i)
         lo, up = nb[nb < i], nb[nb > i]
         assert L["level_lower"][i] == (L["level_lower"][lo].max() + 1 if lo.size else 0)
-        assert L["level_upper"][i] == (L["level_upper"][up].max() + 1 if up.size else 0)
+        # the backward sweep runs the lower levels from the deepest: every
+        # upper dependency (a larger caller number) is at a smaller level
+        if up.size:
+            assert L["level_upper"][up].max() < L["level_upper"][i]
+    nlev = L["level_lower"].max() + 1
+    assert np.array_equal(L["level_upper"], nlev - 1 - L["level_lower"])
 
 
 def test_box_depth_and_rcm_bandwidth():

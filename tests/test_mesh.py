@@ -65,8 +65,8 @@ def test_momentum_matrix_is_an_m_matrix():
 
 
 def test_cases_are_deterministic():
-    a = mesh.make_case(**cases.config_kwargs(2, scale=0.1))
-    b = mesh.make_case(**cases.config_kwargs(2, scale=0.1))
+    a = mesh.make_case(**cases.workload_kwargs(2, scale=0.1))
+    b = mesh.make_case(**cases.workload_kwargs(2, scale=0.1))
     assert np.array_equal(a.val, b.val) and np.array_equal(a.col, b.col)
     assert np.array_equal(a.b, b.b)
 

@@ -314,10 +314,10 @@ def test_oracle_bitwise_vs_golden(oracle, path):
     assert np.array_equal(r["x"], arr["x"]) and np.array_equal(r["r"], arr["r"])
 
 
-def test_config1_golden_meets_tolerance():
-    meta, arr = _load(GOLDEN / "config1_cg_p2.npz")
+def test_poisson32_golden_meets_tolerance():
+    meta, arr = _load(GOLDEN / "poisson32_cg_p2.npz")
     assert meta["niter"] < meta["cap"] and meta["fin_res"] < meta["tol"]
-    assert meta["case"] == cases.config_kwargs(1) or meta["case"]["nx"] == 32
+    assert meta["case"] == cases.workload_kwargs(1) or meta["case"]["nx"] == 32
 
 
 # ---------------------------------------------------------------- planted mistakes
