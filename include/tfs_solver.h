@@ -83,11 +83,23 @@ int tfs_matrix_values_changed(tfs_matrix *A);
 double *tfs_matrix_values_device(tfs_matrix *A);
 
 /* Schedule of the preconditioner sweeps (bitwise the same results):
- *   TFS_SWEEP_LEVELS     one launch per level of the dependency DAG;
- *   TFS_SWEEP_STATIC     one persistent cooperative launch per sweep (default);
- *   TFS_SWEEP_DYNAMIC    one persistent launch taking chunks from a counter --
- *                        safe when several solves share the GPU concurrently. */
-enum { TFS_SWEEP_LEVELS = 0, TFS_SWEEP_STATIC = 1, TFS_SWEEP_DYNAMIC = 2 };
+ *   TFS_SWEEP_LEVELS       one launch per level of the dependency DAG;
+ *   TFS_SWEEP_STATIC       one persistent launch per sweep, workgroup w owning
+ *                          row chunks w, w+G, ... (default where it times
+ *                          faster than the levels); needs all G workgroups
+ *                          resident, which an ordinary launch gets unless a
+ *                          never-ending kernel holds CUs -- then the sweep's
+ *                          bounded waits end it with TFS_ERR_DEVICE;
+ *   TFS_SWEEP_DYNAMIC      one persistent launch taking chunks from a counter --
+ *                          safe when several solves share the GPU concurrently;
+ *   TFS_SWEEP_COOPERATIVE  the static schedule under a cooperative launch (the
+ *                          runtime guarantees co-residency; slower to launch). */
+enum {
+  TFS_SWEEP_LEVELS = 0,
+  TFS_SWEEP_STATIC = 1,
+  TFS_SWEEP_DYNAMIC = 2,
+  TFS_SWEEP_COOPERATIVE = 3
+};
 int tfs_matrix_set_sweep_schedule(tfs_matrix *A, int schedule);
 
 typedef struct {

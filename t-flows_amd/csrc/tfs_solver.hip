@@ -158,6 +158,7 @@ struct tfs_matrix {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool persistent = true;      // one launch per sweep (else one per level)
   bool dynamic = false;        // persistent sweeps take chunks from a counter
+  bool coop = false;           // static schedule under a cooperative launch
   bool sweep_fixed = false;    // schedule set by the caller or the environment
   bool sweep_tuned = false;    // schedule picked by timing (first factor)
   unsigned int *sweep_ctr = nullptr;  // the dynamic schedule's counters
@@ -258,6 +259,22 @@ void launch_sweep(tfs_matrix *A, const DevSweep &S, const Tri<T> &Tr, T *d, cons
   if (A->persistent && A->dynamic) {
     tm_begin(kc);
     hipLaunchKernelGGL((k_sweep_syncfree<T, OP, true>), dim3(A->grid_persistent), dim3(kChunk), 0,
+                       cur_stream(), A->n, Tr, d, const_cast<T *>(b), x, armv, st, A->sweep_err,
+                       A->sweep_ctr);
+    tm_end(kc);
+    return;
+  }
+  if (A->persistent && !A->coop) {
+    // The static schedule needs every workgroup of the grid resident at
+    // once.  The grid is capped at the occupancy of the heaviest sweep
+    // kernel, so an ordinary launch has all its workgroups dispatched as
+    // soon as the workgroups of other, terminating kernels have left; it
+    // saves the 10-20 us of queue synchronisation a cooperative launch
+    // costs on each side.  Only another never-ending kernel holding CUs
+    // could keep a workgroup out: the bounded polls then end the sweep with
+    // an error (solves sharing the GPU use the dynamic schedule).
+    tm_begin(kc);
+    hipLaunchKernelGGL((k_sweep_syncfree<T, OP, false>), dim3(A->grid_persistent), dim3(kChunk), 0,
                        cur_stream(), A->n, Tr, d, const_cast<T *>(b), x, armv, st, A->sweep_err,
                        A->sweep_ctr);
     tm_end(kc);
@@ -690,6 +707,7 @@ int build_inner(tfs_matrix *A) {
   }
   C->persistent = A->persistent;
   C->dynamic = A->dynamic;
+  C->coop = A->coop;
   A->inner = C;
   A->inner_colours = ncol;
   A->inner_ready = false;
@@ -1029,12 +1047,11 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
   TFS_HIP(hipMemset(A->d_st, 0, 2 * sizeof(KState)));
   TFS_HIP(hipMemset(A->val, 0, sizeof(double) * (size_t)std::max(A->nnz, 1LL)));
   TFS_HIP(hipHostMalloc((void **)&A->h_st, 2 * sizeof(KState), hipHostMallocDefault));
-  // Persistent sweeps: as many workgroups as can be co-resident for the
-  // heaviest sweep kernel (the cooperative launch refuses more).
-  int dev = 0, n_cu = 0, per_cu = 8, coop = 0;
+  // Persistent sweeps: at most as many workgroups as can be co-resident for
+  // the heaviest sweep kernel (the static schedule needs them all resident).
+  int dev = 0, n_cu = 0, per_cu = 8;
   TFS_HIP(hipGetDevice(&dev));
   TFS_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-  TFS_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
   TFS_TRY((occupancy<double, OP_FACTOR>(&per_cu)));
   TFS_TRY((occupancy<double, OP_FWD>(&per_cu)));
   TFS_TRY((occupancy<double, OP_BWD>(&per_cu)));
@@ -1048,10 +1065,11 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
   const int chunks = (n + kChunk * kSweepRows - 1) / (kChunk * kSweepRows);
   A->grid_persistent =
       std::max(1, std::min({chunks, n_cu * std::max(1, per_cu), kSweepGroups}));
-  A->persistent = coop != 0 && per_cu > 0;
+  A->persistent = per_cu > 0;
   if (const char *e = std::getenv("TFS_SWEEP")) {
     if (std::strcmp(e, "levels") == 0) A->persistent = false;
     if (std::strcmp(e, "dynamic") == 0) A->dynamic = true;
+    if (std::strcmp(e, "cooperative") == 0) A->coop = true;
     A->sweep_fixed = true;  // "static" (or anything else): the persistent static schedule
   }
   if (const char *e = std::getenv("TFS_SWEEP_GROUPS")) {  // cap on the persistent grid
@@ -1605,9 +1623,9 @@ void batch_cg_iteration(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KS
 }
 
 // Captures kGraphIters batched iterations into a graph (once per precision
-// and preconditioner).  If the capture fails -- e.g. a cooperative launch is
-// not capturable -- it is retried with per-level sweeps, and without a graph
-// as a last resort.
+// and preconditioner).  If the capture fails -- e.g. under the cooperative
+// schedule, whose launch is not capturable -- it is retried with per-level
+// sweeps, and without a graph as a last resort.
 template <typename T>
 int batch_graph(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KState *all,
                 hipGraphExec_t *exec) {
@@ -1869,6 +1887,7 @@ int build_batch_inner(tfs_batch *B) {
   }
   C->A->persistent = A->persistent;
   C->A->dynamic = A->dynamic;
+  C->A->coop = A->coop;
   B->inner = C;
   A->inner_ready = false;
   return 0;
@@ -2298,12 +2317,14 @@ extern "C" int tfs_batch_get_info(const tfs_batch *B, int *nsys, int *n_total,
 }
 
 extern "C" int tfs_matrix_set_sweep_schedule(tfs_matrix *A, int schedule) {
-  if (!A || schedule < TFS_SWEEP_LEVELS || schedule > TFS_SWEEP_DYNAMIC) return TFS_ERR_INVALID;
+  if (!A || schedule < TFS_SWEEP_LEVELS || schedule > TFS_SWEEP_COOPERATIVE)
+    return TFS_ERR_INVALID;
   StreamScope scope(A->stream);
   TFS_HIP(hipStreamSynchronize(A->stream));
   const bool was_levels = !A->persistent;
   A->persistent = schedule != TFS_SWEEP_LEVELS;
   A->dynamic = schedule == TFS_SWEEP_DYNAMIC;
+  A->coop = schedule == TFS_SWEEP_COOPERATIVE;
   A->sweep_fixed = true;
   if (was_levels && A->persistent) {
     // the persistent sweeps need their outputs armed

@@ -38,7 +38,7 @@ TFS_ERR_NO_DEVICE = -5
 CG, BICG, CGS = 0, 1, 2
 TFS_INNER_REFERENCE, TFS_INNER_COLOURED = 0, 1
 TFS_REFINE_FCG, TFS_REFINE_DEFECT = 0, 1
-TFS_SWEEP_LEVELS, TFS_SWEEP_STATIC, TFS_SWEEP_DYNAMIC = 0, 1, 2
+TFS_SWEEP_LEVELS, TFS_SWEEP_STATIC, TFS_SWEEP_DYNAMIC, TFS_SWEEP_COOPERATIVE = 0, 1, 2, 3
 _METHODS = {"cg": CG, "bicg": BICG, "cgs": CGS}
 
 _I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -197,7 +197,8 @@ class Matrix:
                "tfs_matrix_set_spmv_shape")
 
     def set_sweep_schedule(self, schedule):
-        """TFS_SWEEP_LEVELS / _STATIC / _DYNAMIC (otherwise timed on the first factor)."""
+        """TFS_SWEEP_LEVELS / _STATIC / _DYNAMIC / _COOPERATIVE (otherwise timed on the
+        first factor)."""
         _check(self._lib.tfs_matrix_set_sweep_schedule(self._h, int(schedule)),
                "tfs_matrix_set_sweep_schedule")
 

@@ -134,12 +134,14 @@ def test_spmv_and_residual_bitwise(torch, S, oracle, name):
 
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("prec", [0, 1, 2])
-@pytest.mark.parametrize("sweep", ["static", "dynamic", "levels", "env-levels", "timed"])
+@pytest.mark.parametrize("sweep", ["static", "dynamic", "cooperative", "levels", "env-levels",
+                                   "timed"])
 def test_factor_and_sweeps_bitwise(torch, S, oracle, name, prec, sweep, monkeypatch):
-    """Every sweep schedule: one persistent launch per sweep (static or
-    dynamic chunk order), one launch per level (set on the matrix or by
-    TFS_SWEEP=levels when it is created), and the schedule timed on the
-    first factor (the default)."""
+    """Every sweep schedule: one persistent launch per sweep (static chunk
+    order under an ordinary or a cooperative launch, or dynamic chunk
+    order), one launch per level (set on the matrix or by TFS_SWEEP=levels
+    when it is created), and the schedule timed on the first factor (the
+    default)."""
     c = get_case(name)
     print(f"case {name} prec={prec} sweep={sweep}", flush=True)
     if sweep == "env-levels":
@@ -147,8 +149,9 @@ def test_factor_and_sweeps_bitwise(torch, S, oracle, name, prec, sweep, monkeypa
     else:
         monkeypatch.delenv("TFS_SWEEP", raising=False)
     A = matrix(S, c)
-    if sweep in ("static", "dynamic", "levels"):
+    if sweep in ("static", "dynamic", "cooperative", "levels"):
         A.set_sweep_schedule({"static": S.TFS_SWEEP_STATIC, "dynamic": S.TFS_SWEEP_DYNAMIC,
+                              "cooperative": S.TFS_SWEEP_COOPERATIVE,
                               "levels": S.TFS_SWEEP_LEVELS}[sweep])
     A.prec_form(prec)
     d = torch.zeros(c.n, dtype=torch.float64, device="cuda")
