@@ -647,17 +647,21 @@ __device__ __forceinline__ void coherent_store(T *p, T v) {
 // caller's entry order -- so the result is bitwise the level-scheduled one.
 // Flag and data share one location, so no fence or counter is needed.
 // Workgroup w takes chunks w, w+G, ... (kChunk*kSweepRows positions each, in level
-// order); positions depend only on lower positions, and the cooperative launch
-// keeps all G workgroups resident, so the lowest unfinished row always belongs
-// to a running workgroup whose dependencies are complete.  Every wait is
+// order); positions depend only on lower positions, and with all G workgroups
+// resident (G is capped at the kernel's occupancy) the lowest unfinished row
+// always belongs to a running workgroup whose dependencies are complete.
+// Memory round trips per chunk: (1) everything indexed by the position alone
+// (row pointers, factor, right-hand-side index or diagonal); (2) the row's
+// entries and its right-hand side; (3) all of its dependencies at once --
+// they are loaded together and then consumed in entry order.  Every wait is
 // bounded (kSpinLimit polls): on expiry the kernel records an error and stores
 // a value, so it always terminates.
 // Sentinel upkeep (no extra passes): FWD arms arm[p] (the backward sweep's
 // output); BWD re-arms b at the entry it consumes (the forward sweep's
 // output); FACTOR's output is armed by the caller.
 //
-// DYN selects the schedule: false -- static (chunk w, w+G, ...; needs the
-// cooperative launch's co-residency); true -- dynamic: chunks are taken in
+// DYN selects the schedule: false -- static (chunk w, w+G, ...; needs all G
+// workgroups co-resident); true -- dynamic: chunks are taken in
 // order from a counter (ctr[0]), so a chunk is only ever held by a running
 // workgroup and the schedule is deadlock-free whatever else shares the GPU
 // (used when several solves run concurrently).  The last workgroup to leave
@@ -682,22 +686,30 @@ k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, i
     if (c >= nchunks) break;
     // R rows per thread, kChunk apart (consecutive threads, consecutive rows):
     // their loads are issued together and their waits overlap
-    int p[R], e0[R], len[R], used[R];
+    int p[R], e0[R], len[R], used[R], src[R];
     bool done[R];
     T s[R], dp[R];
     int pc[R][kPrefetch];
     T pv[R][kPrefetch];
+    // round 1: what the position alone indexes
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       p[j] = c * kSpan + j * kChunk + (int)threadIdx.x;
       done[j] = p[j] >= n;
-      e0[j] = len[j] = used[j] = 0;
+      e0[j] = len[j] = used[j] = src[j] = 0;
       dp[j] = s[j] = T(0);
       if (!done[j]) {
         e0[j] = Tr.ptr[p[j]];
         len[j] = Tr.ptr[p[j] + 1] - e0[j];
+        if (OP == OP_FACTOR) {
+          s[j] = Tr.diag[p[j]];
+        } else {
+          dp[j] = d[p[j]];
+          src[j] = Tr.in_idx[p[j]];
+        }
       }
     }
+    // round 2: the row's entries and its right-hand side
 #pragma unroll
     for (int j = 0; j < R; ++j) {
 #pragma unroll
@@ -707,18 +719,12 @@ k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, i
           pv[j][k] = Tr.val[e0[j] + k];
         }
       }
-      if (done[j]) continue;
-      if (OP == OP_FACTOR) {
-        s[j] = Tr.diag[p[j]];
-      } else {
-        dp[j] = d[p[j]];
-        const int src = Tr.in_idx[p[j]];
-        s[j] = b[src];
-        if (OP == OP_FWD) arm[p[j]] = Sentinel<T>::value();
-        if (OP == OP_BWD) {
-          b[src] = Sentinel<T>::value();  // this sweep is the only reader of b[src]
-          s[j] = s[j] / (dp[j] + tiny<T>());
-        }
+      if (done[j] || OP == OP_FACTOR) continue;
+      s[j] = b[src[j]];
+      if (OP == OP_FWD) arm[p[j]] = Sentinel<T>::value();
+      if (OP == OP_BWD) {
+        b[src[j]] = Sentinel<T>::value();  // this sweep is the only reader of b[src]
+        s[j] = s[j] / (dp[j] + tiny<T>());
       }
     }
     // A row's value is stored INSIDE the polling loop, in the iteration in
@@ -732,6 +738,9 @@ k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, i
     // load returns either the final value or a (possibly stale) sentinel.
     // Only a sentinel sends the row to the device-coherent polls, which
     // bypass the XCD's L2 and fetch whole lines from memory.
+    // Each pass loads every outstanding prefetched dependency at once, then
+    // folds them in entry order up to the first sentinel (the values after
+    // it are loaded again in the next pass).
     unsigned int spins = 0;
     bool first = true;
     for (;;) {
@@ -739,26 +748,30 @@ k_sweep_syncfree(int n, Tri<T> Tr, T *d, T *b, T *x, T *arm, const KState *st, i
 #pragma unroll
       for (int j = 0; j < R; ++j) {
         if (done[j]) continue;
+        T v[kPrefetch];
+#pragma unroll
+        for (int k = 0; k < kPrefetch; ++k)
+          if (k >= used[j] && k < len[j])
+            v[k] = first ? out[pc[j][k]] : coherent_load(&out[pc[j][k]]);
         bool blocked = false;
 #pragma unroll
         for (int k = 0; k < kPrefetch; ++k) {
           if (!blocked && k >= used[j] && k < len[j]) {
-            const T v = first ? out[pc[j][k]] : coherent_load(&out[pc[j][k]]);
-            if (Sentinel<T>::is(v)) {
+            if (Sentinel<T>::is(v[k])) {
               blocked = true;
             } else {
-              s[j] = OP == OP_FACTOR ? s[j] - v * pv[j][k] * pv[j][k] : s[j] - pv[j][k] * v;
+              s[j] = OP == OP_FACTOR ? s[j] - v[k] * pv[j][k] * pv[j][k] : s[j] - pv[j][k] * v[k];
               used[j] = k + 1;
             }
           }
         }
         for (int e = e0[j] + used[j]; !blocked && e < e0[j] + len[j]; ++e) {  // beyond the prefetch
-          const T v = first ? out[Tr.col[e]] : coherent_load(&out[Tr.col[e]]);
-          if (Sentinel<T>::is(v)) {
+          const T w = first ? out[Tr.col[e]] : coherent_load(&out[Tr.col[e]]);
+          if (Sentinel<T>::is(w)) {
             blocked = true;
           } else {
             const T a = Tr.val[e];
-            s[j] = OP == OP_FACTOR ? s[j] - v * a * a : s[j] - a * v;
+            s[j] = OP == OP_FACTOR ? s[j] - w * a * a : s[j] - a * w;
             used[j] = e - e0[j] + 1;
           }
         }
