@@ -1062,7 +1062,7 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
   // the whole chip's worth of pollers the polls delay the stores they wait
   // for (measured on MI355X: a 382-level sweep takes half the time with 256
   // to 1024 workgroups as with 2048, a 19-level one 13 % less with 1024).
-  const int chunks = (n + kChunk * kSweepRows - 1) / (kChunk * kSweepRows);
+  const int chunks = (n + kChunk - 1) / kChunk;
   A->grid_persistent =
       std::max(1, std::min({chunks, n_cu * std::max(1, per_cu), kSweepGroups}));
   A->persistent = per_cu > 0;
