@@ -39,7 +39,7 @@ constexpr int kSmallLevel = 2048;        // a level this small joins a walked ru
 constexpr int kChunk = 256;              // threads per persistent-sweep workgroup
 constexpr int kPrefetch = 8;             // row entries held in registers
 constexpr unsigned kSpinLimit = 1u << 20;
-constexpr int kSweepGroups = 1024;       // cap on a persistent sweep's workgroups
+constexpr int kSweepGroups = 768;        // cap on a persistent sweep's workgroups
 constexpr int kInnerBlock = 1;           // rows per colour block of the FP32 inner order
 
 // EPI_AUX0..5 store the plain sum into aux[k] (dot products), EPI_NAUX0..5
@@ -275,6 +275,54 @@ k_gather(int n, const TI *__restrict__ in, const int *__restrict__ idx, TO *__re
          const KState *st) {
   if (st && solve_done(st)) return;
   TFS_GRID_STRIDE(i, n) out[i] = (TO)in[idx ? idx[i] : i];
+}
+
+// The values in every layout in one pass, one thread per S row: the caller
+// row's entries are read once (contiguous, caller entry order) and written to
+// the S copy, the two diagonal copies, and the strictly-lower / strictly-upper
+// runs of the row in the L / U layouts (which keep the caller's entry order:
+// the lower run is the entries before the diagonal, the upper run those
+// after it).
+template <typename T>
+__device__ __forceinline__ void values_put(int j, T v, int e0, int dj, int l0, int u0, int k,
+                                           int p, T *s_val, T *s_diag, T *l_val, T *u_val,
+                                           T *l_diag) {
+  s_val[e0 + j] = v;
+  if (j < dj) {
+    l_val[l0 + j] = v;
+  } else if (j == dj) {
+    s_diag[k] = v;
+    l_diag[p] = v;
+  } else {
+    u_val[u0 + (j - dj - 1)] = v;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_values(int n, const double *__restrict__ val, const int *__restrict__ s_row,
+         const int *__restrict__ s_src, const int *__restrict__ s_diag_src,
+         const int *__restrict__ s_lpos, const int *__restrict__ s_to_u,
+         const int *__restrict__ l_ptr, const int *__restrict__ u_ptr, T *__restrict__ s_val,
+         T *__restrict__ s_diag, T *__restrict__ l_val, T *__restrict__ u_val,
+         T *__restrict__ l_diag) {
+  TFS_GRID_STRIDE(k, n) {
+    const int e0 = s_row[k], len = s_row[k + 1] - e0;
+    const int c0 = s_src[e0];           // the row's first entry in val[]
+    const int dj = s_diag_src[k] - c0;  // the diagonal's place in the row
+    const int p = s_lpos[k], q = s_to_u[k];
+    const int l0 = l_ptr[p], u0 = u_ptr[q];
+    // the first kPrefetch entries: all loads issued before any store
+    T v[kPrefetch];
+#pragma unroll
+    for (int j = 0; j < kPrefetch; ++j)
+      if (j < len) v[j] = (T)val[c0 + j];
+#pragma unroll
+    for (int j = 0; j < kPrefetch; ++j)
+      if (j < len) values_put(j, v[j], e0, dj, l0, u0, k, p, s_val, s_diag, l_val, u_val, l_diag);
+    for (int j = kPrefetch; j < len; ++j)
+      values_put(j, (T)val[c0 + j], e0, dj, l0, u0, k, p, s_val, s_diag, l_val, u_val, l_diag);
+  }
 }
 
 // out[idx[i]] = in[i]

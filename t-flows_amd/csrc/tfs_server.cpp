@@ -59,12 +59,12 @@ struct Request {
 
 // Device bytes of a resident case before its first solve (then replaced by
 // the exact figure): per entry, col + source index in the S layout, the
-// caller-order and S-order FP64 values, and col + source + value in the L or
-// U triangle (off-diagonal entries); per row, row pointers, maps, diagonal
+// caller-order and S-order FP64 values, and col + value in the L or U
+// triangle (off-diagonal entries); per row, row pointers, maps, diagonal
 // copies, the factor in two orders, two staging vectors and up to 12 work
 // vectors (Cgs); plus the fixed per-matrix buffers.
 long long estimate_bytes(int n, long long nnz) {
-  return 24 * nnz + 16 * (nnz - n) + 200 * (long long)n + (32 << 10);
+  return 24 * nnz + 12 * (nnz - n) + 200 * (long long)n + (32 << 10);
 }
 
 }  // namespace

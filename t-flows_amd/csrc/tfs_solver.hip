@@ -106,7 +106,7 @@ struct DevSweep {
   int nlev = 0;
   std::vector<int> lev_ptr;           // host copy
   std::vector<Segment> segs;
-  int *lev = nullptr, *ptr = nullptr, *col = nullptr, *src = nullptr, *diag_src = nullptr;
+  int *lev = nullptr, *ptr = nullptr, *col = nullptr;
   int *of_orig = nullptr;
   long long ntri = 0;
 };
@@ -142,6 +142,7 @@ struct tfs_matrix {
   int spmv_waves = kSpmvWaves;  // wavefronts per SpMV workgroup
   DevSweep L, U;
   int *l_to_s = nullptr, *u_to_l = nullptr, *s_to_u = nullptr;
+  int *s_lpos = nullptr;  // L position of the row at S position k
   double *val = nullptr;       // caller-order FP64 values
   Values<double> v64;
   Values<float> v32;
@@ -339,11 +340,10 @@ int prepare_values(tfs_matrix *A) {
   Values<T> &V = vals<T>(A);
   if (V.ready) return 0;
   const int n = A->n;
-  gather((int)A->nnz, A->val, A->s_src, V.s_val);
-  gather(n, A->val, A->s_diag_src, V.s_diag);
-  if (A->L.ntri) gather((int)A->L.ntri, A->val, A->L.src, V.l_val);
-  if (A->U.ntri) gather((int)A->U.ntri, A->val, A->U.src, V.u_val);
-  gather(n, A->val, A->L.diag_src, V.l_diag);
+  if (n > 0)
+    hipLaunchKernelGGL(k_values<T>, dim3(vgrid(n)), dim3(kRedThreads), 0, cur_stream(), n,
+                       A->val, A->s_row, A->s_src, A->s_diag_src, A->s_lpos, A->s_to_u,
+                       A->L.ptr, A->U.ptr, V.s_val, V.s_diag, V.l_val, V.u_val, V.l_diag);
   V.ready = true;
   return 0;
 }
@@ -1000,8 +1000,6 @@ int upload_sweep(tfs_matrix *A, const SweepLayout &S, DevSweep &D) {
   TFS_TRY(upload(A, &D.lev, S.lev_ptr));
   TFS_TRY(upload(A, &D.ptr, S.ptr));
   TFS_TRY(upload(A, &D.col, S.col));
-  TFS_TRY(upload(A, &D.src, S.src));
-  TFS_TRY(upload(A, &D.diag_src, S.diag_src));
   TFS_TRY(upload(A, &D.of_orig, S.of_orig));
   return 0;
 }
@@ -1034,6 +1032,11 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
   TFS_TRY(upload(A, &A->l_to_s, Ly.l_to_s));
   TFS_TRY(upload(A, &A->u_to_l, Ly.u_to_l));
   TFS_TRY(upload(A, &A->s_to_u, Ly.s_to_u));
+  {
+    std::vector<int> lpos((size_t)n);
+    for (int k = 0; k < n; ++k) lpos[(size_t)k] = Ly.L.of_orig[(size_t)Ly.s_orig[(size_t)k]];
+    TFS_TRY(upload(A, &A->s_lpos, lpos));
+  }
   const size_t nb = sizeof(double) * (size_t)std::max(n, 1);
   TFS_TRY(dev_alloc(A, (void **)&A->val, sizeof(double) * (size_t)std::max(A->nnz, 1LL)));
   TFS_TRY(dev_alloc(A, (void **)&A->stage_x, nb));
@@ -1060,8 +1063,9 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
   TFS_TRY((occupancy<float, OP_BWD>(&per_cu)));
   // At most kSweepGroups workgroups: every waiting row polls memory, and with
   // the whole chip's worth of pollers the polls delay the stores they wait
-  // for (measured on MI355X: a 382-level sweep takes half the time with 256
-  // to 1024 workgroups as with 2048, a 19-level one 13 % less with 1024).
+  // for (measured on MI355X, 19-level momentum 128^3 sweep: 73 us with 768
+  // workgroups, 75 with 1024, 85 with 1280, 100 with 1536 or more; the
+  // 766-level Poisson 256^3 one: 1.15 ms with 768, 1.18 with 1024).
   const int chunks = (n + kChunk - 1) / kChunk;
   A->grid_persistent =
       std::max(1, std::min({chunks, n_cu * std::max(1, per_cu), kSweepGroups}));
@@ -1072,9 +1076,9 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
     if (std::strcmp(e, "cooperative") == 0) A->coop = true;
     A->sweep_fixed = true;  // "static" (or anything else): the persistent static schedule
   }
-  if (const char *e = std::getenv("TFS_SWEEP_GROUPS")) {  // cap on the persistent grid
+  if (const char *e = std::getenv("TFS_SWEEP_GROUPS")) {  // the cap, up to the occupancy
     const int g = std::atoi(e);
-    if (g > 0) A->grid_persistent = std::min(A->grid_persistent, g);
+    if (g > 0) A->grid_persistent = std::max(1, std::min({chunks, n_cu * std::max(1, per_cu), g}));
   }
   A->bandwidth = Ly.bandwidth;
   // the uploads and fills above ran on the null stream; the matrix stream is
