@@ -33,7 +33,10 @@
  * done) and are not reentrant for the same matrix handle.  Indices cross the
  * ABI as int32 (the reference's default integer); `base` selects 0- or
  * 1-based numbering.  Device pointers are HIP device addresses on the device
- * the library was initialised on.
+ * the library was initialised on.  Each matrix does its work on a stream of
+ * its own that does not wait for other streams: device arrays passed in must
+ * be complete when the call is made (the caller synchronises whatever wrote
+ * them), and are complete when it returns.
  */
 #ifndef TFS_SOLVER_H
 #define TFS_SOLVER_H
