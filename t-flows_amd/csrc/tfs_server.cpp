@@ -11,7 +11,10 @@
 // soon as a worker and the memory for it are free.  Each solve's arithmetic
 // depends only on its own case (per-matrix state, fixed-order reductions), so
 // a result is the same alone and in any mix.  The preconditioner sweeps use
-// the dynamic schedule, which stays deadlock-free when launches share the GPU.
+// the static persistent schedule with each sweep capped at 1/W of the
+// workgroups that fit on the GPU at once: the at most W sweeps in flight
+// (one per worker) then all fit together, so none can wait on a workgroup
+// that another sweep keeps out, and every other kernel terminates by itself.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,6 +28,7 @@
 #include <thread>
 #include <vector>
 
+#include "tfs_internal.hpp"
 #include "tfs_solver.h"
 
 namespace {
@@ -83,6 +87,7 @@ struct tfs_server {
   int next_case = 1;
   bool stop = false;
   std::vector<std::thread> workers;
+  int nworkers = 1;  // set before the worker threads start
   Clock::time_point t0 = Clock::now();
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
@@ -157,7 +162,9 @@ struct tfs_server {
       if (create) {
         rc = tfs_matrix_create(c->n, c->nnz, c->row.data(), c->col.data(), c->dia.data(), c->base,
                                &c->A);
-        if (rc == 0) rc = tfs_matrix_set_sweep_schedule(c->A, TFS_SWEEP_DYNAMIC);
+        if (rc == 0) rc = tfs_matrix_set_sweep_schedule(c->A, TFS_SWEEP_STATIC);
+        if (rc == 0)
+          tfs::set_sweep_grid(c->A, tfs::sweep_capacity(c->A) / nworkers);
         if (rc == 0) rc = tfs_matrix_set_values(c->A, c->val.data());
       }
       if (rc == 0) {
@@ -209,6 +216,7 @@ extern "C" int tfs_server_create(long long hbm_budget_bytes, int workers, tfs_se
   if (rc < 0) return rc;
   auto *S = new tfs_server;
   S->budget = hbm_budget_bytes;
+  S->nworkers = workers;
   for (int w = 0; w < workers; ++w) S->workers.emplace_back([S] { S->worker(); });
   *out = S;
   return TFS_OK;

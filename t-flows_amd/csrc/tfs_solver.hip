@@ -164,6 +164,7 @@ struct tfs_matrix {
   bool sweep_tuned = false;    // schedule picked by timing (first factor)
   unsigned int *sweep_ctr = nullptr;  // the dynamic schedule's counters
   int grid_persistent = 0;
+  int sweep_capacity = 1;      // co-resident workgroups of the heaviest sweep kernel
   long long bytes = 0;
   int bandwidth = 0;
   std::vector<void *> allocs;
@@ -1067,8 +1068,8 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
   // workgroups, 75 with 1024, 85 with 1280, 100 with 1536 or more; the
   // 766-level Poisson 256^3 one: 1.15 ms with 768, 1.18 with 1024).
   const int chunks = (n + kChunk - 1) / kChunk;
-  A->grid_persistent =
-      std::max(1, std::min({chunks, n_cu * std::max(1, per_cu), kSweepGroups}));
+  A->sweep_capacity = std::max(1, n_cu * std::max(1, per_cu));
+  A->grid_persistent = std::max(1, std::min({chunks, A->sweep_capacity, kSweepGroups}));
   A->persistent = per_cu > 0;
   if (const char *e = std::getenv("TFS_SWEEP")) {
     if (std::strcmp(e, "levels") == 0) A->persistent = false;
@@ -1220,6 +1221,12 @@ extern "C" int tfs_matrix_values_changed(tfs_matrix *A) {
 extern "C" double *tfs_matrix_values_device(tfs_matrix *A) { return A ? A->val : nullptr; }
 
 hipStream_t tfs::matrix_stream(const tfs_matrix *A) { return A ? A->stream : nullptr; }
+int tfs::sweep_capacity(const tfs_matrix *A) { return A ? A->sweep_capacity : 1; }
+void tfs::set_sweep_grid(tfs_matrix *A, int groups) {
+  if (!A) return;
+  A->grid_persistent = std::max(1, std::min(A->grid_persistent, groups));
+  if (A->inner) set_sweep_grid(A->inner, groups);
+}
 
 extern "C" int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info) {
   if (!A || !info) return TFS_ERR_INVALID;
