@@ -599,7 +599,7 @@ def main():
     ap.add_argument("--cases", type=int, default=24)
     ap.add_argument("--visits", type=int, default=10)
     ap.add_argument("--rate", type=float, default=200.0)
-    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB, 5 times (the "
