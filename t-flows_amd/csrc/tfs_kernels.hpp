@@ -623,20 +623,45 @@ enum SweepOp { OP_FACTOR = 0, OP_FWD, OP_BWD };
 template <typename T>
 __device__ __forceinline__ T tiny() { return (T)kTiny; }
 
+// One row of a level (all its dependencies are in earlier levels).  Three
+// memory rounds: the row pointers and what the position indexes; the first
+// kPrefetch entries and the right-hand side; their dependencies, all at
+// once.  The terms are then folded in entry order (entries beyond the
+// prefetch one by one), as the reference sums them.
 template <typename T, int OP>
 __device__ __forceinline__ void sweep_row(const Tri<T> &Tr, T *d, const T *b, T *x, int p) {
+  T *out = OP == OP_FACTOR ? d : x;
   const int e0 = Tr.ptr[p], e1 = Tr.ptr[p + 1];
+  T s, dp = T(0);
+  int src = 0;
   if (OP == OP_FACTOR) {
-    T s = Tr.diag[p];
-    for (int e = e0; e < e1; ++e) s = s - d[Tr.col[e]] * Tr.val[e] * Tr.val[e];
-    d[p] = T(1) / s;
+    s = Tr.diag[p];
   } else {
-    const T dp = d[p];
-    T s = b[Tr.in_idx[p]];
-    if (OP == OP_BWD) s = s / (dp + tiny<T>());
-    for (int e = e0; e < e1; ++e) s = s - Tr.val[e] * x[Tr.col[e]];
-    x[p] = s * dp;
+    dp = d[p];
+    src = Tr.in_idx[p];
   }
+  int c[kPrefetch];
+  T a[kPrefetch], v[kPrefetch];
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k) {
+    if (e0 + k < e1) {
+      c[k] = Tr.col[e0 + k];
+      a[k] = Tr.val[e0 + k];
+    }
+  }
+  if (OP != OP_FACTOR) s = b[src];
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k)
+    if (e0 + k < e1) v[k] = out[c[k]];
+  if (OP == OP_BWD) s = s / (dp + tiny<T>());
+#pragma unroll
+  for (int k = 0; k < kPrefetch; ++k)
+    if (e0 + k < e1) s = OP == OP_FACTOR ? s - v[k] * a[k] * a[k] : s - a[k] * v[k];
+  for (int e = e0 + kPrefetch; e < e1; ++e) {
+    const T w = out[Tr.col[e]], av = Tr.val[e];
+    s = OP == OP_FACTOR ? s - w * av * av : s - av * w;
+  }
+  out[p] = OP == OP_FACTOR ? T(1) / s : s * dp;
 }
 
 template <typename T, int OP>
