@@ -1,6 +1,7 @@
 /*
  * tfs_mesh.h -- host-side C ABI of libtfs_mesh.so: CRS topology from a face
- * list, and the synthetic hex-mesh cases the solver is measured on.
+ * list, the synthetic hex-mesh cases the solver is measured on, and the
+ * reference's own mesh files (.cns / .geo) with the pressure matrix on them.
  *
  * The topology builder is the counterpart of the reference's
  * Matrix_Mod_Topology (Sources/Process/Matrix_Mod/Topology.f90:2-127): same
@@ -104,6 +105,54 @@ int tfs_layout_block_colour(int n, const int *row, const int *col, int block, in
  * pattern in S order. */
 int tfs_layout_summary(int n, const int *row, const int *col, const int *dia, int *s_orig,
                        int *level_lower, int *level_upper, int *bandwidth);
+
+/* ---- T-FlowS mesh files -------------------------------------------------
+ * The .cns (connectivity) and .geo (geometry) files the reference's mesh
+ * programs write (Sources/Shared/Save_Cns_Geo.f90) and its Process reads:
+ * Fortran unformatted sequential records, 4-byte integers, 8-byte reals.
+ * Cells are 1..n_cells; boundary cells -1..-n_bnd_cells.  Status: */
+enum {
+  TFS_MESH_OK = 0,
+  TFS_MESH_ERR_INVALID = -1, /* bad argument or inconsistent indices        */
+  TFS_MESH_ERR_IO = -2,      /* the file cannot be opened                   */
+  TFS_MESH_ERR_FORMAT = -3   /* a record is missing or of the wrong length  */
+};
+/* Reads a .cns file (replaces Sources/Process/Load_Cns.f90:25-69).  Always
+ * returns the sizes; with faces_c non-NULL also the face list
+ * faces_c[2*n_faces] as (c1, c2) pairs -- the reference's faces_c(2,n_faces)
+ * in column-major order, c2 negative for boundary faces -- and with material
+ * non-NULL the material of the cells, then of the boundary cells
+ * (n_cells + n_bnd_cells entries). */
+int tfs_mesh_read_cns(const char *path, int *n_cells, int *n_bnd_cells, int *n_faces,
+                      int *faces_c, int *material);
+/* Reads a .geo file (replaces Sources/Shared/Load_Geo.f90:23-50) for the sizes
+ * of its .cns.  Outputs (each may be NULL), x, y and z stored one block after
+ * the other:
+ *   xc    3*(n_cells+n_bnd_cells)  cell centres: cells, then boundary cells
+ *                                  -1..-n_bnd (the boundary face centres)
+ *   vol   n_cells                  cell volumes
+ *   s     3*n_faces                face surface vectors (pointing c1 -> c2)
+ *   shift 3*n_faces                periodic shift of c2 (zero off periodic faces)
+ *   f     n_faces                  interpolation weights
+ *   xf    3*n_faces                face centres */
+int tfs_mesh_read_geo(const char *path, int n_cells, int n_bnd_cells, int n_faces, double *xc,
+                      double *vol, double *s, double *shift, double *f, double *xf);
+/* Face coefficients of Sources/Process/Compute_Geometry.f90:22-49:
+ * scoef[s] = |S|^2 / (d . S), d = centre of c2 (+ shift) - centre of c1;
+ * d (3*n_faces, may be NULL) receives the distance vectors. */
+int tfs_mesh_face_coef(int n_cells, int n_bnd_cells, int n_faces, const int *faces_c,
+                       const double *xc, const double *s, const double *shift, double *d,
+                       double *scoef);
+/* The fractional-step pressure matrix values of
+ * Sources/Process/Pressure_Matrix_Fractional.f90:19-48 on a topology from
+ * tfs_topology_build (pos, dia with index base `base`): val[nnz] := 0; each
+ * interior face sets its two off-diagonals to -dt scoef and adds dt scoef to
+ * both diagonals, in face order; a boundary face adds dt scoef to its cell's
+ * diagonal where bnd_on[s] is set (NULL: nowhere -- walls and symmetry, the
+ * sequential case). */
+int tfs_mesh_pressure_values(int n_cells, int64_t n_faces, const int *faces_c, const int *pos,
+                             const int *dia, int base, int64_t nnz, const double *scoef,
+                             double dt, const unsigned char *bnd_on, double *val);
 
 #ifdef __cplusplus
 }

@@ -160,6 +160,13 @@ class Reference:
         lib.tfs_ref_pressure_matrix.restype = ctypes.c_int
         lib.tfs_ref_pressure_matrix.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _I32P,
                                                 ctypes.c_double, _F64P, ctypes.c_int, _F64P]
+        if hasattr(lib, "tfs_ref_mesh_load"):
+            lib.tfs_ref_mesh_load.restype = ctypes.c_int
+            lib.tfs_ref_mesh_load.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
+                                              _I32P]
+            lib.tfs_ref_mesh_get.restype = ctypes.c_int
+            lib.tfs_ref_mesh_get.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                             ctypes.c_void_p]
         self.lib = lib
         self.n = 0
 
@@ -218,3 +225,40 @@ class Reference:
         nnz = self.lib.tfs_ref_pressure_matrix(n_cells, n_bnd, nf, faces_c, float(dt),
                                                _c64(scoef), cap, val)
         return val[:nnz]
+
+    # what tfs_ref_mesh_get returns, by code: (name, integer array?)
+    MESH_ARRAYS = {1: ("faces_c", True), 2: ("row", True), 3: ("col", True), 4: ("dia", True),
+                   5: ("xc", False), 6: ("vol", False), 7: ("s", False), 8: ("shift", False),
+                   9: ("f", False), 10: ("xf", False), 11: ("d", False), 12: ("scoef", False),
+                   13: ("val", False)}
+
+    def mesh(self, stem, dt=1.0):
+        """The reference's Load_Cns + Load_Geo on <stem>.cns / <stem>.geo,
+        Compute_Geometry (every boundary a wall), Matrix_Mod_Topology and
+        Pressure_Matrix_Fractional with time step dt.  Returns dict of the
+        arrays listed in MESH_ARRAYS (row/col/dia 1-based, as the reference
+        holds them) plus the sizes n_cells, n_bnd_cells, n_faces, nnz.  The
+        reference reads from the working directory, so the call runs there."""
+        import os
+        stem = pathlib.Path(stem).resolve()
+        sizes = np.zeros(4, np.int32)
+        here = os.getcwd()
+        os.chdir(stem.parent)
+        try:
+            name = stem.name.encode()
+            rc = self.lib.tfs_ref_mesh_load(name, len(name), float(dt), sizes)
+        finally:
+            os.chdir(here)
+        if rc != 0:
+            raise RuntimeError(f"tfs_ref_mesh_load({stem}) returned {rc}")
+        nc, nb, nf, nnz = (int(v) for v in sizes)
+        cap = max(nc + nb, nf, nnz, nc + 1) * 3
+        out = dict(n_cells=nc, n_bnd_cells=nb, n_faces=nf, nnz=nnz)
+        ibuf = np.zeros(cap, np.int32)
+        rbuf = np.zeros(cap, np.float64)
+        for code, (key, is_int) in self.MESH_ARRAYS.items():
+            k = self.lib.tfs_ref_mesh_get(code, cap, ibuf.ctypes.data, rbuf.ctypes.data)
+            if k < 0:
+                raise RuntimeError(f"tfs_ref_mesh_get({code}) returned {k}")
+            out[key] = (ibuf if is_int else rbuf)[:k].copy()
+        return out

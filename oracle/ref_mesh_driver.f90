@@ -1,0 +1,178 @@
+!==============================================================================!
+!   ref_mesh_driver.f90 -- C-callable driver around the REFERENCE mesh        !
+!   readers and the geometry / pressure-matrix routines that use them:        !
+!     Load_Cns          Sources/Process/Load_Cns.f90:25-69                    !
+!     Load_Geo          Sources/Shared/Load_Geo.f90:23-50                     !
+!     Compute_Geometry  Sources/Process/Compute_Geometry.f90:22-49 (Scoef)    !
+!     Matrix_Mod_Topology, Pressure_Matrix_Fractional                         !
+!                                                                              !
+!   TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles it with the           !
+!   reference's own sources where they lie, into oracle/_ref/libtfs_ref.so;   !
+!   tests use it to pin the product's .cns/.geo readers, its face            !
+!   coefficients and its pressure-matrix values bitwise on a real T-FlowS     !
+!   mesh.  The product never loads it.                                        !
+!                                                                              !
+!   tfs_ref_mesh_load reads <name>.cns and <name>.geo from the working        !
+!   directory, keeps copies of what was read, runs Compute_Geometry, builds   !
+!   A with Matrix_Mod_Topology and assembles the fractional-step pressure     !
+!   matrix with every boundary a wall (the sequential build, no BUFFER        !
+!   faces).  tfs_ref_mesh_get copies one of the kept arrays out.              !
+!==============================================================================!
+  module tfs_ref_mesh_driver
+  use iso_c_binding
+  use all_mod
+  use pro_mod
+  use Grid_Mod
+  use Matrix_Mod
+  implicit none
+
+  integer :: m_nc = 0, m_nb = 0, m_nf = 0, m_nnz = 0
+  integer, allocatable :: m_faces(:), m_row(:), m_col(:), m_dia(:)
+  real,    allocatable :: m_xc(:), m_vol(:), m_s(:), m_shift(:), m_f(:), m_xf(:)
+  real,    allocatable :: m_d(:), m_scoef(:), m_val(:)
+
+  contains
+
+  !---------------------------------------------------------------------------!
+  !   sizes(4) <- n_cells, n_bnd_cells, n_faces, nnz;  returns 0 on success.  !
+  !---------------------------------------------------------------------------!
+  integer(c_int) function tfs_ref_mesh_load(cname, nlen, dt_in, sizes)      &
+                 bind(C, name='tfs_ref_mesh_load')
+  character(kind=c_char), intent(in) :: cname(*)
+  integer(c_int), value              :: nlen
+  real(c_double), value              :: dt_in
+  integer(c_int), intent(out)        :: sizes(4)
+  type(Grid_Type), allocatable, target :: g
+  integer :: k, c, s, nc, nb, nf
+
+  name = ' '
+  do k = 1, nlen
+    name(k:k) = cname(k)
+  end do
+
+  if(allocated(material)) deallocate(material)
+  if(allocated(TypeBC))   deallocate(TypeBC)
+  if(allocated(bcmark))   deallocate(bcmark)
+  if(allocated(CopyC))    deallocate(CopyC)
+  allocate(g)
+  call Load_Cns(g, 0)
+  nc = g % n_cells;  nb = g % n_bnd_cells;  nf = g % n_faces
+
+  if(allocated(delta))  deallocate(delta)
+  if(allocated(WallDs)) deallocate(WallDs)
+  allocate(delta(-nb:nc));  delta = 0.0
+  allocate(WallDs(-nb:nc)); WallDs = 0.0
+  call Load_Geo(g, 0)
+
+  ! what was read: faces, cell centres (cells, then boundary cells -1..-nb),
+  ! volumes, face surfaces, the periodic shifts, weights, face centres
+  if(allocated(m_faces)) deallocate(m_faces, m_xc, m_vol, m_s, m_shift, m_f, m_xf)
+  allocate(m_faces(2*nf), m_xc(3*(nc+nb)), m_vol(nc), m_s(3*nf), m_shift(3*nf),  &
+           m_f(nf), m_xf(3*nf))
+  do s = 1, nf
+    m_faces(2*s-1) = g % faces_c(1,s)
+    m_faces(2*s)   = g % faces_c(2,s)
+  end do
+  do c = 1, nc
+    m_xc(c)            = g % xc(c)
+    m_xc(nc+nb+c)      = g % yc(c)
+    m_xc(2*(nc+nb)+c)  = g % zc(c)
+    m_vol(c)           = g % vol(c)
+  end do
+  do c = 1, nb
+    m_xc(nc+c)            = g % xc(-c)
+    m_xc(nc+nb+nc+c)      = g % yc(-c)
+    m_xc(2*(nc+nb)+nc+c)  = g % zc(-c)
+  end do
+  do s = 1, nf
+    m_s(s)        = g % sx(s);  m_s(nf+s)      = g % sy(s);  m_s(2*nf+s)      = g % sz(s)
+    m_shift(s)    = g % dx(s);  m_shift(nf+s)  = g % dy(s);  m_shift(2*nf+s)  = g % dz(s)
+    m_f(s)        = g % f(s)
+    m_xf(s)       = g % xf(s);  m_xf(nf+s)     = g % yf(s);  m_xf(2*nf+s)     = g % zf(s)
+  end do
+
+  ! Compute_Geometry's inputs beyond the grid: every material a fluid,
+  ! every boundary a wall
+  if(allocated(Scoef))      deallocate(Scoef)
+  if(allocated(fF))         deallocate(fF)
+  if(allocated(IsNearWall)) deallocate(IsNearWall)
+  if(allocated(xp))         deallocate(xp, yp, zp)
+  allocate(Scoef(nf), fF(nf), IsNearWall(-nb:nc))
+  allocate(xp(g % n_materials), yp(g % n_materials), zp(g % n_materials))
+  xp = 0.0;  yp = 0.0;  zp = 0.0
+  StateMat = FLUID
+  TypeBC = WALL
+  call Compute_Geometry(g)
+
+  if(allocated(m_d)) deallocate(m_d, m_scoef)
+  allocate(m_d(3*nf), m_scoef(nf))
+  do s = 1, nf
+    m_d(s) = g % dx(s);  m_d(nf+s) = g % dy(s);  m_d(2*nf+s) = g % dz(s)
+    m_scoef(s) = Scoef(s)
+  end do
+
+  if(allocated(A % row)) deallocate(A % row)
+  if(allocated(A % col)) deallocate(A % col)
+  if(allocated(A % dia)) deallocate(A % dia)
+  if(allocated(A % val)) deallocate(A % val)
+  if(allocated(A % mir)) deallocate(A % mir)
+  if(allocated(A % pos)) deallocate(A % pos)
+  if(allocated(A % sav)) deallocate(A % sav)
+  if(allocated(A % bou)) deallocate(A % bou)
+  call Matrix_Mod_Allocate(g, A)
+  call Matrix_Mod_Topology(g, A)
+  dt = dt_in
+  call Pressure_Matrix_Fractional(g)
+
+  m_nnz = A % nonzeros - 1
+  if(allocated(m_row)) deallocate(m_row, m_col, m_dia, m_val)
+  allocate(m_row(nc+1), m_col(m_nnz), m_dia(nc), m_val(m_nnz))
+  m_row = A % row(1:nc+1)
+  m_col = A % col(1:m_nnz)
+  m_dia = A % dia(1:nc)
+  m_val = A % val(1:m_nnz)
+
+  m_nc = nc;  m_nb = nb;  m_nf = nf
+  sizes(1) = nc;  sizes(2) = nb;  sizes(3) = nf;  sizes(4) = m_nnz
+  deallocate(g)
+  tfs_ref_mesh_load = 0
+  end function
+
+  !---------------------------------------------------------------------------!
+  !   which: 1 faces_c (2*nf, int)   2 row (nc+1, int)   3 col (nnz, int)      !
+  !          4 dia (nc, int)         5 cell centres x..,y..,z.. (3*(nc+nb))    !
+  !          6 vol (nc)              7 surfaces (3*nf)   8 shifts as read      !
+  !          9 f (nf)               10 face centres (3*nf)                     !
+  !         11 distances after Compute_Geometry (3*nf)  12 Scoef (nf)          !
+  !         13 val (nnz)                                                        !
+  !   Integer arrays go to iout, real ones to rout; returns the count copied,  !
+  !   -1 if cap is too small or which is unknown.                              !
+  !---------------------------------------------------------------------------!
+  integer(c_int) function tfs_ref_mesh_get(which, cap, iout, rout)          &
+                 bind(C, name='tfs_ref_mesh_get')
+  integer(c_int), value        :: which, cap
+  integer(c_int), intent(out)  :: iout(*)
+  real(c_double), intent(out)  :: rout(*)
+  integer :: n
+
+  tfs_ref_mesh_get = -1
+  select case(which)
+    case(1);  n = size(m_faces); if(n > cap) return; iout(1:n) = m_faces
+    case(2);  n = size(m_row);   if(n > cap) return; iout(1:n) = m_row
+    case(3);  n = size(m_col);   if(n > cap) return; iout(1:n) = m_col
+    case(4);  n = size(m_dia);   if(n > cap) return; iout(1:n) = m_dia
+    case(5);  n = size(m_xc);    if(n > cap) return; rout(1:n) = m_xc
+    case(6);  n = size(m_vol);   if(n > cap) return; rout(1:n) = m_vol
+    case(7);  n = size(m_s);     if(n > cap) return; rout(1:n) = m_s
+    case(8);  n = size(m_shift); if(n > cap) return; rout(1:n) = m_shift
+    case(9);  n = size(m_f);     if(n > cap) return; rout(1:n) = m_f
+    case(10); n = size(m_xf);    if(n > cap) return; rout(1:n) = m_xf
+    case(11); n = size(m_d);     if(n > cap) return; rout(1:n) = m_d
+    case(12); n = size(m_scoef); if(n > cap) return; rout(1:n) = m_scoef
+    case(13); n = size(m_val);   if(n > cap) return; rout(1:n) = m_val
+    case default; return
+  end select
+  tfs_ref_mesh_get = n
+  end function
+
+  end module

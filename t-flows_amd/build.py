@@ -38,7 +38,7 @@ def _run(cmd, cwd=None):
 def build_mesh():
     out = PKG / "libtfs_mesh.so"
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-Wall",
-          f"-I{INCLUDE}", f"-I{CSRC}", "-o", out, CSRC / "tfs_mesh.cpp", CSRC / "tfs_layout.cpp",
+          f"-I{INCLUDE}", f"-I{CSRC}", "-o", out, CSRC / "tfs_mesh.cpp", CSRC / "tfs_meshio.cpp", CSRC / "tfs_layout.cpp",
           CSRC / "tfs_layout_capi.cpp"])
     return out
 

@@ -12,6 +12,10 @@ matrices follow the reference's assembly formulas:
   ``A12 = A0 - min(F,0)``, ``A21 = A0 + max(F,0)`` and an inertia diagonal
   ``V/dt`` (Sources/Process/NewUVW.f90:349-384,500-506), with face fluxes from a
   seeded divergence-free field.
+
+``mesh_case`` builds the same Poisson matrix on a real T-FlowS mesh read from
+its ``.cns`` / ``.geo`` files (``read_cns``, ``read_geo``, ``face_coef``,
+``pressure_values``).
 """
 import ctypes
 from dataclasses import dataclass, field
@@ -76,6 +80,19 @@ def _lib():
         lib.tfs_layout_block_colour.restype = ctypes.c_int
         lib.tfs_layout_block_colour.argtypes = [ctypes.c_int, _I32P, _I32P, ctypes.c_int, _I32P,
                                                 _I32P]
+        lib.tfs_mesh_read_cns.restype = ctypes.c_int
+        lib.tfs_mesh_read_cns.argtypes = [ctypes.c_char_p] + [ctypes.POINTER(ctypes.c_int)] * 3 + \
+            [ctypes.c_void_p, ctypes.c_void_p]
+        lib.tfs_mesh_read_geo.restype = ctypes.c_int
+        lib.tfs_mesh_read_geo.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int] + [ctypes.c_void_p] * 6
+        lib.tfs_mesh_face_coef.restype = ctypes.c_int
+        lib.tfs_mesh_face_coef.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _I32P, _F64P,
+                                           _F64P, _F64P, ctypes.c_void_p, _F64P]
+        lib.tfs_mesh_pressure_values.restype = ctypes.c_int
+        lib.tfs_mesh_pressure_values.argtypes = [ctypes.c_int, ctypes.c_int64, _I32P, _I32P,
+                                                 _I32P, ctypes.c_int, ctypes.c_int64, _F64P,
+                                                 ctypes.c_double, ctypes.c_void_p, _F64P]
         lib.tfs_layout_summary.restype = ctypes.c_int
         lib.tfs_layout_summary.argtypes = [ctypes.c_int, _I32P, _I32P, _I32P, _I32P, _I32P,
                                            _I32P, ctypes.POINTER(ctypes.c_int)]
@@ -205,3 +222,103 @@ def layout_summary(row, col, dia):
     if rc != 0:
         raise ValueError("tfs_layout_summary failed")
     return dict(s_orig=s_orig, level_lower=ll, level_upper=lu, bandwidth=bw.value)
+
+
+# ---- T-FlowS mesh files ------------------------------------------------------
+MESH_ERRORS = {-1: "invalid argument", -2: "cannot open the file", -3: "bad record"}
+TAG_MESH_RHS = 0x4D455348   # stream of the right-hand side of mesh_case
+
+
+def _mesh_check(rc, what):
+    if rc != 0:
+        raise ValueError(f"{what}: {MESH_ERRORS.get(rc, rc)}")
+
+
+def read_cns(path):
+    """Connectivity of a T-FlowS .cns file (Sources/Process/Load_Cns.f90:25-69):
+    dict(n_cells, n_bnd_cells, n_faces, faces_c, material).  faces_c is the
+    reference's faces_c(2, n_faces) flattened column-major (c1, c2 per face),
+    1-based, c2 = -1..-n_bnd_cells for boundary faces; material lists the
+    cells, then the boundary cells."""
+    lib = _lib()
+    nc, nb, nf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    p = str(path).encode()
+    _mesh_check(lib.tfs_mesh_read_cns(p, ctypes.byref(nc), ctypes.byref(nb), ctypes.byref(nf),
+                                      None, None), f"read_cns({path})")
+    faces = np.empty(2 * nf.value, np.int32)
+    material = np.empty(nc.value + nb.value, np.int32)
+    _mesh_check(lib.tfs_mesh_read_cns(p, ctypes.byref(nc), ctypes.byref(nb), ctypes.byref(nf),
+                                      faces.ctypes.data, material.ctypes.data),
+                f"read_cns({path})")
+    return dict(n_cells=nc.value, n_bnd_cells=nb.value, n_faces=nf.value, faces_c=faces,
+                material=material)
+
+
+def read_geo(path, cns):
+    """Geometry of a .geo file for its .cns (Sources/Shared/Load_Geo.f90:23-50):
+    dict(xc (3, n_cells + n_bnd_cells) -- cells, then boundary cells --, vol,
+    s (3, n_faces) surface vectors, shift (3, n_faces) periodic shifts, f
+    weights, xf (3, n_faces) face centres)."""
+    nc, nb, nf = cns["n_cells"], cns["n_bnd_cells"], cns["n_faces"]
+    out = dict(xc=np.empty((3, nc + nb)), vol=np.empty(nc), s=np.empty((3, nf)),
+               shift=np.empty((3, nf)), f=np.empty(nf), xf=np.empty((3, nf)))
+    _mesh_check(_lib().tfs_mesh_read_geo(str(path).encode(), nc, nb, nf,
+                                         *(out[k].ctypes.data for k in
+                                           ("xc", "vol", "s", "shift", "f", "xf"))),
+                f"read_geo({path})")
+    return out
+
+
+def face_coef(cns, geo):
+    """(scoef, d): Scoef = |S|^2 / (d . S) per face and the centre-to-centre
+    vectors d (3, n_faces), Sources/Process/Compute_Geometry.f90:22-49."""
+    nf = cns["n_faces"]
+    scoef = np.empty(nf)
+    d = np.empty((3, nf))
+    _mesh_check(_lib().tfs_mesh_face_coef(cns["n_cells"], cns["n_bnd_cells"], nf, cns["faces_c"],
+                                          geo["xc"].reshape(-1), geo["s"].reshape(-1),
+                                          geo["shift"].reshape(-1), d.ctypes.data, scoef),
+                "face_coef")
+    return scoef, d
+
+
+def pressure_values(n_cells, faces_c, pos, dia, nnz, scoef, dt=1.0, bnd_on=None, base=0):
+    """Values of the fractional-step pressure matrix on a topology from
+    ``topology(..., with_pos=True)``: Pressure_Matrix_Fractional.f90:19-48."""
+    faces_c = np.ascontiguousarray(faces_c, np.int32)
+    val = np.empty(int(nnz))
+    bnd = None if bnd_on is None else np.ascontiguousarray(bnd_on, np.uint8)
+    _mesh_check(_lib().tfs_mesh_pressure_values(
+        int(n_cells), faces_c.size // 2, faces_c, np.ascontiguousarray(pos, np.int32),
+        np.ascontiguousarray(dia, np.int32), int(base), int(nnz),
+        np.ascontiguousarray(scoef, np.float64), float(dt),
+        None if bnd is None else bnd.ctypes.data, val), "pressure_values")
+    return val
+
+
+def mesh_case(stem, dt=1.0, pin_first=True, seed=20261202, base=0):
+    """The fractional-step pressure system on the T-FlowS mesh <stem>.cns /
+    <stem>.geo: topology as Matrix_Mod_Topology builds it, values as
+    Pressure_Matrix_Fractional forms them (every boundary a wall), cell 1
+    pinned by doubling a_11 when pin_first, and a seeded zero-mean uniform
+    right-hand side.  face_coefs holds (a12 = a21 = dt Scoef, no cell term)
+    for the device assembly."""
+    stem = str(stem)
+    cns = read_cns(stem + ".cns")
+    geo = read_geo(stem + ".geo", cns)
+    scoef, _ = face_coef(cns, geo)
+    n, faces = cns["n_cells"], cns["faces_c"]
+    row, col, dia, pos = topology(n, faces, base=0, with_pos=True)
+    val = pressure_values(n, faces, pos, dia, len(col), scoef, dt)
+    if pin_first:
+        val[dia[0]] *= 2.0
+    lib = _lib()
+    b = np.array([2 * lib.tfs_rng_u01(seed, TAG_MESH_RHS, i) - 1 for i in range(n)])
+    b -= b.mean()
+    if base:
+        row, col, dia = row + base, col + base, dia + base
+    a = dt * scoef
+    return Case(n=n, row=row, col=col, dia=dia, val=val, b=b, base=base, dt=dt, faces=faces,
+                face_coefs=(a, a.copy(), np.zeros(n)),
+                params=dict(mesh=stem, dt=dt, pin_first=pin_first, seed=seed,
+                            n_bnd_cells=cns["n_bnd_cells"], n_faces=cns["n_faces"]))

@@ -1,0 +1,87 @@
+"""Writes tests/golden/mesh/: real T-FlowS meshes and what the REFERENCE makes
+of them.
+
+Run in a container that has the reference tree.  ``make -C oracle ref mesh``
+compiles the reference's mesh generator and its solver path from their own
+sources and runs the generator on its Tests/ domain files (cavity, and the
+non-orthogonal backstep).  This script then keeps, per mesh:
+
+  * <name>.cns.gz / <name>.geo.gz  -- the generator's output files, as data;
+  * in mesh_reference.json: the sizes and the SHA-256 of every array the
+    reference's Load_Cns / Load_Geo read, of Compute_Geometry's distances and
+    Scoef, and of Matrix_Mod_Topology's row/col/dia and
+    Pressure_Matrix_Fractional's val at time step DT (little-endian int32 /
+    float64 bytes, 1-based indices as the reference holds them);
+  * in <name>_cg_p2.npz: the reference's Cg (prec 2) on that pressure matrix
+    with cell 1 pinned (a_11 doubled) and the seeded right-hand side of
+    tflows_amd.mesh.mesh_case, to tol 1e-10 ||b||: x, niter, ini_res, fin_res.
+
+    python tests/golden/make_mesh_golden.py
+"""
+import gzip
+import hashlib
+import json
+import pathlib
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+from conftest import load_package  # noqa: E402
+
+load_package()
+from tflows_amd import mesh  # noqa: E402
+from oracle.oracle import Reference  # noqa: E402
+
+OUT = HERE / "mesh"
+GEN = ROOT / "oracle" / "_ref" / "gen"
+MESHES = ["cavity", "backstep_nonorthogonal"]
+DT = 0.013
+SEED = 20261202
+
+
+def sha(a):
+    a = np.ascontiguousarray(a)
+    a = a.astype(a.dtype.newbyteorder("<"), copy=False)
+    return hashlib.sha256(a.tobytes()).hexdigest()
+
+
+def main():
+    subprocess.run(["make", "-C", str(ROOT / "oracle"), "ref", "mesh"], check=True)
+    OUT.mkdir(exist_ok=True)
+    ref = Reference()
+    meta = {"dt": DT, "meshes": {}}
+    for name in MESHES:
+        stem = GEN / name / name
+        for ext in (".cns", ".geo"):
+            with open(str(stem) + ext, "rb") as src, \
+                    gzip.GzipFile(OUT / (name + ext + ".gz"), "wb", 9, mtime=0) as dst:
+                shutil.copyfileobj(src, dst)
+        r = ref.mesh(stem, dt=DT)
+        entry = {k: r[k] for k in ("n_cells", "n_bnd_cells", "n_faces", "nnz")}
+        entry["sha256"] = {key: sha(r[key]) for key, _ in Reference.MESH_ARRAYS.values()}
+        meta["meshes"][name] = entry
+
+        # the pinned pressure system, solved by the reference
+        row, col, dia = r["row"] - 1, r["col"] - 1, r["dia"] - 1
+        val = r["val"].copy()
+        val[dia[0]] *= 2.0
+        c = mesh.mesh_case(stem, dt=DT, seed=SEED)
+        assert np.array_equal(c.val, val) and np.array_equal(c.col, col)
+        tol = 1e-10 * float(np.linalg.norm(c.b))
+        ref.set_matrix(row, col, dia, val)
+        out = ref.solve("cg", c.b, prec=2, niter=20000, tol=tol)
+        np.savez_compressed(OUT / f"{name}_cg_p2.npz", x=out["x"], niter=out["niter"],
+                            ini_res=out["ini_res"], fin_res=out["fin_res"], tol=tol,
+                            b_sha256=sha(c.b), seed=SEED, dt=DT)
+        print(name, entry["n_cells"], "cells; Cg prec 2:", out["niter"], "iterations")
+    (OUT / "mesh_reference.json").write_text(json.dumps(meta, indent=1) + "\n")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,246 @@
+"""T-FlowS mesh ingestion (libtfs_mesh.so: tfs_mesh_read_cns / read_geo /
+face_coef / pressure_values, and the topology builder on the meshes read),
+pinned on the CPU:
+
+  * bitwise against the reference's own Load_Cns, Load_Geo, Compute_Geometry,
+    Matrix_Mod_Topology and Pressure_Matrix_Fractional: through the SHA-256
+    of every array they produce on the committed fixture meshes (written by
+    tests/golden/make_mesh_golden.py), and live through oracle/_ref where it
+    was built, also on the larger refined and periodic meshes made there;
+  * against what geometry fixes: every cell is closed (its outward surface
+    vectors sum to zero), the divergence theorem gives back every volume
+    from the face centres and surfaces (with the periodic shifts), and on
+    the cavity mesh the matrix annihilates a coordinate field in every cell
+    whose faces are all orthogonal, interior and not periodic (Scoef =
+    |S| / |d| there, so the row is a sum of outward surface components);
+  * the matrix is symmetric with zero row sums before pinning;
+  * broken and mismatched files fail with the documented error codes.
+
+The fixture meshes are the output files of the reference's mesh generator
+(compiled from its sources) on its cavity and non-orthogonal backstep
+domain files.
+"""
+import gzip
+import hashlib
+import json
+import pathlib
+import shutil
+
+import numpy as np
+import pytest
+
+from tflows_amd import mesh
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden" / "mesh"
+GEN = pathlib.Path(__file__).resolve().parent.parent / "oracle" / "_ref" / "gen"
+META = json.loads((GOLDEN / "mesh_reference.json").read_text())
+DT = META["dt"]
+FIXTURES = list(META["meshes"])
+LARGE = ["cylinder", "chan"]
+
+
+def sha(a):
+    a = np.ascontiguousarray(a)
+    return hashlib.sha256(a.astype(a.dtype.newbyteorder("<"), copy=False).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="session")
+def mesh_dir(tmp_path_factory):
+    d = tmp_path_factory.mktemp("meshes")
+    for name in FIXTURES:
+        for ext in (".cns", ".geo"):
+            with gzip.open(GOLDEN / (name + ext + ".gz"), "rb") as src, \
+                    open(d / (name + ext), "wb") as dst:
+                shutil.copyfileobj(src, dst)
+    return d
+
+
+def ingest(stem):
+    """Everything the product makes of a mesh, keyed like Reference.mesh."""
+    stem = str(stem)
+    cns = mesh.read_cns(stem + ".cns")
+    geo = mesh.read_geo(stem + ".geo", cns)
+    scoef, d = mesh.face_coef(cns, geo)
+    n = cns["n_cells"]
+    row, col, dia, pos = mesh.topology(n, cns["faces_c"], base=1, with_pos=True)
+    val = mesh.pressure_values(n, cns["faces_c"], pos, dia, len(col), scoef, DT, base=1)
+    return dict(n_cells=n, n_bnd_cells=cns["n_bnd_cells"], n_faces=cns["n_faces"], nnz=len(col),
+                faces_c=cns["faces_c"], row=row, col=col, dia=dia, xc=geo["xc"].ravel(),
+                vol=geo["vol"], s=geo["s"].ravel(), shift=geo["shift"].ravel(), f=geo["f"],
+                xf=geo["xf"].ravel(), d=d.ravel(), scoef=scoef, val=val, pos=pos, cns=cns,
+                geo=geo)
+
+
+_cache = {}
+
+
+def get(mesh_dir, name):
+    if name not in _cache:
+        _cache[name] = ingest(mesh_dir / name)
+    return _cache[name]
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_matches_reference_digests(mesh_dir, name):
+    got = get(mesh_dir, name)
+    want = META["meshes"][name]
+    for k in ("n_cells", "n_bnd_cells", "n_faces", "nnz"):
+        assert got[k] == want[k], k
+    bad = [k for k, h in want["sha256"].items() if sha(got[k]) != h]
+    assert not bad, f"arrays differing from the reference: {bad}"
+
+
+def _live_meshes():
+    out = [("fixture", n) for n in FIXTURES]
+    out += [("generated", n) for n in LARGE]
+    return out
+
+
+@pytest.mark.parametrize("where,name", _live_meshes())
+def test_matches_reference_live(reference, mesh_dir, where, name):
+    if where == "fixture":
+        stem = mesh_dir / name
+    else:
+        stem = GEN / name / name
+        if not pathlib.Path(str(stem) + ".geo").exists():
+            pytest.skip(f"{stem}.geo not made here (make -C oracle mesh-large)")
+    ref = reference.mesh(stem, dt=DT)
+    got = ingest(stem)
+    for k in ("n_cells", "n_bnd_cells", "n_faces", "nnz"):
+        assert got[k] == ref[k], k
+    bad = [key for key, _ in reference.MESH_ARRAYS.values()
+           if not np.array_equal(got[key], ref[key])]
+    assert not bad, f"arrays differing from the reference: {bad}"
+
+
+def _faces(m):
+    f = m["faces_c"].reshape(-1, 2)
+    return f[:, 0] - 1, f[:, 1]
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_cells_closed_and_volumes(mesh_dir, name):
+    m = get(mesh_dir, name)
+    geo, nc = m["geo"], m["n_cells"]
+    c1, c2 = _faces(m)
+    inn = c2 > 0
+    s, xf, sh = geo["s"], geo["xf"], geo["shift"]
+    area = np.sqrt((s * s).sum(0))
+    closure = np.zeros((3, nc))
+    for k in range(3):
+        np.add.at(closure[k], c1, s[k])
+        np.add.at(closure[k], c2[inn] - 1, -s[k][inn])
+    assert np.abs(closure).max() <= 1e-12 * area.mean()
+    # V = 1/3 sum over faces of x_f . S_out; the neighbour across a periodic
+    # face sees that face at x_f - shift
+    v = np.zeros(nc)
+    np.add.at(v, c1, (xf * s).sum(0))
+    np.add.at(v, c2[inn] - 1, -((xf - sh) * s).sum(0)[inn])
+    v /= 3.0
+    assert np.abs(v - geo["vol"]).max() <= 1e-11 * geo["vol"].max()
+    assert (geo["vol"] > 0).all()
+    # centre-to-centre vectors point across the face from c1
+    assert (m["scoef"] > 0).all()
+
+
+def test_orthogonal_mesh_annihilates_coordinates(mesh_dir):
+    m = get(mesh_dir, "cavity")
+    geo, nc = m["geo"], m["n_cells"]
+    c1, c2 = _faces(m)
+    periodic = np.abs(geo["shift"]).sum(0) > 0
+    boundary = c2 < 0
+    # faces where the centre-to-centre vector is not along S (the cavity's
+    # refined patches, whose faces meet cells of another size)
+    d = m["d"].reshape(3, -1)
+    s = geo["s"]
+    cos = (d * s).sum(0) / np.sqrt((d * d).sum(0) * (s * s).sum(0))
+    oblique = cos < 1.0 - 1e-12
+    skip = np.zeros(nc, bool)
+    skip[c1[periodic | boundary | oblique]] = True
+    skip[c2[(periodic | oblique) & ~boundary] - 1] = True
+    row, col, val = m["row"] - 1, m["col"] - 1, m["val"]
+    for k in range(3):
+        x = geo["xc"][k, :nc]
+        ax = np.zeros(nc)
+        for i in np.flatnonzero(~skip):
+            ax[i] = val[row[i]:row[i + 1]] @ x[col[row[i]:row[i + 1]]]
+        scale = np.abs(val[m["dia"] - 1]).max() * np.abs(x).max()
+        assert np.abs(ax[~skip]).max() <= 1e-12 * scale
+    assert (~skip).sum() > nc // 4
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_pressure_matrix_symmetric_zero_row_sums(mesh_dir, name):
+    m = get(mesh_dir, name)
+    row, col, dia, val = m["row"] - 1, m["col"] - 1, m["dia"] - 1, m["val"]
+    n = m["n_cells"]
+    rows = np.repeat(np.arange(n), np.diff(row))
+    a = {(int(i), int(j)): v for i, j, v in zip(rows, col, val)}
+    assert all(a[(j, i)] == v for (i, j), v in a.items())
+    off = val.copy()
+    off[dia] = 0.0
+    rowsum = np.add.reduceat(off, row[:-1]) + val[dia]
+    assert np.abs(rowsum).max() <= 1e-13 * val[dia].max()
+    assert (val[dia] > 0).all() and (off <= 0).all()
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_mesh_case(mesh_dir, name):
+    c = mesh.mesh_case(mesh_dir / name, dt=DT, seed=20261202)
+    m = get(mesh_dir, name)
+    want = m["val"].copy()
+    want[m["dia"][0] - 1] *= 2.0
+    assert np.array_equal(c.val, want)
+    assert np.array_equal(c.row, m["row"] - 1) and np.array_equal(c.col, m["col"] - 1)
+    z = np.load(GOLDEN / f"{name}_cg_p2.npz", allow_pickle=False)
+    assert sha(c.b) == str(z["b_sha256"])
+    assert abs(c.b.sum()) <= 1e-12 * np.abs(c.b).sum()
+    c1 = mesh.mesh_case(mesh_dir / name, dt=DT, seed=20261202, base=1, pin_first=False)
+    assert np.array_equal(c1.row, m["row"]) and np.array_equal(c1.val, m["val"])
+
+
+def test_reference_solution_fixture_self_consistent(oracle, mesh_dir):
+    """The committed reference solution solves the system mesh_case builds
+    (the oracle's residual of it is below the recorded tolerance)."""
+    c = mesh.mesh_case(mesh_dir / "cavity", dt=DT)
+    z = np.load(GOLDEN / "cavity_cg_p2.npz", allow_pickle=False)
+    r = oracle.residual(c.row, c.col, c.dia, c.val, z["x"], c.b)
+    assert np.linalg.norm(r) < 2.0 * float(z["tol"])
+
+
+# ---- error handling ----------------------------------------------------------
+
+def test_errors(mesh_dir, tmp_path):
+    cavity = mesh_dir / "cavity"
+    with pytest.raises(ValueError, match="cannot open"):
+        mesh.read_cns(tmp_path / "missing.cns")
+    raw = (pathlib.Path(str(cavity) + ".cns")).read_bytes()
+    (tmp_path / "short.cns").write_bytes(raw[: len(raw) // 2])
+    with pytest.raises(ValueError, match="bad record"):
+        mesh.read_cns(tmp_path / "short.cns")
+    bad = bytearray(raw)
+    bad[0] = 7                                    # first record's length marker
+    (tmp_path / "marker.cns").write_bytes(bytes(bad))
+    with pytest.raises(ValueError, match="bad record"):
+        mesh.read_cns(tmp_path / "marker.cns")
+    # a face naming a cell beyond n_cells
+    cns = mesh.read_cns(str(cavity) + ".cns")
+    nf = cns["n_faces"]
+    faces2 = np.ascontiguousarray(cns["faces_c"][1::2])
+    at = raw.find(faces2.tobytes())
+    assert at > 0 and np.frombuffer(raw[at - 4:at], np.int32)[0] == 4 * nf
+    faces2[0] = cns["n_cells"] + 1
+    (tmp_path / "cell.cns").write_bytes(raw[:at] + faces2.tobytes() + raw[at + 4 * nf:])
+    with pytest.raises(ValueError, match="bad record"):
+        mesh.read_cns(tmp_path / "cell.cns")
+    # a .geo read with the sizes of another mesh
+    other = mesh.read_cns(str(mesh_dir / "backstep_nonorthogonal") + ".cns")
+    with pytest.raises(ValueError, match="bad record"):
+        mesh.read_geo(str(cavity) + ".geo", other)
+    lib = mesh._lib()
+    assert lib.tfs_mesh_read_cns(None, None, None, None, None, None) == -1
+    assert lib.tfs_mesh_read_geo(b"x", 0, 0, 0, *([None] * 6)) == -1
+    sc = np.zeros(1)
+    bad_faces = np.array([1, 3], np.int32)        # c2 beyond n_cells = 2
+    assert lib.tfs_mesh_face_coef(2, 0, 1, bad_faces, np.zeros(6), np.zeros(3), np.zeros(3),
+                                  None, sc) == -1
