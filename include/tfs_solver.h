@@ -246,6 +246,55 @@ int tfs_assemble_device(tfs_assembly *S, const double *d_a12, const double *d_a2
                         const unsigned char *d_bnd_on, const double *d_cell, double urf,
                         double *d_sav);
 
+/* ---- topology on the device (replaces Matrix_Mod_Topology) -------------------
+ * Sources/Process/Matrix_Mod/Topology.f90:2-127 (its row/col/dia/pos; mir is
+ * not formed): the CRS structure of a face list built from device arrays.
+ * d_faces_c = faces_c(1:2, n_faces) column-major, 1-based cells, c2 <= 0 a
+ * boundary face (as tfs_mesh_read_cns returns it).  Output in index base
+ * `base`: d_row[n_cells + 1], d_col[<= col_cap] (each row ascending),
+ * d_dia[n_cells] (the last entry of row c equal to c), and, if d_pos is not
+ * NULL, d_pos[2 n_faces] (the last position of c2 in row c1 and of c1 in row
+ * c2; base - 1 for boundary faces).  Bitwise equal to tfs_topology_build.
+ * d_faces_c and d_pos must be 8-byte aligned.  *nnz receives the entry count
+ * (n_cells + 2 x interior faces); TFS_ERR_INVALID for a face with c1 outside
+ * 1..n_cells or c2 above n_cells, or for nnz above col_cap or 2^31 - 2
+ * (*nnz then says how many are needed).  Synchronous: runs on the calling
+ * thread's default stream and returns when the arrays are complete. */
+int tfs_topology_build_device(int n_cells, long long n_faces, const int *d_faces_c, int base,
+                              long long col_cap, int *d_row, int *d_col, int *d_dia, int *d_pos,
+                              long long *nnz);
+
+/* ---- least-squares cell gradients ----------------------------------------------
+ * Sources/Process/Compute_Gradient_Matrix.f90:21-116 (G = inverse of the sum
+ * of d d^T over a cell's faces), GraPhi.f90:25-173 (the gradient of phi, all
+ * three components in one call) and GradP.f90:24-64 (the pressure gradient
+ * with the boundary values copied, extrapolated by 1.2 x the gradient, and the
+ * gradient formed again), with Boundary = .TRUE. as every call site passes it
+ * and every material a fluid.  Bitwise equal to the reference (each cell sums
+ * its faces in face order).
+ * tfs_gradient_create (host arrays, once per mesh): faces_c as
+ * tfs_mesh_read_cns returns it (c1, c2 per face, 1-based, c2 = -1..-n_bnd for
+ * boundary cells; each boundary cell on one face); d = the face vectors after
+ * Compute_Geometry (tfs_mesh_face_coef), three blocks of n_faces (dx, dy,
+ * dz); xc = cell centres, three blocks of n_cells + n_bnd (cells, then
+ * boundary cells; needed by tfs_grad_p_device only, may be NULL otherwise);
+ * bc = the kind of each boundary cell (TFS_BC_*; NULL = walls).  G is formed
+ * on the device at creation.
+ * Device arrays: phi has n_cells + n_bnd entries (cells, then boundary cells);
+ * px, py, pz n_cells each.  tfs_grad_p_device rewrites phi's boundary part,
+ * as GradP does.  Synchronous, on the calling thread's default stream. */
+enum { TFS_BC_WALL = 0, TFS_BC_SYMMETRY = 1, TFS_BC_PRESSURE = 2, TFS_BC_BUFFER = 3 };
+typedef struct tfs_gradient tfs_gradient;
+int tfs_gradient_create(int n_cells, int n_bnd_cells, long long n_faces, const int *faces_c,
+                        const double *d, const double *xc, const unsigned char *bc,
+                        tfs_gradient **out);
+int tfs_gradient_destroy(tfs_gradient *S);
+/* host copy of G(1:6, c), six per cell (the reference's column-major G) */
+int tfs_gradient_matrix_get(tfs_gradient *S, double *g);
+int tfs_grad_phi_device(tfs_gradient *S, const double *d_phi, double *d_px, double *d_py,
+                        double *d_pz);
+int tfs_grad_p_device(tfs_gradient *S, double *d_phi, double *d_px, double *d_py, double *d_pz);
+
 /* ---- serving: continuous batching of solves on one GPU ------------------------
  * Not in the reference.  A case (one system) is registered with its structure
  * and values; its GPU copy is made when its first solve is admitted and stays

@@ -61,7 +61,44 @@ class Oracle:
         lib.or_assemble.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, _I32P, ctypes.c_int,
                                     _F64P, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                     ctypes.c_double, _F64P, ctypes.c_void_p]
+        _VP = ctypes.c_void_p
+        lib.or_gradient_matrix.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _F64P, _F64P]
+        lib.or_graphi.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _F64P, _F64P, _F64P,
+                                  ctypes.c_int, _VP, _F64P]
+        lib.or_gradp.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _F64P, _F64P, _F64P,
+                                 ctypes.c_int, _VP, _F64P, _F64P, _F64P, _F64P]
         self.lib = lib
+
+    # ---- least-squares gradients: d (3, n_faces); g (n_cells, 6) = G(1:6, c)
+    def gradient_matrix(self, n_cells, faces_c, d):
+        faces_c = _c32(faces_c)
+        g = np.zeros((n_cells, 6))
+        self.lib.or_gradient_matrix(n_cells, faces_c.size // 2, faces_c, _c64(d).ravel(), g)
+        return g
+
+    def graphi(self, n_cells, faces_c, d, g, phi, bc=None):
+        """(phi_x, phi_y, phi_z): GraPhi for i = 1, 2, 3 (Boundary = .TRUE.)."""
+        faces_c = _c32(faces_c)
+        bcb = None if bc is None else np.ascontiguousarray(bc, np.uint8)
+        out = []
+        for i in (1, 2, 3):
+            o = np.zeros(n_cells)
+            self.lib.or_graphi(n_cells, faces_c.size // 2, faces_c, _c64(d).ravel(), _c64(g), _c64(phi),
+                               i, None if bcb is None else bcb.ctypes.data, o)
+            out.append(o)
+        return tuple(out)
+
+    def gradp(self, n_cells, faces_c, d, g, xc, phi, bc=None):
+        """GradP: (phi with its boundary part rewritten, px, py, pz)."""
+        faces_c = _c32(faces_c)
+        xc = _c64(xc)
+        n_bnd = xc.size // 3 - n_cells
+        bcb = None if bc is None else np.ascontiguousarray(bc, np.uint8)
+        phi = _c64(phi).copy()
+        px, py, pz = (np.zeros(n_cells) for _ in range(3))
+        self.lib.or_gradp(n_cells, faces_c.size // 2, faces_c, _c64(d).ravel(), _c64(g), xc.ravel(), n_bnd,
+                          None if bcb is None else bcb.ctypes.data, phi, px, py, pz)
+        return phi, px, py, pz
 
     def _mat(self, row, col, dia, val):
         keep = (_c32(row), _c32(col), _c32(dia), _c64(val))
@@ -164,6 +201,8 @@ class Reference:
             lib.tfs_ref_mesh_load.restype = ctypes.c_int
             lib.tfs_ref_mesh_load.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
                                               _I32P]
+            lib.tfs_ref_mesh_grad.restype = ctypes.c_int
+            lib.tfs_ref_mesh_grad.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 6
             lib.tfs_ref_mesh_get.restype = ctypes.c_int
             lib.tfs_ref_mesh_get.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                              ctypes.c_void_p]
@@ -262,3 +301,19 @@ class Reference:
                 raise RuntimeError(f"tfs_ref_mesh_get({code}) returned {k}")
             out[key] = (ibuf if is_int else rbuf)[:k].copy()
         return out
+
+    def mesh_grad(self, mode, n_cells, n_bnd, phi=None, bc=None):
+        """On the grid of the last mesh() call: mode 0 -> G (n_cells, 6) from
+        Compute_Gradient_Matrix; mode 1 -> (px, py, pz) from GraPhi; mode 2 ->
+        (phi, px, py, pz) from GradP.  bc: kind per boundary cell (0 wall,
+        1 symmetry, 2 pressure, 3 buffer), default walls."""
+        bcb = np.zeros(max(n_bnd, 1), np.int8) if bc is None else \
+            np.ascontiguousarray(bc, np.int8)
+        f = np.zeros(n_cells + n_bnd) if phi is None else _c64(phi).copy()
+        px, py, pz = (np.zeros(n_cells) for _ in range(3))
+        g = np.zeros((n_cells, 6))
+        rc = self.lib.tfs_ref_mesh_grad(mode, bcb.ctypes.data, f.ctypes.data, px.ctypes.data,
+                                        py.ctypes.data, pz.ctypes.data, g.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"tfs_ref_mesh_grad({mode}) returned {rc}")
+        return g if mode == 0 else (px, py, pz) if mode == 1 else (f, px, py, pz)

@@ -27,6 +27,7 @@
   implicit none
 
   integer :: m_nc = 0, m_nb = 0, m_nf = 0, m_nnz = 0
+  type(Grid_Type), allocatable, target :: mg     ! the grid last loaded
   integer, allocatable :: m_faces(:), m_row(:), m_col(:), m_dia(:)
   real,    allocatable :: m_xc(:), m_vol(:), m_s(:), m_shift(:), m_f(:), m_xf(:)
   real,    allocatable :: m_d(:), m_scoef(:), m_val(:)
@@ -42,7 +43,6 @@
   integer(c_int), value              :: nlen
   real(c_double), value              :: dt_in
   integer(c_int), intent(out)        :: sizes(4)
-  type(Grid_Type), allocatable, target :: g
   integer :: k, c, s, nc, nb, nf
 
   name = ' '
@@ -54,15 +54,16 @@
   if(allocated(TypeBC))   deallocate(TypeBC)
   if(allocated(bcmark))   deallocate(bcmark)
   if(allocated(CopyC))    deallocate(CopyC)
-  allocate(g)
-  call Load_Cns(g, 0)
-  nc = g % n_cells;  nb = g % n_bnd_cells;  nf = g % n_faces
+  if(allocated(mg)) deallocate(mg)
+  allocate(mg)
+  call Load_Cns(mg, 0)
+  nc = mg % n_cells;  nb = mg % n_bnd_cells;  nf = mg % n_faces
 
   if(allocated(delta))  deallocate(delta)
   if(allocated(WallDs)) deallocate(WallDs)
   allocate(delta(-nb:nc));  delta = 0.0
   allocate(WallDs(-nb:nc)); WallDs = 0.0
-  call Load_Geo(g, 0)
+  call Load_Geo(mg, 0)
 
   ! what was read: faces, cell centres (cells, then boundary cells -1..-nb),
   ! volumes, face surfaces, the periodic shifts, weights, face centres
@@ -70,25 +71,25 @@
   allocate(m_faces(2*nf), m_xc(3*(nc+nb)), m_vol(nc), m_s(3*nf), m_shift(3*nf),  &
            m_f(nf), m_xf(3*nf))
   do s = 1, nf
-    m_faces(2*s-1) = g % faces_c(1,s)
-    m_faces(2*s)   = g % faces_c(2,s)
+    m_faces(2*s-1) = mg % faces_c(1,s)
+    m_faces(2*s)   = mg % faces_c(2,s)
   end do
   do c = 1, nc
-    m_xc(c)            = g % xc(c)
-    m_xc(nc+nb+c)      = g % yc(c)
-    m_xc(2*(nc+nb)+c)  = g % zc(c)
-    m_vol(c)           = g % vol(c)
+    m_xc(c)            = mg % xc(c)
+    m_xc(nc+nb+c)      = mg % yc(c)
+    m_xc(2*(nc+nb)+c)  = mg % zc(c)
+    m_vol(c)           = mg % vol(c)
   end do
   do c = 1, nb
-    m_xc(nc+c)            = g % xc(-c)
-    m_xc(nc+nb+nc+c)      = g % yc(-c)
-    m_xc(2*(nc+nb)+nc+c)  = g % zc(-c)
+    m_xc(nc+c)            = mg % xc(-c)
+    m_xc(nc+nb+nc+c)      = mg % yc(-c)
+    m_xc(2*(nc+nb)+nc+c)  = mg % zc(-c)
   end do
   do s = 1, nf
-    m_s(s)        = g % sx(s);  m_s(nf+s)      = g % sy(s);  m_s(2*nf+s)      = g % sz(s)
-    m_shift(s)    = g % dx(s);  m_shift(nf+s)  = g % dy(s);  m_shift(2*nf+s)  = g % dz(s)
-    m_f(s)        = g % f(s)
-    m_xf(s)       = g % xf(s);  m_xf(nf+s)     = g % yf(s);  m_xf(2*nf+s)     = g % zf(s)
+    m_s(s)        = mg % sx(s);  m_s(nf+s)      = mg % sy(s);  m_s(2*nf+s)      = mg % sz(s)
+    m_shift(s)    = mg % dx(s);  m_shift(nf+s)  = mg % dy(s);  m_shift(2*nf+s)  = mg % dz(s)
+    m_f(s)        = mg % f(s)
+    m_xf(s)       = mg % xf(s);  m_xf(nf+s)     = mg % yf(s);  m_xf(2*nf+s)     = mg % zf(s)
   end do
 
   ! Compute_Geometry's inputs beyond the grid: every material a fluid,
@@ -98,16 +99,16 @@
   if(allocated(IsNearWall)) deallocate(IsNearWall)
   if(allocated(xp))         deallocate(xp, yp, zp)
   allocate(Scoef(nf), fF(nf), IsNearWall(-nb:nc))
-  allocate(xp(g % n_materials), yp(g % n_materials), zp(g % n_materials))
+  allocate(xp(mg % n_materials), yp(mg % n_materials), zp(mg % n_materials))
   xp = 0.0;  yp = 0.0;  zp = 0.0
   StateMat = FLUID
   TypeBC = WALL
-  call Compute_Geometry(g)
+  call Compute_Geometry(mg)
 
   if(allocated(m_d)) deallocate(m_d, m_scoef)
   allocate(m_d(3*nf), m_scoef(nf))
   do s = 1, nf
-    m_d(s) = g % dx(s);  m_d(nf+s) = g % dy(s);  m_d(2*nf+s) = g % dz(s)
+    m_d(s) = mg % dx(s);  m_d(nf+s) = mg % dy(s);  m_d(2*nf+s) = mg % dz(s)
     m_scoef(s) = Scoef(s)
   end do
 
@@ -119,10 +120,10 @@
   if(allocated(A % pos)) deallocate(A % pos)
   if(allocated(A % sav)) deallocate(A % sav)
   if(allocated(A % bou)) deallocate(A % bou)
-  call Matrix_Mod_Allocate(g, A)
-  call Matrix_Mod_Topology(g, A)
+  call Matrix_Mod_Allocate(mg, A)
+  call Matrix_Mod_Topology(mg, A)
   dt = dt_in
-  call Pressure_Matrix_Fractional(g)
+  call Pressure_Matrix_Fractional(mg)
 
   m_nnz = A % nonzeros - 1
   if(allocated(m_row)) deallocate(m_row, m_col, m_dia, m_val)
@@ -134,8 +135,66 @@
 
   m_nc = nc;  m_nb = nb;  m_nf = nf
   sizes(1) = nc;  sizes(2) = nb;  sizes(3) = nf;  sizes(4) = m_nnz
-  deallocate(g)
   tfs_ref_mesh_load = 0
+  end function
+
+  !---------------------------------------------------------------------------!
+  !   The least-squares gradients on the grid last loaded, every boundary      !
+  !   cell -b of kind bc(b): 0 WALL, 1 SYMMETRY, 2 PRESSURE, 3 BUFFER.          !
+  !   mode 0: Compute_Gradient_Matrix(grid, .TRUE.) -> g(6*nc), G(1:6,c)       !
+  !   mode 1: GraPhi(grid, phi, i, phi_i, .TRUE.) for i = 1, 2, 3 -> px, py, pz!
+  !   mode 2: GradP(grid, phi, px, py, pz); phi's boundary part rewritten      !
+  !   phi holds cells 1..nc, then boundary cells -1..-nb (nc+nb entries).      !
+  !   Modes 1 and 2 use the G of the last mode-0 call.                         !
+  !---------------------------------------------------------------------------!
+  integer(c_int) function tfs_ref_mesh_grad(mode, bc, phi, px, py, pz, gout) &
+                 bind(C, name='tfs_ref_mesh_grad')
+  integer(c_int), value          :: mode
+  integer(c_signed_char), intent(in) :: bc(*)
+  real(c_double), intent(inout)  :: phi(*)
+  real(c_double), intent(out)    :: px(*), py(*), pz(*), gout(*)
+  real, allocatable :: f(:), fx(:), fy(:), fz(:)
+  integer :: c, nc, nb
+  integer, parameter :: kinds(0:3) = (/WALL, SYMMETRY, PRESSURE, BUFFER/)
+
+  tfs_ref_mesh_grad = -1
+  if(.not. allocated(mg)) return
+  nc = m_nc;  nb = m_nb
+  do c = 1, nb
+    TypeBC(-c) = kinds(bc(c))
+  end do
+  if(mode == 0) then
+    if(allocated(G)) deallocate(G)
+    allocate(G(6, nc))
+    call Compute_Gradient_Matrix(mg, .TRUE.)
+    do c = 1, nc
+      gout(6*(c-1)+1 : 6*c) = G(1:6, c)
+    end do
+    tfs_ref_mesh_grad = 0
+    return
+  end if
+  if(.not. allocated(G)) return
+  allocate(f(-nb:nc), fx(-nb:nc), fy(-nb:nc), fz(-nb:nc))
+  fx = 0.0;  fy = 0.0;  fz = 0.0
+  f(1:nc) = phi(1:nc)
+  do c = 1, nb
+    f(-c) = phi(nc + c)
+  end do
+  if(mode == 1) then
+    call GraPhi(mg, f, 1, fx, .TRUE.)
+    call GraPhi(mg, f, 2, fy, .TRUE.)
+    call GraPhi(mg, f, 3, fz, .TRUE.)
+  else if(mode == 2) then
+    call GradP(mg, f, fx, fy, fz)
+  else
+    return
+  end if
+  px(1:nc) = fx(1:nc);  py(1:nc) = fy(1:nc);  pz(1:nc) = fz(1:nc)
+  phi(1:nc) = f(1:nc)
+  do c = 1, nb
+    phi(nc + c) = f(-c)
+  end do
+  tfs_ref_mesh_grad = 0
   end function
 
   !---------------------------------------------------------------------------!

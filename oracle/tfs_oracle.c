@@ -30,6 +30,11 @@
  *   or_assemble           : bitwise against the reference Pressure_Matrix_Fractional;
  *                           exact (integer-valued) coefficients against a dense
  *                           sum of per-face 2x2 element matrices.
+ *   or_gradient_matrix / or_graphi / or_gradp : bitwise against the reference's
+ *                           Compute_Gradient_Matrix / GraPhi / GradP on real
+ *                           meshes; G times the normal matrix is the identity;
+ *                           the gradient of a linear field is exact away from
+ *                           periodic faces; symmetric boundaries drop their faces.
  */
 #include "tfs_oracle.h"
 
@@ -390,4 +395,104 @@ void or_assemble(int n_cells, int n_faces, const int *faces_c, const int *pos, c
     if (sav) sav[c] = *d;
     if (urf != 1.0) *d = *d / urf;
   }
+}
+
+/* Compute_Gradient_Matrix.f90:21-28 (G := 0), :30-70 (Boundary = .TRUE.: both
+ * cells of an interior face, the inside cell of a boundary face, gain d d^T;
+ * the fluid-solid branch :41-53 is not taken), :96-116 (inverse of the
+ * symmetric 3x3 G by cofactors over Jac + TINY). */
+void or_gradient_matrix(int n_cells, int n_faces, const int *faces_c, const double *d,
+                        double *g) {
+  const double *dx = d, *dy = d + n_faces, *dz = d + 2 * (size_t)n_faces;
+  for (int c = 0; c < 6 * n_cells; ++c) g[c] = 0.0;
+  for (int s = 0; s < n_faces; ++s) {
+    const int c1 = faces_c[2 * s], c2 = faces_c[2 * s + 1];
+    const double x = dx[s], y = dy[s], z = dz[s];
+    double *g1 = g + 6 * (size_t)(c1 - 1);
+    g1[0] = g1[0] + x * x;
+    g1[1] = g1[1] + y * y;
+    g1[2] = g1[2] + z * z;
+    g1[3] = g1[3] + x * y;
+    g1[4] = g1[4] + x * z;
+    g1[5] = g1[5] + y * z;
+    if (c2 > 0) {
+      double *g2 = g + 6 * (size_t)(c2 - 1);
+      g2[0] = g2[0] + x * x;
+      g2[1] = g2[1] + y * y;
+      g2[2] = g2[2] + z * z;
+      g2[3] = g2[3] + x * y;
+      g2[4] = g2[4] + x * z;
+      g2[5] = g2[5] + y * z;
+    }
+  }
+  for (int c = 0; c < n_cells; ++c) {
+    double *G = g + 6 * (size_t)c, inv[6];
+    const double jac = G[0] * G[1] * G[2] - G[0] * G[5] * G[5] - G[3] * G[3] * G[2] +
+                       2.0 * G[3] * G[4] * G[5] - G[4] * G[4] * G[1];
+    inv[0] = +(G[1] * G[2] - G[5] * G[5]) / (jac + OR_TINY);
+    inv[1] = +(G[0] * G[2] - G[4] * G[4]) / (jac + OR_TINY);
+    inv[2] = +(G[0] * G[1] - G[3] * G[3]) / (jac + OR_TINY);
+    inv[3] = -(G[3] * G[2] - G[4] * G[5]) / (jac + OR_TINY);
+    inv[4] = +(G[3] * G[5] - G[4] * G[1]) / (jac + OR_TINY);
+    inv[5] = -(G[0] * G[5] - G[3] * G[4]) / (jac + OR_TINY);
+    for (int k = 0; k < 6; ++k) G[k] = inv[k];
+  }
+}
+
+/* GraPhi.f90:25-27 (phii := 0) and the face loop of component i (:29-73 for
+ * x, :75-123 for y, :125-173 for z) with Boundary = .TRUE. (:62-66): both
+ * cells use the same d and the same difference phi(c2) - phi(c1), zero on a
+ * SYMMETRY boundary (:41-44), each weighted by its own row of G. */
+void or_graphi(int n_cells, int n_faces, const int *faces_c, const double *d, const double *g,
+               const double *phi, int i, const unsigned char *bc, double *phii) {
+  const double *dx = d, *dy = d + n_faces, *dz = d + 2 * (size_t)n_faces;
+  /* the G entries of row i: (1,4,5), (4,2,6), (5,6,3) */
+  static const int row[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+  const int *r = row[i - 1];
+  for (int c = 0; c < n_cells; ++c) phii[c] = 0.0;
+  for (int s = 0; s < n_faces; ++s) {
+    const int c1 = faces_c[2 * s], c2 = faces_c[2 * s + 1];
+    const size_t k1 = (size_t)(c1 - 1);
+    const size_t k2 = c2 > 0 ? (size_t)(c2 - 1) : (size_t)n_cells + (size_t)(-c2 - 1);
+    double dphi = phi[k2] - phi[k1];
+    if (c2 < 0 && bc && bc[-c2 - 1] == OR_BC_SYMMETRY) dphi = 0.0;
+    const double x = dx[s], y = dy[s], z = dz[s];
+    const double *g1 = g + 6 * k1;
+    phii[k1] = phii[k1] + dphi * (g1[r[0]] * x + g1[r[1]] * y + g1[r[2]] * z);
+    if (c2 > 0) {
+      const double *g2 = g + 6 * k2;
+      phii[k2] = phii[k2] + dphi * (g2[r[0]] * x + g2[r[1]] * y + g2[r[2]] * z);
+    }
+  }
+}
+
+/* GradP.f90:26-44: boundary cells other than BUFFER and PRESSURE take the
+ * inside value, the gradient follows; :46-64: one pass extrapolating those
+ * boundary values along the gradient with factor 1.2, then the gradient
+ * again.  phi's boundary part is rewritten, as in the reference. */
+void or_gradp(int n_cells, int n_faces, const int *faces_c, const double *d, const double *g,
+              const double *xc, int n_bnd, const unsigned char *bc, double *phi, double *px,
+              double *py, double *pz) {
+  const size_t na = (size_t)n_cells + (size_t)n_bnd;
+  for (int s = 0; s < n_faces; ++s) {
+    const int c1 = faces_c[2 * s], c2 = faces_c[2 * s + 1];
+    const int kind = bc && c2 < 0 ? bc[-c2 - 1] : OR_BC_WALL;
+    if (c2 < 0 && kind != OR_BC_BUFFER && kind != OR_BC_PRESSURE)
+      phi[(size_t)n_cells + (size_t)(-c2 - 1)] = phi[c1 - 1];
+  }
+  or_graphi(n_cells, n_faces, faces_c, d, g, phi, 1, bc, px);
+  or_graphi(n_cells, n_faces, faces_c, d, g, phi, 2, bc, py);
+  or_graphi(n_cells, n_faces, faces_c, d, g, phi, 3, bc, pz);
+  for (int s = 0; s < n_faces; ++s) {
+    const int c1 = faces_c[2 * s], c2 = faces_c[2 * s + 1];
+    const int kind = bc && c2 < 0 ? bc[-c2 - 1] : OR_BC_WALL;
+    if (c2 < 0 && kind != OR_BC_BUFFER && kind != OR_BC_PRESSURE) {
+      const size_t k1 = (size_t)(c1 - 1), k2 = (size_t)n_cells + (size_t)(-c2 - 1);
+      phi[k2] = phi[k1] + 1.2 * (px[k1] * (xc[k2] - xc[k1]) + py[k1] * (xc[na + k2] - xc[na + k1]) +
+                                 pz[k1] * (xc[2 * na + k2] - xc[2 * na + k1]));
+    }
+  }
+  or_graphi(n_cells, n_faces, faces_c, d, g, phi, 1, bc, px);
+  or_graphi(n_cells, n_faces, faces_c, d, g, phi, 2, bc, py);
+  or_graphi(n_cells, n_faces, faces_c, d, g, phi, 3, bc, pz);
 }

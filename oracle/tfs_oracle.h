@@ -75,6 +75,22 @@ void or_assemble(int n_cells, int n_faces, const int *faces_c, const int *pos, c
                  int nnz, const double *a12, const double *a21, const unsigned char *bnd_on,
                  const double *cell, double urf, double *val, double *sav);
 
+/* Least-squares gradients (Sources/Process/Compute_Gradient_Matrix.f90,
+ * GraPhi.f90, GradP.f90), Boundary = .TRUE. as every call site passes it and
+ * every material a fluid.  Cells 0..n_cells-1, boundary cells -1..-n_bnd at
+ * n_cells + (-c2 - 1) in phi / xc; d = the face vectors after
+ * Compute_Geometry in three blocks of n_faces (dx, dy, dz); g = G(1:6, c)
+ * column-major, 6 per cell.  bc[b] is the kind of boundary cell -(b+1):
+ * OR_BC_* below (NULL: every boundary a wall). */
+enum { OR_BC_WALL = 0, OR_BC_SYMMETRY = 1, OR_BC_PRESSURE = 2, OR_BC_BUFFER = 3 };
+void or_gradient_matrix(int n_cells, int n_faces, const int *faces_c, const double *d,
+                        double *g);
+void or_graphi(int n_cells, int n_faces, const int *faces_c, const double *d, const double *g,
+               const double *phi, int i, const unsigned char *bc, double *phii);
+void or_gradp(int n_cells, int n_faces, const int *faces_c, const double *d, const double *g,
+              const double *xc, int n_bnd, const unsigned char *bc, double *phi, double *px,
+              double *py, double *pz);
+
 #ifdef __cplusplus
 }
 #endif

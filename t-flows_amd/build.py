@@ -26,7 +26,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-HIP_SOURCES = ["tfs_solver.hip", "tfs_layout.cpp", "tfs_server.cpp", "tfs_assembly.hip"]
+HIP_SOURCES = ["tfs_solver.hip", "tfs_layout.cpp", "tfs_server.cpp", "tfs_assembly.hip",
+               "tfs_topology.hip", "tfs_gradient.hip"]
 HIP_DEPS = ["tfs_device.hpp", "tfs_kernels.hpp", "tfs_layout.hpp"]
 
 

@@ -1,0 +1,262 @@
+// tfs_gradient.hip -- least-squares cell gradients on the device.
+//
+// T-FlowS computes cell gradients by least squares over the face neighbours:
+//   Compute_Gradient_Matrix  Sources/Process/Compute_Gradient_Matrix.f90:21-116
+//     G(c) = inverse of sum over the faces of c of d d^T (d the centre-to-centre
+//     vector of the face after Compute_Geometry), by cofactors over Jac + TINY;
+//   GraPhi                   Sources/Process/GraPhi.f90:25-173
+//     grad phi(c) = G(c) sum over the faces of c of (phi(c2) - phi(c1)) d,
+//     zero difference across a SYMMETRY boundary;
+//   GradP                    Sources/Process/GradP.f90:24-64
+//     boundary cells (not BUFFER / PRESSURE) take the inside value, gradient,
+//     boundary values extrapolated along it with factor 1.2, gradient again.
+// All with Boundary = .TRUE., as every call site passes it: both cells of an
+// interior face and the inside cell of a boundary face take the face's term.
+//
+// The reference accumulates in one sequential loop over faces.  Here every
+// cell sums its own faces -- kept in ascending face order in a per-cell
+// incidence list built once -- so each sum has the reference's order and the
+// result is bitwise the same, one thread per cell and no atomics.  The three
+// components of a gradient are formed in one pass (the reference makes three
+// passes over the same data, one per component; each component's sum is
+// unchanged).  Bound by HBM: per incidence, the face's cells, its d and two
+// gathered phi values; per cell, G and the three components.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "tfs_solver.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr double kTiny = 1.e-30;   // Sources/Shared/allp_mod.f90:40
+
+#define TFS_G_HIP(expr)                        \
+  do {                                         \
+    if ((expr) != hipSuccess) {                \
+      (void)hipGetLastError();                 \
+      return TFS_ERR_DEVICE;                   \
+    }                                          \
+  } while (0)
+
+int grid_of(long long m) {
+  long long g = (m + kThreads - 1) / kThreads;
+  return (int)(g < 1 ? 1 : (g > 8192 ? 8192 : g));
+}
+
+// G(c): the normal matrix summed over the cell's faces in face order, then its
+// inverse (Compute_Gradient_Matrix.f90:30-70, :96-116).  g holds six blocks of
+// n: G(1..6, c) at g[k n + c].
+__global__ void __launch_bounds__(kThreads)
+k_gmatrix(int n, long long nf, const int *__restrict__ ptr, const long long *__restrict__ inc,
+          const double *__restrict__ d, double *__restrict__ g) {
+  for (int c = blockIdx.x * kThreads + threadIdx.x; c < n; c += gridDim.x * kThreads) {
+    double G0 = 0.0, G1 = 0.0, G2 = 0.0, G3 = 0.0, G4 = 0.0, G5 = 0.0;
+    for (int k = ptr[c]; k < ptr[c + 1]; ++k) {
+      const long long s = inc[k] >> 1;
+      const double x = d[s], y = d[nf + s], z = d[2 * nf + s];
+      G0 = G0 + x * x;
+      G1 = G1 + y * y;
+      G2 = G2 + z * z;
+      G3 = G3 + x * y;
+      G4 = G4 + x * z;
+      G5 = G5 + y * z;
+    }
+    const double jac = G0 * G1 * G2 - G0 * G5 * G5 - G3 * G3 * G2 + 2.0 * G3 * G4 * G5 -
+                       G4 * G4 * G1;
+    g[c] = +(G1 * G2 - G5 * G5) / (jac + kTiny);
+    g[n + c] = +(G0 * G2 - G4 * G4) / (jac + kTiny);
+    g[2LL * n + c] = +(G0 * G1 - G3 * G3) / (jac + kTiny);
+    g[3LL * n + c] = -(G3 * G2 - G4 * G5) / (jac + kTiny);
+    g[4LL * n + c] = +(G3 * G5 - G4 * G1) / (jac + kTiny);
+    g[5LL * n + c] = -(G0 * G5 - G3 * G4) / (jac + kTiny);
+  }
+}
+
+// GraPhi for i = 1, 2, 3 at once (GraPhi.f90:25-73, :75-123, :125-173 with
+// Boundary = .TRUE.).  faces: (c1, c2) 1-based, c2 < 0 a boundary cell at
+// phi[n + (-c2 - 1)]; sym[b] != 0 for a SYMMETRY boundary cell -(b+1).
+__global__ void __launch_bounds__(kThreads)
+k_graphi(int n, long long nf, const int *__restrict__ ptr, const long long *__restrict__ inc,
+         const int2 *__restrict__ faces, const double *__restrict__ d,
+         const double *__restrict__ g, const unsigned char *__restrict__ sym,
+         const double *__restrict__ phi, double *__restrict__ px, double *__restrict__ py,
+         double *__restrict__ pz) {
+  for (int c = blockIdx.x * kThreads + threadIdx.x; c < n; c += gridDim.x * kThreads) {
+    const double g0 = g[c], g1 = g[n + c], g2 = g[2LL * n + c], g3 = g[3LL * n + c],
+                 g4 = g[4LL * n + c], g5 = g[5LL * n + c];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int k = ptr[c]; k < ptr[c + 1]; ++k) {
+      const long long s = inc[k] >> 1;
+      const int2 f = faces[s];
+      const long long k2 = f.y > 0 ? (long long)(f.y - 1) : (long long)n + (-f.y - 1);
+      double dphi = phi[k2] - phi[f.x - 1];
+      if (f.y < 0 && sym[-f.y - 1]) dphi = 0.0;
+      const double x = d[s], y = d[nf + s], z = d[2 * nf + s];
+      ax = ax + dphi * (g0 * x + g3 * y + g4 * z);
+      ay = ay + dphi * (g3 * x + g1 * y + g5 * z);
+      az = az + dphi * (g4 * x + g5 * y + g2 * z);
+    }
+    px[c] = ax;
+    py[c] = ay;
+    pz[c] = az;
+  }
+}
+
+// GradP.f90:32-40: the boundary cells listed in bf (inside cell, boundary slot)
+// take the inside value
+__global__ void __launch_bounds__(kThreads)
+k_bnd_copy(long long m, const int2 *__restrict__ bf, double *__restrict__ phi) {
+  for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < m;
+       i += (long long)gridDim.x * kThreads) {
+    const int2 e = bf[i];
+    phi[e.y] = phi[e.x];
+  }
+}
+
+// GradP.f90:48-58: extrapolation along the inside cell's gradient
+__global__ void __launch_bounds__(kThreads)
+k_bnd_extrap(long long m, long long na, const int2 *__restrict__ bf, const double *__restrict__ xc,
+             const double *__restrict__ px, const double *__restrict__ py,
+             const double *__restrict__ pz, double *__restrict__ phi) {
+  for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < m;
+       i += (long long)gridDim.x * kThreads) {
+    const int2 e = bf[i];
+    const int c1 = e.x, c2 = e.y;
+    phi[c2] = phi[c1] + 1.2 * (px[c1] * (xc[c2] - xc[c1]) + py[c1] * (xc[na + c2] - xc[na + c1]) +
+                               pz[c1] * (xc[2 * na + c2] - xc[2 * na + c1]));
+  }
+}
+
+}  // namespace
+
+struct tfs_gradient {
+  int n = 0, nb = 0;
+  long long nf = 0, nbf = 0;
+  int *ptr = nullptr;
+  long long *inc = nullptr;
+  int2 *faces = nullptr, *bf = nullptr;
+  double *d = nullptr, *g = nullptr, *xc = nullptr;
+  unsigned char *sym = nullptr;
+};
+
+extern "C" int tfs_gradient_destroy(tfs_gradient *S) {
+  if (!S) return TFS_OK;
+  for (void *p : {(void *)S->ptr, (void *)S->inc, (void *)S->faces, (void *)S->bf, (void *)S->d,
+                  (void *)S->g, (void *)S->xc, (void *)S->sym})
+    if (p) (void)hipFree(p);
+  delete S;
+  return TFS_OK;
+}
+
+extern "C" int tfs_gradient_create(int n_cells, int n_bnd_cells, long long n_faces,
+                                   const int *faces_c, const double *d, const double *xc,
+                                   const unsigned char *bc, tfs_gradient **out) {
+  if (!out) return TFS_ERR_INVALID;
+  *out = nullptr;
+  if (n_cells < 1 || n_bnd_cells < 0 || n_faces < 0 || (n_faces > 0 && (!faces_c || !d)))
+    return TFS_ERR_INVALID;
+  const int n = n_cells, nb = n_bnd_cells;
+  const long long nf = n_faces;
+  std::vector<int> cnt((size_t)n + 1, 0), seen((size_t)nb, 0);
+  std::vector<unsigned char> sym((size_t)std::max(nb, 1), 0);
+  std::vector<int2> bf;
+  for (long long s = 0; s < nf; ++s) {
+    const int c1 = faces_c[2 * s], c2 = faces_c[2 * s + 1];
+    if (c1 < 1 || c1 > n || c2 == 0 || c2 > n || c2 < -nb) return TFS_ERR_INVALID;
+    cnt[(size_t)c1]++;
+    if (c2 > 0) {
+      cnt[(size_t)c2]++;
+    } else {
+      // a boundary cell belongs to one face: GradP writes it from that face
+      if (seen[(size_t)(-c2 - 1)]++) return TFS_ERR_INVALID;
+      const int kind = bc ? bc[-c2 - 1] : TFS_BC_WALL;
+      if (kind > TFS_BC_BUFFER) return TFS_ERR_INVALID;
+      if (kind == TFS_BC_SYMMETRY) sym[(size_t)(-c2 - 1)] = 1;
+      if (kind != TFS_BC_BUFFER && kind != TFS_BC_PRESSURE)
+        bf.push_back(make_int2(c1 - 1, n + (-c2 - 1)));
+    }
+  }
+  // cell -> incident faces in ascending face order (a counting sort over the
+  // faces in order keeps each list sorted)
+  for (int c = 0; c < n; ++c) cnt[(size_t)c + 1] += cnt[(size_t)c];
+  std::vector<long long> inc((size_t)cnt[(size_t)n]);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (long long s = 0; s < nf; ++s) {
+    const int c1 = faces_c[2 * s] - 1, c2 = faces_c[2 * s + 1];
+    inc[(size_t)fill[(size_t)c1]++] = 2 * s;
+    if (c2 > 0) inc[(size_t)fill[(size_t)(c2 - 1)]++] = 2 * s + 1;
+  }
+  auto *S = new tfs_gradient;
+  S->n = n;
+  S->nb = nb;
+  S->nf = nf;
+  S->nbf = (long long)bf.size();
+  auto up = [](void **dst, const void *src, size_t bytes) -> bool {
+    if (hipMalloc(dst, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
+    return bytes == 0 || !src || hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  const size_t na = (size_t)n + (size_t)nb;
+  bool ok = up((void **)&S->ptr, cnt.data(), sizeof(int) * cnt.size()) &&
+            up((void **)&S->inc, inc.data(), sizeof(long long) * inc.size()) &&
+            up((void **)&S->faces, faces_c, sizeof(int2) * (size_t)nf) &&
+            up((void **)&S->bf, bf.data(), sizeof(int2) * bf.size()) &&
+            up((void **)&S->d, d, 3 * sizeof(double) * (size_t)nf) &&
+            up((void **)&S->g, nullptr, 6 * sizeof(double) * (size_t)n) &&
+            up((void **)&S->sym, sym.data(), sym.size()) &&
+            (!xc || up((void **)&S->xc, xc, 3 * sizeof(double) * na));
+  if (!ok) {
+    (void)hipGetLastError();
+    tfs_gradient_destroy(S);
+    return TFS_ERR_NOMEM;
+  }
+  hipLaunchKernelGGL(k_gmatrix, dim3(grid_of(n)), dim3(kThreads), 0, nullptr, n, nf, S->ptr,
+                     S->inc, S->d, S->g);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) {
+    (void)hipGetLastError();
+    tfs_gradient_destroy(S);
+    return TFS_ERR_DEVICE;
+  }
+  *out = S;
+  return TFS_OK;
+}
+
+extern "C" int tfs_gradient_matrix_get(tfs_gradient *S, double *g) {
+  if (!S || !g) return TFS_ERR_INVALID;
+  std::vector<double> soa(6 * (size_t)S->n);
+  TFS_G_HIP(hipMemcpy(soa.data(), S->g, sizeof(double) * soa.size(), hipMemcpyDeviceToHost));
+  for (int c = 0; c < S->n; ++c)
+    for (int k = 0; k < 6; ++k) g[6 * (size_t)c + k] = soa[(size_t)k * S->n + c];
+  return TFS_OK;
+}
+
+extern "C" int tfs_grad_phi_device(tfs_gradient *S, const double *d_phi, double *d_px,
+                                   double *d_py, double *d_pz) {
+  if (!S || !d_phi || !d_px || !d_py || !d_pz) return TFS_ERR_INVALID;
+  hipLaunchKernelGGL(k_graphi, dim3(grid_of(S->n)), dim3(kThreads), 0, nullptr, S->n, S->nf,
+                     S->ptr, S->inc, S->faces, S->d, S->g, S->sym, d_phi, d_px, d_py, d_pz);
+  TFS_G_HIP(hipGetLastError());
+  TFS_G_HIP(hipStreamSynchronize(nullptr));
+  return TFS_OK;
+}
+
+extern "C" int tfs_grad_p_device(tfs_gradient *S, double *d_phi, double *d_px, double *d_py,
+                                 double *d_pz) {
+  if (!S || !d_phi || !d_px || !d_py || !d_pz || (S->nbf > 0 && !S->xc)) return TFS_ERR_INVALID;
+  const long long na = (long long)S->n + S->nb;
+  if (S->nbf > 0)
+    hipLaunchKernelGGL(k_bnd_copy, dim3(grid_of(S->nbf)), dim3(kThreads), 0, nullptr, S->nbf,
+                       S->bf, d_phi);
+  hipLaunchKernelGGL(k_graphi, dim3(grid_of(S->n)), dim3(kThreads), 0, nullptr, S->n, S->nf,
+                     S->ptr, S->inc, S->faces, S->d, S->g, S->sym, d_phi, d_px, d_py, d_pz);
+  if (S->nbf > 0)
+    hipLaunchKernelGGL(k_bnd_extrap, dim3(grid_of(S->nbf)), dim3(kThreads), 0, nullptr, S->nbf,
+                       na, S->bf, S->xc, d_px, d_py, d_pz, d_phi);
+  hipLaunchKernelGGL(k_graphi, dim3(grid_of(S->n)), dim3(kThreads), 0, nullptr, S->n, S->nf,
+                     S->ptr, S->inc, S->faces, S->d, S->g, S->sym, d_phi, d_px, d_py, d_pz);
+  TFS_G_HIP(hipGetLastError());
+  TFS_G_HIP(hipStreamSynchronize(nullptr));
+  return TFS_OK;
+}

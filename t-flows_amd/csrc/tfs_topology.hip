@@ -1,0 +1,283 @@
+// tfs_topology.hip -- the CRS topology of a face list, built on the device.
+//
+// Sources/Process/Matrix_Mod/Topology.f90 forms the matrix structure from the
+// faces in four sequential loops: stencil widths (:34-41), row starts with the
+// diagonal first (:54-59), off-diagonal columns appended in face order
+// (:62-71), each row sorted ascending with its diagonal located (:75-82), and
+// the two entries of every interior face located (:106-121).  Here:
+//
+//   k_width   one thread per face, atomic width counts per cell;
+//   k_scan_*  row starts: per-block sums, one scan of the block sums, per-block
+//             rescan (an exclusive scan of width + 1 with 64-bit totals);
+//   k_fill    one thread per face, atomic cursor per row; each off-diagonal
+//             entry carries a tag naming its face and side (2 s + side);
+//   k_sort    one thread per cell: insertion sort of its row's (column, tag)
+//             pairs by column, the diagonal's position (the last entry equal
+//             to the cell, as the reference's loop leaves it), and pos: every
+//             tag in a run of equal columns gets the run's last position (the
+//             reference's search keeps the last match).
+//
+// Atomics make the fill order nondeterministic, but a row holds a multiset of
+// columns and sorting it fixes the order, so the result is bitwise the
+// sequential one; tags only travel with their columns.  Everything is
+// integer traffic: the kernels are bound by the device-scope atomics (width
+// counts and fill cursors land on cells in mesh order, which is random on an
+// unstructured numbering) and by the scattered pos writes, not by arithmetic.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+
+#include "tfs_solver.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kScanItems = 4;                       // per thread
+constexpr int kScanTile = kThreads * kScanItems;    // cells per scan block
+
+#define TFS_T_HIP(expr)        \
+  do {                         \
+    if ((expr) != hipSuccess) { \
+      (void)hipGetLastError(); \
+      rc = TFS_ERR_DEVICE;     \
+      goto done;               \
+    }                          \
+  } while (0)
+
+int grid_of(long long m) {
+  long long g = (m + kThreads - 1) / kThreads;
+  return (int)(g < 1 ? 1 : (g > 8192 ? 8192 : g));
+}
+
+__global__ void __launch_bounds__(kThreads) k_fill_int(long long m, int v, int *__restrict__ a) {
+  for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < m;
+       i += (long long)gridDim.x * kThreads)
+    a[i] = v;
+}
+
+// widths of the off-diagonal part; err gets 1 for c1 outside 1..n or c2 > n
+// (c2 <= 0 is a boundary face, as the reference's test c2 > 0 has it)
+__global__ void __launch_bounds__(kThreads)
+k_width(int n, long long nf, const int *__restrict__ faces, int *__restrict__ width,
+        int *__restrict__ err) {
+  for (long long s = blockIdx.x * (long long)kThreads + threadIdx.x; s < nf;
+       s += (long long)gridDim.x * kThreads) {
+    const int2 f = reinterpret_cast<const int2 *>(faces)[s];
+    if (f.x < 1 || f.x > n || f.y > n) {
+      atomicOr(err, 1);
+      continue;
+    }
+    if (f.y > 0) {
+      atomicAdd(&width[f.x - 1], 1);
+      atomicAdd(&width[f.y - 1], 1);
+    }
+  }
+}
+
+// block sums of (width + 1) over tiles of kScanTile cells
+__global__ void __launch_bounds__(kThreads)
+k_scan_sums(int n, const int *__restrict__ width, long long *__restrict__ bsum) {
+  __shared__ long long part[kThreads / 64];
+  const long long base = (long long)blockIdx.x * kScanTile;
+  long long t = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const long long c = base + (long long)k * kThreads + threadIdx.x;
+    if (c < n) t += width[c] + 1;
+  }
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long s = 0;
+    for (int w = 0; w < kThreads / 64; ++w) s += part[w];
+    bsum[blockIdx.x] = s;
+  }
+}
+
+// exclusive scan of the block sums in place (one workgroup); total -> *total
+__global__ void __launch_bounds__(kThreads)
+k_scan_top(int nb, long long *__restrict__ bsum, long long *__restrict__ total) {
+  __shared__ long long buf[kThreads];
+  long long carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += kThreads) {
+    const int b = b0 + threadIdx.x;
+    const long long v = b < nb ? bsum[b] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < kThreads; o <<= 1) {     // Hillis-Steele inclusive scan
+      const long long add = threadIdx.x >= o ? buf[threadIdx.x - o] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (b < nb) bsum[b] = carry + buf[threadIdx.x] - v;
+    carry += buf[kThreads - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+// row(c) = base + exclusive prefix of (width + 1); the diagonal goes first in
+// its row and the row's fill cursor restarts at 1 (Topology.f90:54-59)
+__global__ void __launch_bounds__(kThreads)
+k_scan_rows(int n, int base, const long long *__restrict__ bsum, int *__restrict__ width,
+            int *__restrict__ row, int *__restrict__ col) {
+  __shared__ int wsum[kThreads / 64];
+  const long long t0 = (long long)blockIdx.x * kScanTile;
+  // a thread's kScanItems consecutive cells
+  int v[kScanItems], acc = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const long long c = t0 + (long long)threadIdx.x * kScanItems + k;
+    v[k] = c < n ? width[c] + 1 : 0;
+    acc += v[k];
+  }
+  // exclusive scan of the per-thread totals: within the wave, then waves
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int inc = acc;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wv; ++w) before += wsum[w];
+  long long off = bsum[blockIdx.x] + base + before + inc - acc;
+  for (int k = 0; k < kScanItems; ++k) {
+    const long long c = t0 + (long long)threadIdx.x * kScanItems + k;
+    if (c < n) {
+      row[c] = (int)off;
+      col[off - base] = (int)c + base;
+      width[c] = 1;
+      if (c == n - 1) row[n] = (int)(off + v[k]);
+    }
+    off += v[k];
+  }
+}
+
+// off-diagonal columns through per-row cursors (Topology.f90:62-71); the
+// entry's tag 2 s + side locates pos(side + 1, s) after the sort; boundary
+// faces get pos = base - 1 here
+__global__ void __launch_bounds__(kThreads)
+k_fill(long long nf, int base, const int *__restrict__ faces, const int *__restrict__ row,
+       int *__restrict__ cursor, int *__restrict__ col, long long *__restrict__ tag,
+       int *__restrict__ pos) {
+  for (long long s = blockIdx.x * (long long)kThreads + threadIdx.x; s < nf;
+       s += (long long)gridDim.x * kThreads) {
+    const int2 f = reinterpret_cast<const int2 *>(faces)[s];
+    if (f.y > 0) {
+      const int c1 = f.x - 1, c2 = f.y - 1;
+      const long long q1 = row[c1] - base + atomicAdd(&cursor[c1], 1);
+      const long long q2 = row[c2] - base + atomicAdd(&cursor[c2], 1);
+      col[q1] = c2 + base;
+      col[q2] = c1 + base;
+      if (tag) {
+        tag[q1] = 2 * s;
+        tag[q2] = 2 * s + 1;
+      }
+    } else if (pos) {
+      reinterpret_cast<int2 *>(pos)[s] = make_int2(base - 1, base - 1);
+    }
+  }
+}
+
+// ascending columns per row with their tags, the diagonal's position
+// (Topology.f90:75-82), and the face positions (:106-121)
+__global__ void __launch_bounds__(kThreads)
+k_sort(int n, int base, const int *__restrict__ row, int *__restrict__ col,
+       long long *__restrict__ tag, int *__restrict__ dia, int *__restrict__ pos) {
+  for (int c = blockIdx.x * kThreads + threadIdx.x; c < n; c += gridDim.x * kThreads) {
+    const long long b = row[c] - base, e = row[c + 1] - base;
+    for (long long q = b + 1; q < e; ++q) {
+      const int key = col[q];
+      const long long t = tag ? tag[q] : 0;
+      long long r = q;
+      while (r > b && col[r - 1] > key) {
+        col[r] = col[r - 1];
+        if (tag) tag[r] = tag[r - 1];
+        --r;
+      }
+      col[r] = key;
+      if (tag) tag[r] = t;
+    }
+    long long d = b;
+    for (long long q = b; q < e; ++q)
+      if (col[q] == c + base) d = q;
+    dia[c] = (int)(d + base);
+    if (!tag) continue;
+    // walk down the row: `last` is the end of the current run of equal columns
+    long long last = e - 1;
+    for (long long q = e - 1; q >= b; --q) {
+      if (col[q] != col[last]) last = q;
+      const long long t = tag[q];
+      if (t >= 0) pos[t] = (int)(last + base);
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int tfs_topology_build_device(int n_cells, long long n_faces, const int *d_faces_c,
+                                         int base, long long col_cap, int *d_row, int *d_col,
+                                         int *d_dia, int *d_pos, long long *nnz) {
+  if (n_cells < 1 || n_faces < 0 || (base != 0 && base != 1) || !d_row || !d_col || !d_dia ||
+      (n_faces > 0 && !d_faces_c) || col_cap < n_cells)
+    return TFS_ERR_INVALID;
+  // faces and pos are read and written as (c1, c2) / (pos1, pos2) pairs
+  if ((reinterpret_cast<uintptr_t>(d_faces_c) | reinterpret_cast<uintptr_t>(d_pos)) % 8 != 0)
+    return TFS_ERR_INVALID;
+  if (nnz) *nnz = 0;
+  const int nb = (int)((n_cells + (long long)kScanTile - 1) / kScanTile);
+  int rc = TFS_OK;
+  int *width = nullptr, *err = nullptr;
+  long long *bsum = nullptr, *total = nullptr, *tag = nullptr;
+  int h_err = 0;
+  long long h_total = 0;
+  const hipStream_t st = nullptr;    // the calling thread's default stream
+  TFS_T_HIP(hipMalloc(&width, sizeof(int) * (size_t)n_cells));
+  TFS_T_HIP(hipMalloc(&bsum, sizeof(long long) * (size_t)nb + 2 * sizeof(long long)));
+  total = bsum + nb;
+  err = reinterpret_cast<int *>(bsum + nb + 1);
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_of(n_cells)), dim3(kThreads), 0, st,
+                     (long long)n_cells, 0, width);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kThreads), 0, st, 1LL, 0, err);
+  if (n_faces > 0)
+    hipLaunchKernelGGL(k_width, dim3(grid_of(n_faces)), dim3(kThreads), 0, st, n_cells, n_faces,
+                       d_faces_c, width, err);
+  hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(kThreads), 0, st, n_cells, width, bsum);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kThreads), 0, st, nb, bsum, total);
+  TFS_T_HIP(hipGetLastError());
+  TFS_T_HIP(hipMemcpy(&h_total, total, sizeof(long long), hipMemcpyDeviceToHost));
+  TFS_T_HIP(hipMemcpy(&h_err, err, sizeof(int), hipMemcpyDeviceToHost));
+  if (h_err) {
+    rc = TFS_ERR_INVALID;
+    goto done;
+  }
+  if (h_total > INT_MAX - 1 || h_total > col_cap) {
+    rc = TFS_ERR_INVALID;
+    if (nnz) *nnz = h_total;
+    goto done;
+  }
+  if (d_pos) {
+    TFS_T_HIP(hipMalloc(&tag, sizeof(long long) * (size_t)h_total));
+    // the diagonal entries keep tag -1 (k_scan_rows writes only their column)
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_of(2 * h_total)), dim3(kThreads), 0, st,
+                       2 * h_total, -1, reinterpret_cast<int *>(tag));
+  }
+  hipLaunchKernelGGL(k_scan_rows, dim3(nb), dim3(kThreads), 0, st, n_cells, base, bsum, width,
+                     d_row, d_col);
+  if (n_faces > 0)
+    hipLaunchKernelGGL(k_fill, dim3(grid_of(n_faces)), dim3(kThreads), 0, st, n_faces, base,
+                       d_faces_c, d_row, width, d_col, tag, d_pos);
+  hipLaunchKernelGGL(k_sort, dim3(grid_of(n_cells)), dim3(kThreads), 0, st, n_cells, base, d_row,
+                     d_col, tag, d_dia, d_pos);
+  TFS_T_HIP(hipGetLastError());
+  TFS_T_HIP(hipStreamSynchronize(st));
+  if (nnz) *nnz = h_total;
+done:
+  if (width) (void)hipFree(width);
+  if (bsum) (void)hipFree(bsum);
+  if (tag) (void)hipFree(tag);
+  return rc;
+}

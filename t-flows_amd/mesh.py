@@ -301,8 +301,9 @@ def mesh_case(stem, dt=1.0, pin_first=True, seed=20261202, base=0):
     <stem>.geo: topology as Matrix_Mod_Topology builds it, values as
     Pressure_Matrix_Fractional forms them (every boundary a wall), cell 1
     pinned by doubling a_11 when pin_first, and a seeded zero-mean uniform
-    right-hand side.  face_coefs holds (a12 = a21 = dt Scoef, no cell term)
-    for the device assembly."""
+    right-hand side.  face_coefs holds (a12 = a21 = dt Scoef on interior
+    faces and 0 on boundary faces, no cell term): assembled on the device they
+    give the values before pinning."""
     stem = str(stem)
     cns = read_cns(stem + ".cns")
     geo = read_geo(stem + ".geo", cns)
@@ -318,6 +319,7 @@ def mesh_case(stem, dt=1.0, pin_first=True, seed=20261202, base=0):
     if base:
         row, col, dia = row + base, col + base, dia + base
     a = dt * scoef
+    a[faces[1::2] <= 0] = 0.0      # walls add nothing to the diagonal
     return Case(n=n, row=row, col=col, dia=dia, val=val, b=b, base=base, dt=dt, faces=faces,
                 face_coefs=(a, a.copy(), np.zeros(n)),
                 params=dict(mesh=stem, dt=dt, pin_first=pin_first, seed=seed,

@@ -631,3 +631,99 @@ class Assembly:
                                              ptr(bnd_on, self.nf, "bnd_on"),
                                              ptr(cell, n, "cell"), float(urf),
                                              ptr(sav, n, "sav")), "tfs_assemble_device")
+
+
+def topology_device(n_cells, faces_c, base=1, with_pos=True):
+    """Matrix_Mod_Topology on the GPU (tfs_topology_build_device): faces_c is
+    a contiguous int32 GPU tensor of 2 n_faces entries (c1, c2 per face,
+    1-based, c2 <= 0 boundary).  Returns GPU int32 tensors (row, col, dia,
+    pos) in index base `base` (pos None unless with_pos)."""
+    import torch
+    lib = _lib()
+    if not getattr(lib, "_tfs_topo_typed", False):
+        lib.tfs_topology_build_device.argtypes = [ctypes.c_int, ctypes.c_longlong, _VP,
+                                                  ctypes.c_int, ctypes.c_longlong, _VP, _VP,
+                                                  _VP, _VP, ctypes.POINTER(ctypes.c_longlong)]
+        lib._tfs_topo_typed = True
+    if not (_is_device_tensor(faces_c) and faces_c.dtype == torch.int32
+            and faces_c.is_contiguous() and faces_c.numel() % 2 == 0):
+        raise ValueError("faces_c: contiguous int32 GPU tensor of 2 n_faces entries expected")
+    nf = faces_c.numel() // 2
+    cap = int(n_cells) + 2 * nf
+    opts = dict(dtype=torch.int32, device=faces_c.device)
+    row = torch.empty(int(n_cells) + 1, **opts)
+    col = torch.empty(max(cap, 1), **opts)
+    dia = torch.empty(int(n_cells), **opts)
+    pos = torch.empty(max(2 * nf, 2), **opts) if with_pos else None
+    nnz = ctypes.c_longlong(0)
+    _sync()
+    _check(lib.tfs_topology_build_device(int(n_cells), nf, faces_c.data_ptr() if nf else None,
+                                         int(base), cap, row.data_ptr(), col.data_ptr(),
+                                         dia.data_ptr(), pos.data_ptr() if with_pos else None,
+                                         ctypes.byref(nnz)), "tfs_topology_build_device")
+    return row, col[:nnz.value], dia, (pos[:2 * nf] if with_pos else None)
+
+
+TFS_BC_WALL, TFS_BC_SYMMETRY, TFS_BC_PRESSURE, TFS_BC_BUFFER = 0, 1, 2, 3
+
+
+class Gradient:
+    """Least-squares cell gradients on the GPU (tfs_gradient_*): the
+    counterparts of Compute_Gradient_Matrix, GraPhi and GradP.  faces_c, d
+    (3, n_faces), xc (3, n_cells + n_bnd) and bc (kind per boundary cell) are
+    host arrays as the mesh readers return them; phi and the gradient
+    components are GPU float64 tensors."""
+
+    def __init__(self, n_cells, n_bnd_cells, faces_c, d, xc=None, bc=None):
+        lib = _lib()
+        if not getattr(lib, "_tfs_grad_typed", False):
+            lib.tfs_gradient_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_longlong,
+                                                _I32P, _F64P, _VP, _VP, ctypes.POINTER(_VP)]
+            lib.tfs_gradient_destroy.argtypes = [_VP]
+            lib.tfs_gradient_matrix_get.argtypes = [_VP, _F64P]
+            lib.tfs_grad_phi_device.argtypes = [_VP] + [_VP] * 4
+            lib.tfs_grad_p_device.argtypes = [_VP] + [_VP] * 4
+            lib._tfs_grad_typed = True
+        self._lib = lib
+        self.n, self.nb = int(n_cells), int(n_bnd_cells)
+        faces_c = np.ascontiguousarray(faces_c, np.int32)
+        d = np.ascontiguousarray(d, np.float64).ravel()
+        self._keep = [None if xc is None else np.ascontiguousarray(xc, np.float64).ravel(),
+                      None if bc is None else np.ascontiguousarray(bc, np.uint8)]
+        h = _VP()
+        _check(lib.tfs_gradient_create(self.n, self.nb, faces_c.size // 2, faces_c, d,
+                                       None if xc is None else self._keep[0].ctypes.data,
+                                       None if bc is None else self._keep[1].ctypes.data,
+                                       ctypes.byref(h)), "tfs_gradient_create")
+        self._h = h
+
+    def matrix(self):
+        """G(1:6, c) as an (n_cells, 6) host array."""
+        g = np.zeros((self.n, 6))
+        _check(self._lib.tfs_gradient_matrix_get(self._h, g), "tfs_gradient_matrix_get")
+        return g
+
+    def _args(self, phi, px, py, pz):
+        return (_dptr(phi, self.n + self.nb, "phi"), _dptr(px, self.n, "px"),
+                _dptr(py, self.n, "py"), _dptr(pz, self.n, "pz"))
+
+    def grad_phi(self, phi, px, py, pz):
+        _sync()
+        _check(self._lib.tfs_grad_phi_device(self._h, *self._args(phi, px, py, pz)),
+               "tfs_grad_phi_device")
+
+    def grad_p(self, phi, px, py, pz):
+        _sync()
+        _check(self._lib.tfs_grad_p_device(self._h, *self._args(phi, px, py, pz)),
+               "tfs_grad_p_device")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.tfs_gradient_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -12,6 +12,11 @@ non-orthogonal backstep).  This script then keeps, per mesh:
     Scoef, and of Matrix_Mod_Topology's row/col/dia and
     Pressure_Matrix_Fractional's val at time step DT (little-endian int32 /
     float64 bytes, 1-based indices as the reference holds them);
+  * in mesh_reference.json, "gradients": the SHA-256 of the reference's
+    Compute_Gradient_Matrix G, of GraPhi's three components and of GradP's
+    (phi, px, py, pz) for the seeded field and boundary kinds of
+    tests/mesh_fixtures.grad_inputs (every boundary a wall for "walls", mixed kinds for
+    "mixed");
   * in <name>_cg_p2.npz: the reference's Cg (prec 2) on that pressure matrix
     with cell 1 pinned (a_11 doubled) and the seeded right-hand side of
     tflows_amd.mesh.mesh_case, to tol 1e-10 ||b||: x, niter, ini_res, fin_res.
@@ -37,12 +42,12 @@ from conftest import load_package  # noqa: E402
 load_package()
 from tflows_amd import mesh  # noqa: E402
 from oracle.oracle import Reference  # noqa: E402
+from mesh_fixtures import SEED, grad_inputs  # noqa: E402
 
 OUT = HERE / "mesh"
 GEN = ROOT / "oracle" / "_ref" / "gen"
 MESHES = ["cavity", "backstep_nonorthogonal"]
 DT = 0.013
-SEED = 20261202
 
 
 def sha(a):
@@ -65,6 +70,13 @@ def main():
         r = ref.mesh(stem, dt=DT)
         entry = {k: r[k] for k in ("n_cells", "n_bnd_cells", "n_faces", "nnz")}
         entry["sha256"] = {key: sha(r[key]) for key, _ in Reference.MESH_ARRAYS.values()}
+        nc, nb = r["n_cells"], r["n_bnd_cells"]
+        grads = {"G": sha(ref.mesh_grad(0, nc, nb))}
+        for tag, mixed in (("walls", False), ("mixed", True)):
+            phi, bc = grad_inputs(nc, nb, mixed)
+            grads[f"graphi_{tag}"] = [sha(a) for a in ref.mesh_grad(1, nc, nb, phi, bc)]
+            grads[f"gradp_{tag}"] = [sha(a) for a in ref.mesh_grad(2, nc, nb, phi, bc)]
+        entry["gradients"] = grads
         meta["meshes"][name] = entry
 
         # the pinned pressure system, solved by the reference

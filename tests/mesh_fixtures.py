@@ -1,0 +1,42 @@
+"""Shared by the mesh tests and tests/golden/make_mesh_golden.py: the
+committed reference meshes and the seeded inputs of the gradient checks."""
+import gzip
+import json
+import pathlib
+import shutil
+
+import numpy as np
+
+from tflows_amd import mesh
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden" / "mesh"
+SEED = 20261202
+
+
+def meta():
+    return json.loads((GOLDEN / "mesh_reference.json").read_text())
+
+
+def unpack(dest):
+    """Decompresses the fixture meshes into dest; returns their names."""
+    names = list(meta()["meshes"])
+    for name in names:
+        for ext in (".cns", ".geo"):
+            with gzip.open(GOLDEN / (name + ext + ".gz"), "rb") as src, \
+                    open(pathlib.Path(dest) / (name + ext), "wb") as dst:
+                shutil.copyfileobj(src, dst)
+    return names
+
+
+def grad_inputs(n_cells, n_bnd, mixed):
+    """A seeded field on cells and boundary cells (uniform in [-1, 1)) and
+    boundary kinds (0 wall, 1 symmetry, 2 pressure) from the SplitMix64
+    stream of tflows_amd.mesh, so the inputs are exact on any machine."""
+    lib = mesh._lib()
+    phi = np.array([2.0 * lib.tfs_rng_u01(SEED, 0x47524144, i) - 1.0
+                    for i in range(n_cells + n_bnd)])
+    if not mixed:
+        return phi, np.zeros(n_bnd, np.int8)
+    bc = np.array([int(3.0 * lib.tfs_rng_u01(SEED, 0x4B494E44, i)) for i in range(n_bnd)],
+                  np.int8)
+    return phi, bc

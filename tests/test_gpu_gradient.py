@@ -1,0 +1,111 @@
+"""Least-squares gradients on the GPU (tfs_gradient_*: Compute_Gradient_Matrix,
+GraPhi, GradP) bitwise against the oracle -- itself bitwise equal to the
+reference's routines -- on the reference generator's meshes, for walls and
+for mixed wall / symmetry / pressure boundaries; the reference's own digests
+of the same computation; a linear field's gradient; and the refusals
+(a boundary cell on two faces, an unknown boundary kind, GradP without
+cell centres)."""
+import hashlib
+
+import numpy as np
+import pytest
+
+from mesh_fixtures import grad_inputs, meta, unpack
+from tflows_amd import mesh
+
+pytestmark = pytest.mark.gpu
+
+META = meta()
+NAMES = list(META["meshes"])
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    if not t.cuda.is_available():
+        pytest.fail("no GPU visible")
+    return t
+
+
+@pytest.fixture(scope="module")
+def S():
+    from tflows_amd import solvers
+    return solvers
+
+
+@pytest.fixture(scope="module")
+def meshes(tmp_path_factory):
+    d = tmp_path_factory.mktemp("gradgpu")
+    out = {}
+    for name in unpack(d):
+        cns = mesh.read_cns(d / (name + ".cns"))
+        geo = mesh.read_geo(d / (name + ".geo"), cns)
+        _, dv = mesh.face_coef(cns, geo)
+        out[name] = (cns, geo, dv)
+    return out
+
+
+def sha(a):
+    a = np.ascontiguousarray(a)
+    return hashlib.sha256(a.astype(a.dtype.newbyteorder("<"), copy=False).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("name", NAMES)
+@pytest.mark.parametrize("mixed", [False, True])
+def test_gradients_bitwise(torch, S, oracle, meshes, name, mixed):
+    cns, geo, d = meshes[name]
+    nc, nb, faces = cns["n_cells"], cns["n_bnd_cells"], cns["faces_c"]
+    print(f"gradients {name} n={nc} mixed={mixed}", flush=True)
+    phi, bc = grad_inputs(nc, nb, mixed)
+    G = S.Gradient(nc, nb, faces, d, geo["xc"], bc)
+    g_or = oracle.gradient_matrix(nc, faces, d)
+    assert np.array_equal(G.matrix(), g_or)
+    want = META["meshes"][name]["gradients"]
+    assert sha(G.matrix()) == want["G"]
+    tag = "mixed" if mixed else "walls"
+    dphi = torch.from_numpy(phi).cuda()
+    p = [torch.zeros(nc, dtype=torch.float64, device="cuda") for _ in range(3)]
+    G.grad_phi(dphi, *p)
+    got = [t.cpu().numpy() for t in p]
+    for a, b in zip(got, oracle.graphi(nc, faces, d, g_or, phi, bc)):
+        assert np.array_equal(a, b)
+    assert [sha(a) for a in got] == want[f"graphi_{tag}"]
+    G.grad_p(dphi, *p)
+    got = [dphi.cpu().numpy()] + [t.cpu().numpy() for t in p]
+    for a, b in zip(got, oracle.gradp(nc, faces, d, g_or, geo["xc"], phi, bc)):
+        assert np.array_equal(a, b)
+    assert [sha(a) for a in got] == want[f"gradp_{tag}"]
+
+
+def test_linear_field(torch, S, meshes):
+    cns, geo, d = meshes["backstep_nonorthogonal"]
+    nc, nb = cns["n_cells"], cns["n_bnd_cells"]
+    G = S.Gradient(nc, nb, cns["faces_c"], d)
+    a = np.array([-0.4, 1.9, 0.25])
+    phi = torch.from_numpy(a @ geo["xc"]).cuda()
+    p = [torch.zeros(nc, dtype=torch.float64, device="cuda") for _ in range(3)]
+    G.grad_phi(phi, *p)
+    f = cns["faces_c"].reshape(-1, 2)
+    per = np.abs(geo["shift"]).sum(0) > 0
+    skip = np.zeros(nc, bool)
+    skip[f[per, 0] - 1] = True
+    skip[f[per & (f[:, 1] > 0), 1] - 1] = True
+    for k in range(3):
+        assert np.abs(p[k].cpu().numpy()[~skip] - a[k]).max() < 1e-9 * np.abs(a).max()
+
+
+def test_refusals(torch, S, meshes):
+    cns, geo, d = meshes["cavity"]
+    nc, nb = cns["n_cells"], cns["n_bnd_cells"]
+    faces = cns["faces_c"].copy()
+    bnd = np.flatnonzero(faces[1::2] < 0)
+    faces[2 * bnd[1] + 1] = faces[2 * bnd[0] + 1]        # one boundary cell on two faces
+    with pytest.raises(S.SolverError):
+        S.Gradient(nc, nb, faces, d)
+    with pytest.raises(S.SolverError):
+        S.Gradient(nc, nb, cns["faces_c"], d, bc=np.full(nb, 7, np.uint8))
+    G = S.Gradient(nc, nb, cns["faces_c"], d)            # no centres: GradP refused
+    phi = torch.zeros(nc + nb, dtype=torch.float64, device="cuda")
+    p = [torch.zeros(nc, dtype=torch.float64, device="cuda") for _ in range(3)]
+    with pytest.raises(S.SolverError):
+        G.grad_p(phi, *p)
