@@ -26,7 +26,8 @@ Workloads (sizes and seeds in t-flows_amd/cases.py):
 
 With --ingest it measures the mesh-side device work instead: the topology
 built on the device from the face list of the workload-2 mesh (cells/s), the
-device assembly of its momentum matrix and the least-squares gradients.
+device assembly of its momentum matrix, the least-squares gradients and the
+velocity correction.
 
 One JSON line on stdout; progress goes to stderr.
 """
@@ -674,12 +675,24 @@ def ingest(args):
     p_el, _, _ = timed(lambda: grad.grad_p(phi, *comps), args.steps, args.warmup,
                        torch.cuda.synchronize)
     g_ms, p_ms = 1e3 * g_el / args.steps, 1e3 * p_el / args.steps
+    # Correct_Velocity on the same faces (seeded fields; parity is tested on
+    # the reference's meshes)
+    vel = [torch.from_numpy(rng.standard_normal(c.n + nbc)).cuda() for _ in range(3)]
+    cells = [torch.from_numpy(rng.uniform(0.5, 1.5, c.n)).cuda() for _ in range(3)]
+    fcoef = torch.from_numpy(rng.uniform(-1.0, 0.0, nf)).cuda()
+    fsc = torch.from_numpy(rng.uniform(0.5, 1.5, nf)).cuda()
+    flux = torch.from_numpy(rng.standard_normal(nf)).cuda()
+    bm = torch.empty(c.n, dtype=torch.float64, device="cuda")
+    c_el, _, _ = timed(lambda: grad.correct_velocity(cells[0], cells[1], *comps, *vel, phi, fcoef,
+                                                     cells[2], fsc, 0.01, 1e-3, flux, bm),
+                       args.steps, args.warmup, torch.cuda.synchronize)
+    c_ms = 1e3 * c_el / args.steps
     n_inc = (nnz - c.n) + nbc                     # interior faces twice, boundary once
     # per incidence: its tag, the face's cells, d, two gathered phi; per cell:
     # the list bounds, G, three components out
     g_alg = n_inc * (8 + 8 + 24 + 16) + c.n * (4 + 48 + 24)
     log(f"  gradient (GraPhi, 3 components) {g_ms:.3f} ms ({g_alg / g_ms / 1e6:.0f} GB/s "
-        f"algorithmic); GradP {p_ms:.3f} ms")
+        f"algorithmic); GradP {p_ms:.3f} ms; Correct_Velocity {c_ms:.3f} ms")
     achieved = alg / (ms * 1e-3) / 1e9
     out = {
         "metric": "mesh topology cells/s (device Matrix_Mod_Topology)", "value": value,
@@ -695,7 +708,7 @@ def ingest(args):
         "assembly": {"ms": a_ms, "algorithmic_GBs": a_alg / a_ms / 1e6,
                      "values_upload_ms_at_50GBs": upload_ms},
         "gradient": {"graphi_ms": g_ms, "graphi_algorithmic_GBs": g_alg / g_ms / 1e6,
-                     "gradp_ms": p_ms},
+                     "gradp_ms": p_ms, "correct_velocity_ms": c_ms},
         "cpu_baseline": cpu,
     }
     return out

@@ -295,6 +295,26 @@ int tfs_grad_phi_device(tfs_gradient *S, const double *d_phi, double *d_px, doub
                         double *d_pz);
 int tfs_grad_p_device(tfs_gradient *S, double *d_phi, double *d_px, double *d_py, double *d_pz);
 
+/* Correct_Velocity (Sources/Process/Correct_Velocity.f90:34-135) on the mesh of
+ * a tfs_gradient object (its faces, face vectors and boundary kinds), for one
+ * material: u, v, w (cells then boundary cells) -= (px, py, pz) vol / sav;
+ * PRESSURE boundary cells take the inside velocity; flux(s) += (pp(c2) -
+ * pp(c1)) coef(s) on interior faces (bou of the boundary cell on BUFFER
+ * faces; d_bou NULL = 0); b(c) = (incoming - outgoing fluxes, summed in face
+ * order) / (vol den); *errmax = max |b|; *cfl_max, *pe_max = the maxima of
+ * |dt flux / (Scoef |d|^2)| and |flux / Scoef / (visc + TINY)| over the
+ * interior and BUFFER faces with d_in_m[s] != 0 (d_in_m NULL = all faces).
+ * coef(s) = A % val(A % pos(1,s)); den(c) = DENc(material(c)).  Per cell:
+ * vol, sav, px, py, pz, den, b; per face: coef, scoef, flux, in_m (bytes).
+ * Bitwise equal to the reference for the arrays and errmax. */
+int tfs_correct_velocity_device(tfs_gradient *S, const double *d_vol, const double *d_sav,
+                                const double *d_px, const double *d_py, const double *d_pz,
+                                double *d_u, double *d_v, double *d_w, const double *d_pp,
+                                const double *d_coef, const double *d_bou, const double *d_den,
+                                const unsigned char *d_in_m, const double *d_scoef, double dt,
+                                double visc, double *d_flux, double *d_b, double *errmax,
+                                double *cfl_max, double *pe_max);
+
 /* ---- serving: continuous batching of solves on one GPU ------------------------
  * Not in the reference.  A case (one system) is registered with its structure
  * and values; its GPU copy is made when its first solve is admitted and stays

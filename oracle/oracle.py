@@ -67,7 +67,32 @@ class Oracle:
                                   ctypes.c_int, _VP, _F64P]
         lib.or_gradp.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _F64P, _F64P, _F64P,
                                  ctypes.c_int, _VP, _F64P, _F64P, _F64P, _F64P]
+        lib.or_correct_velocity.restype = ctypes.c_double
+        lib.or_correct_velocity.argtypes = [ctypes.c_int, ctypes.c_int, _I32P] + [_VP] * 16 + \
+            [ctypes.c_double, ctypes.c_double, _VP, _VP, _VP, _VP]
         self.lib = lib
+
+    def correct_velocity(self, n_cells, faces_c, vol, sav, px, py, pz, u, v, w, pp, coef, scoef,
+                         d, dt, visc, flux, den, bc=None, bou=None, in_m=None):
+        """Correct_Velocity: returns dict(u, v, w, flux, b, errmax, cfl, pe);
+        the inputs are not modified."""
+        faces_c = _c32(faces_c)
+        nf = faces_c.size // 2
+        u, v, w, flux = (_c64(a).copy() for a in (u, v, w, flux))
+        keep = [_c64(a) for a in (vol, sav, px, py, pz, pp, coef, den, scoef)] + [_c64(d).ravel()]
+        bcb = None if bc is None else np.ascontiguousarray(bc, np.uint8)
+        bo = None if bou is None else _c64(bou)
+        im = None if in_m is None else np.ascontiguousarray(in_m, np.uint8)
+        b = np.zeros(n_cells)
+        cfl, pe = ctypes.c_double(), ctypes.c_double()
+        ptr = lambda a: None if a is None else a.ctypes.data  # noqa: E731
+        vol_, sav_, px_, py_, pz_, pp_, coef_, den_, scoef_, d_ = keep
+        err = self.lib.or_correct_velocity(
+            n_cells, nf, faces_c, ptr(vol_), ptr(sav_), ptr(px_), ptr(py_), ptr(pz_), ptr(u),
+            ptr(v), ptr(w), ptr(pp_), ptr(coef_), ptr(bo), ptr(bcb), ptr(den_), ptr(im),
+            ptr(scoef_), ptr(d_), float(dt), float(visc), ptr(flux), ptr(b),
+            ctypes.addressof(cfl), ctypes.addressof(pe))
+        return dict(u=u, v=v, w=w, flux=flux, b=b, errmax=err, cfl=cfl.value, pe=pe.value)
 
     # ---- least-squares gradients: d (3, n_faces); g (n_cells, 6) = G(1:6, c)
     def gradient_matrix(self, n_cells, faces_c, d):
@@ -203,6 +228,9 @@ class Reference:
                                               _I32P]
             lib.tfs_ref_mesh_grad.restype = ctypes.c_int
             lib.tfs_ref_mesh_grad.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 6
+            lib.tfs_ref_mesh_correct.restype = ctypes.c_int
+            lib.tfs_ref_mesh_correct.argtypes = [ctypes.c_void_p] * 10 + \
+                [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
             lib.tfs_ref_mesh_get.restype = ctypes.c_int
             lib.tfs_ref_mesh_get.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                              ctypes.c_void_p]
@@ -317,3 +345,21 @@ class Reference:
         if rc != 0:
             raise RuntimeError(f"tfs_ref_mesh_grad({mode}) returned {rc}")
         return g if mode == 0 else (px, py, pz) if mode == 1 else (f, px, py, pz)
+
+    def mesh_correct(self, n_cells, u, v, w, px, py, pz, pp, sav, flux, den, visc, bc=None):
+        """Correct_Velocity on the grid and pressure matrix of the last mesh()
+        call (material 1 everywhere): dict(u, v, w, flux, b, errmax, cfl, pe),
+        cfl and pe to the four digits the reference prints."""
+        n_bnd = len(u) - n_cells
+        bcb = np.zeros(max(n_bnd, 1), np.int8) if bc is None else np.ascontiguousarray(bc, np.int8)
+        u, v, w, flux = (_c64(a).copy() for a in (u, v, w, flux))
+        keep = [_c64(a) for a in (px, py, pz, pp, sav)]
+        b = np.zeros(n_cells)
+        res = np.zeros(3)
+        rc = self.lib.tfs_ref_mesh_correct(bcb.ctypes.data, u.ctypes.data, v.ctypes.data,
+                                           w.ctypes.data, *(a.ctypes.data for a in keep),
+                                           flux.ctypes.data, float(den), float(visc),
+                                           b.ctypes.data, res.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"tfs_ref_mesh_correct returned {rc}")
+        return dict(u=u, v=v, w=w, flux=flux, b=b, errmax=res[0], cfl=res[1], pe=res[2])

@@ -24,6 +24,8 @@
   use pro_mod
   use Grid_Mod
   use Matrix_Mod
+  use Info_Mod
+  use Parameters_Mod, only: FRACT
   implicit none
 
   integer :: m_nc = 0, m_nb = 0, m_nf = 0, m_nnz = 0
@@ -195,6 +197,66 @@
     phi(nc + c) = f(-c)
   end do
   tfs_ref_mesh_grad = 0
+  end function
+
+  !---------------------------------------------------------------------------!
+  !   Correct_Velocity on the grid and pressure matrix last loaded (material  !
+  !   1 everywhere, density den, viscosity visc, time step as loaded).        !
+  !   u, v, w, pp: cells then boundary cells; px, py, pz, sav: cells; flux:   !
+  !   faces; bc as for tfs_ref_mesh_grad.  u, v, w, flux updated in place;    !
+  !   b(nc) <- the mass error; res <- errmax, and the CFL and Peclet maxima    !
+  !   as the reference reports them (read back from its 1pe9.3 text).         !
+  !---------------------------------------------------------------------------!
+  integer(c_int) function tfs_ref_mesh_correct(bc, cu, cv, cw, gx, gy, gz, cpp, csav, &
+                                               cflux, den, cvisc, bout, res)     &
+                 bind(C, name='tfs_ref_mesh_correct')
+  integer(c_signed_char), intent(in) :: bc(*)
+  real(c_double), intent(inout) :: cu(*), cv(*), cw(*), cflux(*)
+  real(c_double), intent(in)    :: gx(*), gy(*), gz(*), cpp(*), csav(*)
+  real(c_double), value         :: den, cvisc
+  real(c_double), intent(out)   :: bout(*), res(3)
+  real, external :: Correct_Velocity
+  integer :: c, nc, nb, nf
+  integer, parameter :: kinds(0:3) = (/WALL, SYMMETRY, PRESSURE, BUFFER/)
+
+  tfs_ref_mesh_correct = -1
+  if(.not. allocated(mg)) return
+  nc = m_nc;  nb = m_nb;  nf = m_nf
+  do c = 1, nb
+    TypeBC(-c) = kinds(bc(c))
+  end do
+  material = 1
+  DENc(1) = den
+  VISc    = cvisc
+  ALGOR   = FRACT
+  if(allocated(U % n)) deallocate(U % n, V % n, W % n, PP % n, P % x, P % y, P % z)
+  if(allocated(Flux))  deallocate(Flux)
+  if(allocated(b))     deallocate(b)
+  allocate(U % n(-nb:nc), V % n(-nb:nc), W % n(-nb:nc), PP % n(-nb:nc))
+  allocate(P % x(-nb:nc), P % y(-nb:nc), P % z(-nb:nc), Flux(nf), b(-nb:nc))
+  P % x = 0.0;  P % y = 0.0;  P % z = 0.0;  b = 0.0
+  U % n(1:nc) = cu(1:nc);  V % n(1:nc) = cv(1:nc);  W % n(1:nc) = cw(1:nc)
+  PP % n(1:nc) = cpp(1:nc)
+  do c = 1, nb
+    U % n(-c) = cu(nc+c);  V % n(-c) = cv(nc+c);  W % n(-c) = cw(nc+c)
+    PP % n(-c) = cpp(nc+c)
+  end do
+  P % x(1:nc) = gx(1:nc);  P % y(1:nc) = gy(1:nc);  P % z(1:nc) = gz(1:nc)
+  A % sav(1:nc) = csav(1:nc)
+  A % bou = 0.0
+  Flux(1:nf) = cflux(1:nf)
+
+  res(1) = Correct_Velocity(mg)
+  read(bulk_info % lines(1)(29:37), *) res(2)
+  read(bulk_info % lines(1)(68:76), *) res(3)
+
+  cu(1:nc) = U % n(1:nc);  cv(1:nc) = V % n(1:nc);  cw(1:nc) = W % n(1:nc)
+  do c = 1, nb
+    cu(nc+c) = U % n(-c);  cv(nc+c) = V % n(-c);  cw(nc+c) = W % n(-c)
+  end do
+  cflux(1:nf) = Flux(1:nf)
+  bout(1:nc) = b(1:nc)
+  tfs_ref_mesh_correct = 0
   end function
 
   !---------------------------------------------------------------------------!

@@ -91,6 +91,21 @@ void or_gradp(int n_cells, int n_faces, const int *faces_c, const double *d, con
               const double *xc, int n_bnd, const unsigned char *bc, double *phi, double *px,
               double *py, double *pz);
 
+/* Correct_Velocity (Sources/Process/Correct_Velocity.f90:34-135) for one
+ * material: vol, sav, px/py/pz, den per cell (den = DENc(material(c)));
+ * u, v, w, pp on cells and boundary cells; coef per face = A % val(A %
+ * pos(1,s)); bou per boundary cell (A % bou; NULL = zeros); in_m per face
+ * (material(c1) or material(c2) is the material reported; NULL = every
+ * face).  Updates u, v, w, flux; writes b (the mass error per cell);
+ * returns errmax, with the CFL and Peclet maxima in *cfl, *pe. */
+double or_correct_velocity(int n_cells, int n_faces, const int *faces_c, const double *vol,
+                           const double *sav, const double *px, const double *py,
+                           const double *pz, double *u, double *v, double *w, const double *pp,
+                           const double *coef, const double *bou, const unsigned char *bc,
+                           const double *den, const unsigned char *in_m, const double *scoef,
+                           const double *d, double dt, double visc, double *flux, double *b,
+                           double *cfl, double *pe);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,6 +21,14 @@
 // passes over the same data, one per component; each component's sum is
 // unchanged).  Bound by HBM: per incidence, the face's cells, its d and two
 // gathered phi values; per cell, G and the three components.
+//
+// The same mesh object serves Correct_Velocity
+// (Sources/Process/Correct_Velocity.f90:34-135): velocities corrected by the
+// pressure gradient, PRESSURE boundary cells given the inside velocity, face
+// fluxes corrected by the pressure-correction difference, the mass error per
+// cell (each cell sums its faces' fluxes in face order, as the reference's
+// loop does) and the maxima of |mass error|, CFL and Peclet numbers (maxima
+// are exact in any order).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -130,6 +138,113 @@ k_bnd_extrap(long long m, long long na, const int2 *__restrict__ bf, const doubl
   }
 }
 
+// ---- Correct_Velocity -------------------------------------------------------
+
+constexpr int kParts = 8192;   // the largest grid of grid_of(): one partial per block
+
+// max over the workgroup of v, written by thread 0 to part[blockIdx.x]
+__device__ void block_max(double v, double *__restrict__ part) {
+  __shared__ double wmax[kThreads / 64];
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmax[0];
+    for (int w = 1; w < kThreads / 64; ++w) m = fmax(m, wmax[w]);
+    part[blockIdx.x] = m;
+  }
+  __syncthreads();
+}
+
+// Correct_Velocity.f90:34-46
+__global__ void __launch_bounds__(kThreads)
+k_cv_cells(int n, const double *__restrict__ vol, const double *__restrict__ sav,
+           const double *__restrict__ px, const double *__restrict__ py,
+           const double *__restrict__ pz, double *__restrict__ u, double *__restrict__ v,
+           double *__restrict__ w) {
+  for (int c = blockIdx.x * kThreads + threadIdx.x; c < n; c += gridDim.x * kThreads) {
+    const double vc = vol[c], sc = sav[c];
+    u[c] = u[c] - px[c] * vc / sc;
+    v[c] = v[c] - py[c] * vc / sc;
+    w[c] = w[c] - pz[c] * vc / sc;
+  }
+}
+
+// Correct_Velocity.f90:48-59 (PRESSURE boundaries), :72-82 (flux correction),
+// :117-135 (CFL and Peclet numbers of the corrected fluxes); per-block maxima
+// to part_cfl / part_pe
+__global__ void __launch_bounds__(kThreads)
+k_cv_faces(int n, long long nf, const int2 *__restrict__ faces, const unsigned char *__restrict__ kind,
+           const double *__restrict__ pp, const double *__restrict__ coef,
+           const double *__restrict__ bou, const unsigned char *__restrict__ in_m,
+           const double *__restrict__ scoef, const double *__restrict__ d, double dt,
+           double visc, double *__restrict__ flux, double *__restrict__ u,
+           double *__restrict__ v, double *__restrict__ w, double *__restrict__ part_cfl,
+           double *__restrict__ part_pe) {
+  double cm = 0.0, pm = 0.0;
+  for (long long s = blockIdx.x * (long long)kThreads + threadIdx.x; s < nf;
+       s += (long long)gridDim.x * kThreads) {
+    const int2 f = faces[s];
+    const int k1 = f.x - 1;
+    const int kb = f.y < 0 ? -f.y - 1 : 0;
+    const int kd = f.y < 0 ? kind[kb] : TFS_BC_WALL;
+    if (f.y < 0 && kd == TFS_BC_PRESSURE) {
+      const long long k2 = (long long)n + kb;
+      u[k2] = u[k1];
+      v[k2] = v[k1];
+      w[k2] = w[k1];
+    }
+    if (f.y > 0 || kd == TFS_BC_BUFFER) {
+      const long long k2 = f.y > 0 ? (long long)(f.y - 1) : (long long)n + kb;
+      const double a = f.y > 0 ? coef[s] : (bou ? bou[kb] : 0.0);
+      const double fl = flux[s] + (pp[k2] - pp[k1]) * a;
+      flux[s] = fl;
+      if (!in_m || in_m[s]) {
+        const double x = d[s], y = d[nf + s], z = d[2 * nf + s];
+        cm = fmax(cm, fabs(dt * fl / (scoef[s] * (x * x + y * y + z * z))));
+        pm = fmax(pm, fabs(fl / scoef[s] / (visc + kTiny)));
+      }
+    }
+  }
+  block_max(cm, part_cfl);
+  block_max(pm, part_pe);
+}
+
+// Correct_Velocity.f90:88-111: b(c) = -sum of outgoing fluxes + incoming, in
+// face order, over vol * density; per-block maxima of |b| to part_err
+__global__ void __launch_bounds__(kThreads)
+k_cv_mass(int n, const int *__restrict__ ptr, const long long *__restrict__ inc,
+          const double *__restrict__ flux, const double *__restrict__ vol,
+          const double *__restrict__ den, double *__restrict__ b, double *__restrict__ part_err) {
+  double em = 0.0;
+  for (int c = blockIdx.x * kThreads + threadIdx.x; c < n; c += gridDim.x * kThreads) {
+    double m = 0.0;
+    for (int k = ptr[c]; k < ptr[c + 1]; ++k) {
+      const long long e = inc[k];
+      const double fl = flux[e >> 1];
+      m = (e & 1) ? m + fl : m - fl;
+    }
+    m = m / (vol[c] * den[c]);
+    b[c] = m;
+    em = fmax(em, fabs(m));
+  }
+  block_max(em, part_err);
+}
+
+// the three maxima over the per-block partials
+__global__ void __launch_bounds__(kThreads)
+k_cv_final(int nf_parts, int n_parts, const double *__restrict__ part, double *__restrict__ out) {
+  double e = 0.0, cm = 0.0, pm = 0.0;
+  for (int i = threadIdx.x; i < n_parts; i += kThreads) e = fmax(e, part[i]);
+  for (int i = threadIdx.x; i < nf_parts; i += kThreads) {
+    cm = fmax(cm, part[kParts + i]);
+    pm = fmax(pm, part[2 * kParts + i]);
+  }
+  block_max(e, out);
+  block_max(cm, out + 1);
+  block_max(pm, out + 2);
+}
+
 }  // namespace
 
 struct tfs_gradient {
@@ -139,13 +254,15 @@ struct tfs_gradient {
   long long *inc = nullptr;
   int2 *faces = nullptr, *bf = nullptr;
   double *d = nullptr, *g = nullptr, *xc = nullptr;
-  unsigned char *sym = nullptr;
+  unsigned char *sym = nullptr, *kind = nullptr;
+  double *part = nullptr;     // 3 kParts partial maxima + 3 results
 };
 
 extern "C" int tfs_gradient_destroy(tfs_gradient *S) {
   if (!S) return TFS_OK;
   for (void *p : {(void *)S->ptr, (void *)S->inc, (void *)S->faces, (void *)S->bf, (void *)S->d,
-                  (void *)S->g, (void *)S->xc, (void *)S->sym})
+                  (void *)S->g, (void *)S->xc, (void *)S->sym, (void *)S->kind,
+                  (void *)S->part})
     if (p) (void)hipFree(p);
   delete S;
   return TFS_OK;
@@ -161,7 +278,7 @@ extern "C" int tfs_gradient_create(int n_cells, int n_bnd_cells, long long n_fac
   const int n = n_cells, nb = n_bnd_cells;
   const long long nf = n_faces;
   std::vector<int> cnt((size_t)n + 1, 0), seen((size_t)nb, 0);
-  std::vector<unsigned char> sym((size_t)std::max(nb, 1), 0);
+  std::vector<unsigned char> sym((size_t)std::max(nb, 1), 0), kinds((size_t)std::max(nb, 1), 0);
   std::vector<int2> bf;
   for (long long s = 0; s < nf; ++s) {
     const int c1 = faces_c[2 * s], c2 = faces_c[2 * s + 1];
@@ -174,6 +291,7 @@ extern "C" int tfs_gradient_create(int n_cells, int n_bnd_cells, long long n_fac
       if (seen[(size_t)(-c2 - 1)]++) return TFS_ERR_INVALID;
       const int kind = bc ? bc[-c2 - 1] : TFS_BC_WALL;
       if (kind > TFS_BC_BUFFER) return TFS_ERR_INVALID;
+      kinds[(size_t)(-c2 - 1)] = (unsigned char)kind;
       if (kind == TFS_BC_SYMMETRY) sym[(size_t)(-c2 - 1)] = 1;
       if (kind != TFS_BC_BUFFER && kind != TFS_BC_PRESSURE)
         bf.push_back(make_int2(c1 - 1, n + (-c2 - 1)));
@@ -206,6 +324,8 @@ extern "C" int tfs_gradient_create(int n_cells, int n_bnd_cells, long long n_fac
             up((void **)&S->d, d, 3 * sizeof(double) * (size_t)nf) &&
             up((void **)&S->g, nullptr, 6 * sizeof(double) * (size_t)n) &&
             up((void **)&S->sym, sym.data(), sym.size()) &&
+            up((void **)&S->kind, kinds.data(), kinds.size()) &&
+            up((void **)&S->part, nullptr, sizeof(double) * (3 * (size_t)kParts + 3)) &&
             (!xc || up((void **)&S->xc, xc, 3 * sizeof(double) * na));
   if (!ok) {
     (void)hipGetLastError();
@@ -258,5 +378,37 @@ extern "C" int tfs_grad_p_device(tfs_gradient *S, double *d_phi, double *d_px, d
                      S->ptr, S->inc, S->faces, S->d, S->g, S->sym, d_phi, d_px, d_py, d_pz);
   TFS_G_HIP(hipGetLastError());
   TFS_G_HIP(hipStreamSynchronize(nullptr));
+  return TFS_OK;
+}
+
+extern "C" int tfs_correct_velocity_device(tfs_gradient *S, const double *d_vol,
+                                           const double *d_sav, const double *d_px,
+                                           const double *d_py, const double *d_pz, double *d_u,
+                                           double *d_v, double *d_w, const double *d_pp,
+                                           const double *d_coef, const double *d_bou,
+                                           const double *d_den, const unsigned char *d_in_m,
+                                           const double *d_scoef, double dt, double visc,
+                                           double *d_flux, double *d_b, double *errmax,
+                                           double *cfl_max, double *pe_max) {
+  if (!S || !d_vol || !d_sav || !d_px || !d_py || !d_pz || !d_u || !d_v || !d_w || !d_pp ||
+      !d_den || !d_b || (S->nf > 0 && (!d_coef || !d_scoef || !d_flux)))
+    return TFS_ERR_INVALID;
+  const int gc = grid_of(S->n), gf = grid_of(S->nf);
+  double *part = S->part;
+  hipLaunchKernelGGL(k_cv_cells, dim3(gc), dim3(kThreads), 0, nullptr, S->n, d_vol, d_sav, d_px,
+                     d_py, d_pz, d_u, d_v, d_w);
+  hipLaunchKernelGGL(k_cv_faces, dim3(gf), dim3(kThreads), 0, nullptr, S->n, S->nf, S->faces,
+                     S->kind, d_pp, d_coef, d_bou, d_in_m, d_scoef, S->d, dt, visc, d_flux, d_u,
+                     d_v, d_w, part + kParts, part + 2 * kParts);
+  hipLaunchKernelGGL(k_cv_mass, dim3(gc), dim3(kThreads), 0, nullptr, S->n, S->ptr, S->inc,
+                     d_flux, d_vol, d_den, d_b, part);
+  hipLaunchKernelGGL(k_cv_final, dim3(1), dim3(kThreads), 0, nullptr, gf, gc, part,
+                     part + 3 * kParts);
+  TFS_G_HIP(hipGetLastError());
+  double out[3];
+  TFS_G_HIP(hipMemcpy(out, part + 3 * kParts, sizeof(out), hipMemcpyDeviceToHost));
+  if (errmax) *errmax = out[0];
+  if (cfl_max) *cfl_max = out[1];
+  if (pe_max) *pe_max = out[2];
   return TFS_OK;
 }

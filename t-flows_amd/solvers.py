@@ -683,6 +683,8 @@ class Gradient:
             lib.tfs_gradient_matrix_get.argtypes = [_VP, _F64P]
             lib.tfs_grad_phi_device.argtypes = [_VP] + [_VP] * 4
             lib.tfs_grad_p_device.argtypes = [_VP] + [_VP] * 4
+            lib.tfs_correct_velocity_device.argtypes = [_VP] + [_VP] * 14 + \
+                [ctypes.c_double, ctypes.c_double, _VP, _VP, _DP, _DP, _DP]
             lib._tfs_grad_typed = True
         self._lib = lib
         self.n, self.nb = int(n_cells), int(n_bnd_cells)
@@ -716,6 +718,38 @@ class Gradient:
         _sync()
         _check(self._lib.tfs_grad_p_device(self._h, *self._args(phi, px, py, pz)),
                "tfs_grad_p_device")
+
+    def correct_velocity(self, vol, sav, px, py, pz, u, v, w, pp, coef, den, scoef, dt, visc,
+                         flux, b, bou=None, in_m=None):
+        """Correct_Velocity on the GPU: u, v, w (cells and boundary cells) and
+        flux (faces) are updated in place, b (cells) receives the mass error;
+        returns (errmax, cfl_max, pe_max).  coef per face = A % val(A % pos(1, s));
+        den per cell = DENc(material(c)); in_m (uint8 per face) marks the
+        faces of the material whose CFL / Peclet numbers are reported."""
+        n, na = self.n, self.n + self.nb
+        nf = flux.numel()
+
+        def opt(t, m, what, dtype=None):
+            if t is None:
+                return None
+            import torch
+            want = torch.float64 if dtype is None else dtype
+            if not (_is_device_tensor(t) and t.dtype == want and t.is_contiguous()
+                    and t.numel() >= m):
+                raise ValueError(f"{what}: contiguous GPU tensor of {m} entries expected")
+            return t.data_ptr()
+        import torch
+        em, cm, pm = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _sync()
+        _check(self._lib.tfs_correct_velocity_device(
+            self._h, _dptr(vol, n, "vol"), _dptr(sav, n, "sav"), _dptr(px, n, "px"),
+            _dptr(py, n, "py"), _dptr(pz, n, "pz"), _dptr(u, na, "u"), _dptr(v, na, "v"),
+            _dptr(w, na, "w"), _dptr(pp, na, "pp"), _dptr(coef, nf, "coef"),
+            opt(bou, self.nb, "bou"), _dptr(den, n, "den"), opt(in_m, nf, "in_m", torch.uint8),
+            _dptr(scoef, nf, "scoef"), float(dt), float(visc), _dptr(flux, nf, "flux"),
+            _dptr(b, n, "b"), ctypes.byref(em), ctypes.byref(cm), ctypes.byref(pm)),
+            "tfs_correct_velocity_device")
+        return em.value, cm.value, pm.value
 
     def close(self):
         if getattr(self, "_h", None):

@@ -40,3 +40,24 @@ def grad_inputs(n_cells, n_bnd, mixed):
     bc = np.array([int(3.0 * lib.tfs_rng_u01(SEED, 0x4B494E44, i)) for i in range(n_bnd)],
                   np.int8)
     return phi, bc
+
+
+CV_DEN, CV_VISC = 1.3, 1e-3
+
+
+def cv_inputs(n_cells, n_bnd, n_faces):
+    """Seeded inputs of the Correct_Velocity checks: u, v, w, pp on cells and
+    boundary cells and px, py, pz on cells uniform in [-1, 1); sav uniform in
+    [0.5, 1.5); fluxes uniform in [-1, 1); boundary kinds mixed (0 wall,
+    1 symmetry, 2 pressure)."""
+    lib = mesh._lib()
+
+    def stream(tag, m, lo=-1.0, width=2.0):
+        return np.array([lo + width * lib.tfs_rng_u01(SEED, tag, i) for i in range(m)])
+    na = n_cells + n_bnd
+    u, v, w, pp = (stream(0x43560000 + k, na) for k in range(4))
+    px, py, pz = (stream(0x43560010 + k, n_cells) for k in range(3))
+    sav = stream(0x43560020, n_cells, 0.5, 1.0)
+    flux = stream(0x43560030, n_faces)
+    _, bc = grad_inputs(n_cells, n_bnd, True)
+    return dict(u=u, v=v, w=w, pp=pp, px=px, py=py, pz=pz, sav=sav, flux=flux, bc=bc)

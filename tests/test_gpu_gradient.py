@@ -109,3 +109,40 @@ def test_refusals(torch, S, meshes):
     p = [torch.zeros(nc, dtype=torch.float64, device="cuda") for _ in range(3)]
     with pytest.raises(S.SolverError):
         G.grad_p(phi, *p)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_correct_velocity_bitwise(torch, S, oracle, meshes, name):
+    """Correct_Velocity on the GPU against the oracle (every array bitwise,
+    the three maxima exactly) and the reference's digests."""
+    from mesh_fixtures import CV_DEN, CV_VISC, cv_inputs
+    cns, geo, d = meshes[name]
+    nc, nb, nf = cns["n_cells"], cns["n_bnd_cells"], cns["n_faces"]
+    print(f"correct_velocity {name} n={nc}", flush=True)
+    dt = META["dt"]
+    scoef, _ = mesh.face_coef(cns, geo)
+    row, col, dia, pos = mesh.topology(nc, cns["faces_c"], base=1, with_pos=True)
+    val = mesh.pressure_values(nc, cns["faces_c"], pos, dia, len(col), scoef, dt, base=1)
+    coef = np.where(pos[0::2] > 0, val[np.maximum(pos[0::2], 1) - 1], 0.0)
+    cv = cv_inputs(nc, nb, nf)
+    den = np.full(nc, CV_DEN)
+    want = oracle.correct_velocity(nc, cns["faces_c"], geo["vol"], cv["sav"], cv["px"], cv["py"],
+                                   cv["pz"], cv["u"], cv["v"], cv["w"], cv["pp"], coef, scoef, d,
+                                   dt, CV_VISC, cv["flux"], den, cv["bc"])
+    G = S.Gradient(nc, nb, cns["faces_c"], d, geo["xc"], cv["bc"])
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+    u, v, w, flux = dev(cv["u"]), dev(cv["v"]), dev(cv["w"]), dev(cv["flux"])
+    b = torch.zeros(nc, dtype=torch.float64, device="cuda")
+    em, cm, pm = G.correct_velocity(dev(geo["vol"]), dev(cv["sav"]), dev(cv["px"]),
+                                    dev(cv["py"]), dev(cv["pz"]), u, v, w, dev(cv["pp"]),
+                                    dev(coef), dev(den), dev(scoef), dt, CV_VISC, flux, b)
+    got = dict(u=u, v=v, w=w, flux=flux, b=b)
+    ref = META["meshes"][name]["correct_velocity"]
+    for k, t in got.items():
+        a = t.cpu().numpy()
+        assert np.array_equal(a, want[k]), k
+        assert sha(a) == ref[k], k
+    assert em == want["errmax"] == float.fromhex(ref["errmax"])
+    assert cm == want["cfl"] and pm == want["pe"]
