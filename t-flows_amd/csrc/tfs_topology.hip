@@ -235,8 +235,12 @@ extern "C" int tfs_topology_build_device(int n_cells, long long n_faces, const i
   int h_err = 0;
   long long h_total = 0;
   const hipStream_t st = nullptr;    // the calling thread's default stream
-  TFS_T_HIP(hipMalloc(&width, sizeof(int) * (size_t)n_cells));
-  TFS_T_HIP(hipMalloc(&bsum, sizeof(long long) * (size_t)nb + 2 * sizeof(long long)));
+  if (hipMalloc(&width, sizeof(int) * (size_t)n_cells) != hipSuccess ||
+      hipMalloc(&bsum, sizeof(long long) * (size_t)nb + 2 * sizeof(long long)) != hipSuccess) {
+    (void)hipGetLastError();
+    rc = TFS_ERR_NOMEM;
+    goto done;
+  }
   total = bsum + nb;
   err = reinterpret_cast<int *>(bsum + nb + 1);
   hipLaunchKernelGGL(k_fill_int, dim3(grid_of(n_cells)), dim3(kThreads), 0, st,
@@ -260,7 +264,11 @@ extern "C" int tfs_topology_build_device(int n_cells, long long n_faces, const i
     goto done;
   }
   if (d_pos) {
-    TFS_T_HIP(hipMalloc(&tag, sizeof(long long) * (size_t)h_total));
+    if (hipMalloc(&tag, sizeof(long long) * (size_t)h_total) != hipSuccess) {
+      (void)hipGetLastError();
+      rc = TFS_ERR_NOMEM;
+      goto done;
+    }
     // the diagonal entries keep tag -1 (k_scan_rows writes only their column)
     hipLaunchKernelGGL(k_fill_int, dim3(grid_of(2 * h_total)), dim3(kThreads), 0, st,
                        2 * h_total, -1, reinterpret_cast<int *>(tag));
