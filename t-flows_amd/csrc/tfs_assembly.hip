@@ -56,19 +56,23 @@ k_faces(long long nf, const int *__restrict__ c2, const int *__restrict__ pos,
 }
 
 // Diagonals: one thread per cell, its faces in ascending face order.  Entry
-// 2 s + k of the incidence list means "face s, this cell is c(k+1)".
+// 2 s + k of the incidence list means "face s, this cell is c(k+1)"; the
+// byte beside it is set for an interior face, so the face's own cells are
+// not gathered (the list and its bytes are contiguous per cell; the
+// coefficients are the only gathers).
 __global__ void __launch_bounds__(kThreads)
 k_cells(int n, const int *__restrict__ ptr, const long long *__restrict__ inc,
-        const int *__restrict__ c2, const double *__restrict__ a12, const double *__restrict__ a21,
-        const unsigned char *__restrict__ bnd_on, const double *__restrict__ cell, double urf,
-        const int *__restrict__ dia, double *__restrict__ val, double *__restrict__ sav) {
+        const unsigned char *__restrict__ interior, const double *__restrict__ a12,
+        const double *__restrict__ a21, const unsigned char *__restrict__ bnd_on,
+        const double *__restrict__ cell, double urf, const int *__restrict__ dia,
+        double *__restrict__ val, double *__restrict__ sav) {
   for (int c = blockIdx.x * kThreads + threadIdx.x; c < n; c += gridDim.x * kThreads) {
     double d = 0.0;
     for (int k = ptr[c]; k < ptr[c + 1]; ++k) {
       const long long e = inc[k], s = e >> 1;
       if (e & 1) {
         d = d + (a21 ? a21[s] : a12[s]);
-      } else if (c2[s] >= 0 || !bnd_on || bnd_on[s]) {
+      } else if (interior[k] || !bnd_on || bnd_on[s]) {
         d = d + a12[s];
       }
     }
@@ -95,11 +99,13 @@ struct tfs_assembly {
   long long nf = 0, nnz = 0;
   int *c2 = nullptr, *pos = nullptr, *dia = nullptr, *ptr = nullptr;
   long long *inc = nullptr;
+  unsigned char *interior = nullptr;   // per incidence: the face is interior
 };
 
 extern "C" int tfs_assembly_destroy(tfs_assembly *S) {
   if (!S) return TFS_OK;
-  for (void *p : {(void *)S->c2, (void *)S->pos, (void *)S->dia, (void *)S->ptr, (void *)S->inc})
+  for (void *p : {(void *)S->c2, (void *)S->pos, (void *)S->dia, (void *)S->ptr, (void *)S->inc,
+                  (void *)S->interior})
     if (p) (void)hipFree(p);
   delete S;
   return TFS_OK;
@@ -148,6 +154,8 @@ extern "C" int tfs_assembly_create(tfs_matrix *A, long long n_faces, const int *
     hinc[(size_t)fill[(size_t)c1]++] = 2 * s;
     if (c2 >= 0) hinc[(size_t)fill[(size_t)c2]++] = 2 * s + 1;
   }
+  std::vector<unsigned char> hint(hinc.size());
+  for (size_t k = 0; k < hinc.size(); ++k) hint[k] = hc2[(size_t)(hinc[k] >> 1)] >= 0 ? 1 : 0;
   auto *S = new tfs_assembly;
   S->A = A;
   S->n = n;
@@ -161,7 +169,8 @@ extern "C" int tfs_assembly_create(tfs_matrix *A, long long n_faces, const int *
       !up((void **)&S->pos, hpos.data(), sizeof(int) * hpos.size()) ||
       !up((void **)&S->dia, hdia.data(), sizeof(int) * hdia.size()) ||
       !up((void **)&S->ptr, cnt.data(), sizeof(int) * cnt.size()) ||
-      !up((void **)&S->inc, hinc.data(), sizeof(long long) * hinc.size())) {
+      !up((void **)&S->inc, hinc.data(), sizeof(long long) * hinc.size()) ||
+      !up((void **)&S->interior, hint.data(), hint.size())) {
     (void)hipGetLastError();
     tfs_assembly_destroy(S);
     return TFS_ERR_NOMEM;
@@ -182,7 +191,7 @@ extern "C" int tfs_assemble_device(tfs_assembly *S, const double *d_a12, const d
                        d_a12, d_a21, val);
   if (S->n > 0)
     hipLaunchKernelGGL(k_cells, dim3(grid_of(S->n)), dim3(kThreads), 0, st, S->n, S->ptr, S->inc,
-                       S->c2, d_a12, d_a21, d_bnd_on, d_cell, urf, S->dia, val, d_sav);
+                       S->interior, d_a12, d_a21, d_bnd_on, d_cell, urf, S->dia, val, d_sav);
   TFS_A_HIP(hipGetLastError());
   TFS_A_HIP(hipStreamSynchronize(st));
   return tfs_matrix_values_changed(S->A);

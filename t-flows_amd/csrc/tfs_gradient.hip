@@ -19,8 +19,10 @@
 // result is bitwise the same, one thread per cell and no atomics.  The three
 // components of a gradient are formed in one pass (the reference makes three
 // passes over the same data, one per component; each component's sum is
-// unchanged).  Bound by HBM: per incidence, the face's cells, its d and two
-// gathered phi values; per cell, G and the three components.
+// unchanged).  The static mesh data of every incidence (d, the other cell, the
+// side) is laid out in the cells' incidence order once, at creation, so the
+// per-call traffic is contiguous except for one gathered phi per incidence;
+// per cell, G, phi and the three components.
 //
 // The same mesh object serves Correct_Velocity
 // (Sources/Process/Correct_Velocity.f90:34-135): velocities corrected by the
@@ -55,16 +57,16 @@ int grid_of(long long m) {
 }
 
 // G(c): the normal matrix summed over the cell's faces in face order, then its
-// inverse (Compute_Gradient_Matrix.f90:30-70, :96-116).  g holds six blocks of
-// n: G(1..6, c) at g[k n + c].
+// inverse (Compute_Gradient_Matrix.f90:30-70, :96-116).  id = d of every
+// incidence (three blocks of m); g holds six blocks of n: G(1..6, c) at
+// g[k n + c].
 __global__ void __launch_bounds__(kThreads)
-k_gmatrix(int n, long long nf, const int *__restrict__ ptr, const long long *__restrict__ inc,
-          const double *__restrict__ d, double *__restrict__ g) {
+k_gmatrix(int n, long long m, const int *__restrict__ ptr, const double *__restrict__ id,
+          double *__restrict__ g) {
   for (int c = blockIdx.x * kThreads + threadIdx.x; c < n; c += gridDim.x * kThreads) {
     double G0 = 0.0, G1 = 0.0, G2 = 0.0, G3 = 0.0, G4 = 0.0, G5 = 0.0;
     for (int k = ptr[c]; k < ptr[c + 1]; ++k) {
-      const long long s = inc[k] >> 1;
-      const double x = d[s], y = d[nf + s], z = d[2 * nf + s];
+      const double x = id[k], y = id[m + k], z = id[2 * m + k];
       G0 = G0 + x * x;
       G1 = G1 + y * y;
       G2 = G2 + z * z;
@@ -84,25 +86,28 @@ k_gmatrix(int n, long long nf, const int *__restrict__ ptr, const long long *__r
 }
 
 // GraPhi for i = 1, 2, 3 at once (GraPhi.f90:25-73, :75-123, :125-173 with
-// Boundary = .TRUE.).  faces: (c1, c2) 1-based, c2 < 0 a boundary cell at
-// phi[n + (-c2 - 1)]; sym[b] != 0 for a SYMMETRY boundary cell -(b+1).
+// Boundary = .TRUE.).  The mesh data of every incidence -- the face vector d
+// (three blocks of m), the phi slot of the face's other cell, and flags
+// (bit 0: this cell is the face's c2; bit 1: a SYMMETRY boundary face) -- sit
+// in the cells' incidence order, so a wavefront reads them in contiguous
+// runs; only phi of the other cell is gathered.  phi(c2) - phi(c1) is formed
+// from the same two values whichever side evaluates it.
 __global__ void __launch_bounds__(kThreads)
-k_graphi(int n, long long nf, const int *__restrict__ ptr, const long long *__restrict__ inc,
-         const int2 *__restrict__ faces, const double *__restrict__ d,
-         const double *__restrict__ g, const unsigned char *__restrict__ sym,
-         const double *__restrict__ phi, double *__restrict__ px, double *__restrict__ py,
-         double *__restrict__ pz) {
+k_graphi(int n, long long m, const int *__restrict__ ptr, const int *__restrict__ other,
+         const unsigned char *__restrict__ flag, const double *__restrict__ id,
+         const double *__restrict__ g, const double *__restrict__ phi, double *__restrict__ px,
+         double *__restrict__ py, double *__restrict__ pz) {
   for (int c = blockIdx.x * kThreads + threadIdx.x; c < n; c += gridDim.x * kThreads) {
     const double g0 = g[c], g1 = g[n + c], g2 = g[2LL * n + c], g3 = g[3LL * n + c],
                  g4 = g[4LL * n + c], g5 = g[5LL * n + c];
+    const double pc = phi[c];
     double ax = 0.0, ay = 0.0, az = 0.0;
     for (int k = ptr[c]; k < ptr[c + 1]; ++k) {
-      const long long s = inc[k] >> 1;
-      const int2 f = faces[s];
-      const long long k2 = f.y > 0 ? (long long)(f.y - 1) : (long long)n + (-f.y - 1);
-      double dphi = phi[k2] - phi[f.x - 1];
-      if (f.y < 0 && sym[-f.y - 1]) dphi = 0.0;
-      const double x = d[s], y = d[nf + s], z = d[2 * nf + s];
+      const double po = phi[other[k]];
+      const unsigned fl = flag[k];
+      double dphi = (fl & 1u) ? pc - po : po - pc;
+      if (fl & 2u) dphi = 0.0;
+      const double x = id[k], y = id[m + k], z = id[2 * m + k];
       ax = ax + dphi * (g0 * x + g3 * y + g4 * z);
       ay = ay + dphi * (g3 * x + g1 * y + g5 * z);
       az = az + dphi * (g4 * x + g5 * y + g2 * z);
@@ -253,16 +258,18 @@ struct tfs_gradient {
   int *ptr = nullptr;
   long long *inc = nullptr;
   int2 *faces = nullptr, *bf = nullptr;
-  double *d = nullptr, *g = nullptr, *xc = nullptr;
-  unsigned char *sym = nullptr, *kind = nullptr;
+  long long m = 0;            // incidences (interior faces twice, boundary faces once)
+  double *d = nullptr, *g = nullptr, *xc = nullptr, *id = nullptr;
+  int *other = nullptr;
+  unsigned char *kind = nullptr, *flag = nullptr;
   double *part = nullptr;     // 3 kParts partial maxima + 3 results
 };
 
 extern "C" int tfs_gradient_destroy(tfs_gradient *S) {
   if (!S) return TFS_OK;
   for (void *p : {(void *)S->ptr, (void *)S->inc, (void *)S->faces, (void *)S->bf, (void *)S->d,
-                  (void *)S->g, (void *)S->xc, (void *)S->sym, (void *)S->kind,
-                  (void *)S->part})
+                  (void *)S->g, (void *)S->xc, (void *)S->kind,
+                  (void *)S->part, (void *)S->id, (void *)S->other, (void *)S->flag})
     if (p) (void)hipFree(p);
   delete S;
   return TFS_OK;
@@ -307,10 +314,29 @@ extern "C" int tfs_gradient_create(int n_cells, int n_bnd_cells, long long n_fac
     inc[(size_t)fill[(size_t)c1]++] = 2 * s;
     if (c2 > 0) inc[(size_t)fill[(size_t)(c2 - 1)]++] = 2 * s + 1;
   }
+  // per incidence, in the same order: d of the face, the phi slot of the
+  // face's other cell, side and symmetry flags
+  const size_t m = inc.size();
+  std::vector<double> id(3 * m);
+  std::vector<int> other(m);
+  std::vector<unsigned char> flag(m);
+  for (size_t k = 0; k < m; ++k) {
+    const long long e = inc[k], s = e >> 1;
+    const int c1 = faces_c[2 * s], c2 = faces_c[2 * s + 1];
+    for (int a = 0; a < 3; ++a) id[(size_t)a * m + k] = d[(size_t)a * (size_t)nf + (size_t)s];
+    if (e & 1) {
+      other[k] = c1 - 1;
+      flag[k] = 1;
+    } else {
+      other[k] = c2 > 0 ? c2 - 1 : n + (-c2 - 1);
+      flag[k] = (c2 < 0 && sym[(size_t)(-c2 - 1)]) ? 2 : 0;
+    }
+  }
   auto *S = new tfs_gradient;
   S->n = n;
   S->nb = nb;
   S->nf = nf;
+  S->m = (long long)m;
   S->nbf = (long long)bf.size();
   auto up = [](void **dst, const void *src, size_t bytes) -> bool {
     if (hipMalloc(dst, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
@@ -323,17 +349,19 @@ extern "C" int tfs_gradient_create(int n_cells, int n_bnd_cells, long long n_fac
             up((void **)&S->bf, bf.data(), sizeof(int2) * bf.size()) &&
             up((void **)&S->d, d, 3 * sizeof(double) * (size_t)nf) &&
             up((void **)&S->g, nullptr, 6 * sizeof(double) * (size_t)n) &&
-            up((void **)&S->sym, sym.data(), sym.size()) &&
             up((void **)&S->kind, kinds.data(), kinds.size()) &&
             up((void **)&S->part, nullptr, sizeof(double) * (3 * (size_t)kParts + 3)) &&
+            up((void **)&S->id, id.data(), sizeof(double) * id.size()) &&
+            up((void **)&S->other, other.data(), sizeof(int) * other.size()) &&
+            up((void **)&S->flag, flag.data(), flag.size()) &&
             (!xc || up((void **)&S->xc, xc, 3 * sizeof(double) * na));
   if (!ok) {
     (void)hipGetLastError();
     tfs_gradient_destroy(S);
     return TFS_ERR_NOMEM;
   }
-  hipLaunchKernelGGL(k_gmatrix, dim3(grid_of(n)), dim3(kThreads), 0, nullptr, n, nf, S->ptr,
-                     S->inc, S->d, S->g);
+  hipLaunchKernelGGL(k_gmatrix, dim3(grid_of(n)), dim3(kThreads), 0, nullptr, n, S->m, S->ptr,
+                     S->id, S->g);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) {
     (void)hipGetLastError();
     tfs_gradient_destroy(S);
@@ -355,8 +383,8 @@ extern "C" int tfs_gradient_matrix_get(tfs_gradient *S, double *g) {
 extern "C" int tfs_grad_phi_device(tfs_gradient *S, const double *d_phi, double *d_px,
                                    double *d_py, double *d_pz) {
   if (!S || !d_phi || !d_px || !d_py || !d_pz) return TFS_ERR_INVALID;
-  hipLaunchKernelGGL(k_graphi, dim3(grid_of(S->n)), dim3(kThreads), 0, nullptr, S->n, S->nf,
-                     S->ptr, S->inc, S->faces, S->d, S->g, S->sym, d_phi, d_px, d_py, d_pz);
+  hipLaunchKernelGGL(k_graphi, dim3(grid_of(S->n)), dim3(kThreads), 0, nullptr, S->n, S->m,
+                     S->ptr, S->other, S->flag, S->id, S->g, d_phi, d_px, d_py, d_pz);
   TFS_G_HIP(hipGetLastError());
   TFS_G_HIP(hipStreamSynchronize(nullptr));
   return TFS_OK;
@@ -369,13 +397,13 @@ extern "C" int tfs_grad_p_device(tfs_gradient *S, double *d_phi, double *d_px, d
   if (S->nbf > 0)
     hipLaunchKernelGGL(k_bnd_copy, dim3(grid_of(S->nbf)), dim3(kThreads), 0, nullptr, S->nbf,
                        S->bf, d_phi);
-  hipLaunchKernelGGL(k_graphi, dim3(grid_of(S->n)), dim3(kThreads), 0, nullptr, S->n, S->nf,
-                     S->ptr, S->inc, S->faces, S->d, S->g, S->sym, d_phi, d_px, d_py, d_pz);
+  hipLaunchKernelGGL(k_graphi, dim3(grid_of(S->n)), dim3(kThreads), 0, nullptr, S->n, S->m,
+                     S->ptr, S->other, S->flag, S->id, S->g, d_phi, d_px, d_py, d_pz);
   if (S->nbf > 0)
     hipLaunchKernelGGL(k_bnd_extrap, dim3(grid_of(S->nbf)), dim3(kThreads), 0, nullptr, S->nbf,
                        na, S->bf, S->xc, d_px, d_py, d_pz, d_phi);
-  hipLaunchKernelGGL(k_graphi, dim3(grid_of(S->n)), dim3(kThreads), 0, nullptr, S->n, S->nf,
-                     S->ptr, S->inc, S->faces, S->d, S->g, S->sym, d_phi, d_px, d_py, d_pz);
+  hipLaunchKernelGGL(k_graphi, dim3(grid_of(S->n)), dim3(kThreads), 0, nullptr, S->n, S->m,
+                     S->ptr, S->other, S->flag, S->id, S->g, d_phi, d_px, d_py, d_pz);
   TFS_G_HIP(hipGetLastError());
   TFS_G_HIP(hipStreamSynchronize(nullptr));
   return TFS_OK;
