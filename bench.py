@@ -693,6 +693,12 @@ def ingest(args):
     g_alg = n_inc * (24 + 4 + 1 + 8) + c.n * (4 + 48 + 8 + 24)
     log(f"  gradient (GraPhi, 3 components) {g_ms:.3f} ms ({g_alg / g_ms / 1e6:.0f} GB/s "
         f"algorithmic); GradP {p_ms:.3f} ms; Correct_Velocity {c_ms:.3f} ms")
+    cpu_rows = None
+    if not args.no_cpu:
+        try:
+            cpu_rows = ingest_cpu_rows(c, a12, ms + a_ms)
+        except Exception as e:  # the baseline is optional; the GPU figures stand
+            log(f"  reference rows not timed: {e}")
     achieved = alg / (ms * 1e-3) / 1e9
     out = {
         "metric": "mesh topology cells/s (device Matrix_Mod_Topology)", "value": value,
@@ -715,8 +721,88 @@ def ingest(args):
                             "traffic": measured_traffic("k_graphi_momentum128_counters"),
                             "algorithmic_bytes": g_alg, "timing": "whole call, wall clock"},
         "cpu_baseline": cpu,
+        "cpu_reference_rows": cpu_rows,
     }
     return out
+
+
+def ingest_cpu_rows(c, a12_dev, gpu_topo_asm_ms):
+    """The reference's own routines timed on the host beside the device ones:
+    Matrix_Mod_Topology + Pressure_Matrix_Fractional on the 128^3 faces
+    (against the device topology + assembly), and GraPhi x 3, GradP and Correct_Velocity on the
+    reference generator's non-orthogonal backstep mesh (18,000 cells; the
+    committed fixture), against the device entry points on the same mesh.
+    Milliseconds per call, one host core; None where oracle/_ref is absent."""
+    import gzip
+    import shutil
+    import tempfile
+    import numpy as np
+    import torch
+    from tflows_amd import mesh, solvers
+    sys.path.insert(0, str(ROOT))
+    from oracle.oracle import Reference, reference_available
+    if not reference_available():
+        return None
+    ref = Reference()
+    rows = {}
+    coefs = a12_dev.cpu().numpy()      # any positive per-face coefficients
+    t0 = time.perf_counter()
+    ref.pressure_matrix(c.n, c.faces, 1.0, coefs)
+    rows["topology_and_pressure_matrix_128"] = {"cpu_ms": 1e3 * (time.perf_counter() - t0),
+                                                "gpu_ms": gpu_topo_asm_ms}
+    with tempfile.TemporaryDirectory() as tmp:
+        gold = ROOT / "tests" / "golden" / "mesh"
+        for ext in (".cns", ".geo"):
+            with gzip.open(gold / ("backstep_nonorthogonal" + ext + ".gz"), "rb") as src, \
+                    open(pathlib.Path(tmp) / ("backstep_nonorthogonal" + ext), "wb") as dst:
+                shutil.copyfileobj(src, dst)
+        stem = str(pathlib.Path(tmp) / "backstep_nonorthogonal")
+        cns = mesh.read_cns(stem + ".cns")
+        geo = mesh.read_geo(stem + ".geo", cns)
+        sc, d = mesh.face_coef(cns, geo)
+        n, nb, fc = cns["n_cells"], cns["n_bnd_cells"], cns["faces_c"]
+        ref.mesh(stem, dt=0.01)
+        ref.mesh_grad(0, n, nb)
+        rng = np.random.default_rng(9)
+        phi = rng.standard_normal(n + nb)
+        reps = 20
+
+        def cpu_ms(f):
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                f()
+            return 1e3 * (time.perf_counter() - t0) / reps
+        G = solvers.Gradient(n, nb, fc, d, geo["xc"])
+        dphi = torch.from_numpy(phi).cuda()
+        comps = [torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(3)]
+
+        def gpu_ms(f):
+            f()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                f()
+            torch.cuda.synchronize()
+            return 1e3 * (time.perf_counter() - t0) / reps
+        rows["graphi_backstep"] = {"cpu_ms": cpu_ms(lambda: ref.mesh_grad(1, n, nb, phi)),
+                                   "gpu_ms": gpu_ms(lambda: G.grad_phi(dphi, *comps))}
+        rows["gradp_backstep"] = {"cpu_ms": cpu_ms(lambda: ref.mesh_grad(2, n, nb, phi)),
+                                  "gpu_ms": gpu_ms(lambda: G.grad_p(dphi, *comps))}
+        u, v, w, pp = (rng.standard_normal(n + nb) for _ in range(4))
+        sav = rng.uniform(0.5, 1.5, n)
+        flux = rng.standard_normal(cns["n_faces"])
+        coef = -0.01 * sc
+        dev = [torch.from_numpy(a).cuda() for a in (geo["vol"], sav, u, v, w, pp, coef,
+                                                      np.ones(n), sc, flux)]
+        bm = torch.empty(n, dtype=torch.float64, device="cuda")
+        rows["correct_velocity_backstep"] = {
+            "cpu_ms": cpu_ms(lambda: ref.mesh_correct(n, u, v, w, phi[:n], phi[:n], phi[:n], pp,
+                                                      sav, flux, 1.0, 1e-3)),
+            "gpu_ms": gpu_ms(lambda: G.correct_velocity(dev[0], dev[1], *comps, dev[2], dev[3],
+                                                        dev[4], dev[5], dev[6], dev[7], dev[8],
+                                                        0.01, 1e-3, dev[9], bm))}
+    log(f"  reference beside the device: {rows}")
+    return rows
 
 
 def main():
