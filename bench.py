@@ -419,6 +419,9 @@ def batch(args):
     B.set_refinement(solvers.TFS_REFINE_FCG if args.refine == "fcg" else solvers.TFS_REFINE_DEFECT)
     if args.inner_block:
         B.set_inner_block(args.inner_block)
+    if args.inner_order:
+        B.set_inner_order(solvers.TFS_INNER_REFERENCE if args.inner_order == "reference"
+                          else solvers.TFS_INNER_COLOURED)
     tols = [1e-10 * float(np.linalg.norm(c.b)) for c in sysl]
     bs = [torch.from_numpy(c.b).cuda() for c in sysl]
     xs = [torch.zeros(c.n, dtype=torch.float64, device="cuda") for c in sysl]
@@ -826,6 +829,8 @@ def main():
                     help="workloads 3-4: outer loop of the refinement")
     ap.add_argument("--inner-block", type=int, default=0,
                     help="workloads 3-4: rows per colour block of the inner order (0: default)")
+    ap.add_argument("--inner-order", choices=["reference", "coloured"], default=None,
+                    help="workload 4: the row order of the FP32 inner factor (default coloured)")
     ap.add_argument("--cases", type=int, default=24)
     ap.add_argument("--visits", type=int, default=10)
     ap.add_argument("--rate", type=float, default=200.0)
