@@ -483,6 +483,7 @@ def batch(args):
                    "fp64_batch_solves_per_s": len(sysl) * args.steps / e64,
                    "speedup_vs_fp64": e64 / e32,
                    "max_rel_l2_vs_fp64": max(errs),
+                   "inner_rtol": args.inner_rtol,
                    "refinement_steps_max": int(np.max(steps_out)),
                    "inner_iterations_max": int(np.max(inner)),
                    "batch_info": B.info(), "cpu_baseline": cpu})
@@ -824,7 +825,10 @@ def main():
                     help="workloads 4-5: CPU seconds of the sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0, help="workload 3: mesh scale per axis")
-    ap.add_argument("--inner-rtol", type=float, default=1e-2)
+    ap.add_argument("--inner-rtol", type=float, default=None,
+                    help="inner FP32 tolerance relative to the outer residual (default 1e-2 "
+                         "for workload 3, 1e-3 for the batch of workload 4: the fastest "
+                         "measured for each)")
     ap.add_argument("--refine", choices=["fcg", "defect"], default="fcg",
                     help="workloads 3-4: outer loop of the refinement")
     ap.add_argument("--inner-block", type=int, default=0,
@@ -840,6 +844,8 @@ def main():
                     help="first run a plain device copy of this many MiB, 5 times (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.inner_rtol is None:
+        args.inner_rtol = 1e-3 if args.workload == 4 else 1e-2
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               f"(the solver path runs on a single MI355X)", file=sys.stderr)
