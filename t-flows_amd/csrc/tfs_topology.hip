@@ -182,6 +182,56 @@ k_fill(long long nf, int base, const int *__restrict__ faces, const int *__restr
   }
 }
 
+constexpr int kRegRow = 16;   // rows up to this long are sorted in registers
+
+// One row of at most kRegRow entries: loaded once, sorted by an odd-even
+// transposition network (stable: only strictly greater keys move down, so the
+// padding keys stay behind every real entry), written back once.
+__device__ __forceinline__ void sort_row_regs(int c, long long b, int w, int base,
+                                              int *__restrict__ col, long long *__restrict__ tag,
+                                              int *__restrict__ dia, int *__restrict__ pos) {
+  int k[kRegRow];
+  long long t[kRegRow];
+#pragma unroll
+  for (int i = 0; i < kRegRow; ++i) {
+    k[i] = i < w ? col[b + i] : INT_MAX;
+    t[i] = (tag && i < w) ? tag[b + i] : -1;
+  }
+#pragma unroll
+  for (int pass = 0; pass < kRegRow; ++pass) {
+#pragma unroll
+    for (int i = pass & 1; i + 1 < kRegRow; i += 2) {
+      if (k[i] > k[i + 1]) {
+        const int kk = k[i];
+        k[i] = k[i + 1];
+        k[i + 1] = kk;
+        const long long tt = t[i];
+        t[i] = t[i + 1];
+        t[i + 1] = tt;
+      }
+    }
+  }
+  int d = 0;
+#pragma unroll
+  for (int i = 0; i < kRegRow; ++i) {
+    if (i < w) {
+      col[b + i] = k[i];
+      if (k[i] == c + base) d = i;
+    }
+  }
+  dia[c] = (int)(b + d + base);
+  if (!tag) return;
+  // from the end: `last` is the end of the current run of equal columns
+  int last = kRegRow - 1;
+#pragma unroll
+  for (int i = kRegRow - 1; i >= 0; --i) {
+    if (i < w) {
+      if (i == w - 1 || k[i] != k[last]) last = i;
+      if (t[i] >= 0) pos[t[i]] = (int)(b + last + base);
+    }
+  }
+}
+
 // ascending columns per row with their tags, the diagonal's position
 // (Topology.f90:75-82), and the face positions (:106-121)
 __global__ void __launch_bounds__(kThreads)
@@ -189,6 +239,11 @@ k_sort(int n, int base, const int *__restrict__ row, int *__restrict__ col,
        long long *__restrict__ tag, int *__restrict__ dia, int *__restrict__ pos) {
   for (int c = blockIdx.x * kThreads + threadIdx.x; c < n; c += gridDim.x * kThreads) {
     const long long b = row[c] - base, e = row[c + 1] - base;
+    if (e - b <= kRegRow) {
+      sort_row_regs(c, b, (int)(e - b), base, col, tag, dia, pos);
+      continue;
+    }
+    // longer rows: insertion sort in place
     for (long long q = b + 1; q < e; ++q) {
       const int key = col[q];
       const long long t = tag ? tag[q] : 0;
