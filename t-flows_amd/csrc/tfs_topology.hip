@@ -6,11 +6,13 @@
 // (:62-71), each row sorted ascending with its diagonal located (:75-82), and
 // the two entries of every interior face located (:106-121).  Here:
 //
-//   k_width   one thread per face, atomic width counts per cell;
+//   k_width   one thread per face, atomic width counts per cell; the count an
+//             atomic returns is the face's place in that row, kept per face;
 //   k_scan_*  row starts: per-block sums, one scan of the block sums, per-block
 //             rescan (an exclusive scan of width + 1 with 64-bit totals);
-//   k_fill    one thread per face, atomic cursor per row; each off-diagonal
-//             entry carries a tag naming its face and side (2 s + side);
+//   k_fill    one thread per face, no atomics: each face's two entries go to
+//             the places k_width returned, each carrying a tag naming its face
+//             and side (2 s + side);
 //   k_sort    one thread per cell: insertion sort of its row's (column, tag)
 //             pairs by column, the diagonal's position (the last entry equal
 //             to the cell, as the reference's loop leaves it), and pos: every
@@ -20,9 +22,10 @@
 // Atomics make the fill order nondeterministic, but a row holds a multiset of
 // columns and sorting it fixes the order, so the result is bitwise the
 // sequential one; tags only travel with their columns.  Everything is
-// integer traffic: the kernels are bound by the device-scope atomics (width
-// counts and fill cursors land on cells in mesh order, which is random on an
-// unstructured numbering) and by the scattered pos writes, not by arithmetic.
+// integer traffic: the kernels are bound by the device-scope atomics (the
+// width counts land on cells in mesh order, which is random on an
+// unstructured numbering) and by the scattered column and pos writes, not by
+// arithmetic.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -60,7 +63,7 @@ __global__ void __launch_bounds__(kThreads) k_fill_int(long long m, int v, int *
 // (c2 <= 0 is a boundary face, as the reference's test c2 > 0 has it)
 __global__ void __launch_bounds__(kThreads)
 k_width(int n, long long nf, const int *__restrict__ faces, int *__restrict__ width,
-        int *__restrict__ err) {
+        int2 *__restrict__ slot, int *__restrict__ err) {
   for (long long s = blockIdx.x * (long long)kThreads + threadIdx.x; s < nf;
        s += (long long)gridDim.x * kThreads) {
     const int2 f = reinterpret_cast<const int2 *>(faces)[s];
@@ -69,8 +72,11 @@ k_width(int n, long long nf, const int *__restrict__ faces, int *__restrict__ wi
       continue;
     }
     if (f.y > 0) {
-      atomicAdd(&width[f.x - 1], 1);
-      atomicAdd(&width[f.y - 1], 1);
+      // the count before this face is the face's place in each row, after
+      // the diagonal
+      const int k1 = atomicAdd(&width[f.x - 1], 1);
+      const int k2 = atomicAdd(&width[f.y - 1], 1);
+      slot[s] = make_int2(k1 + 1, k2 + 1);
     }
   }
 }
@@ -119,9 +125,9 @@ k_scan_top(int nb, long long *__restrict__ bsum, long long *__restrict__ total) 
 }
 
 // row(c) = base + exclusive prefix of (width + 1); the diagonal goes first in
-// its row and the row's fill cursor restarts at 1 (Topology.f90:54-59)
+// its row (Topology.f90:54-59)
 __global__ void __launch_bounds__(kThreads)
-k_scan_rows(int n, int base, const long long *__restrict__ bsum, int *__restrict__ width,
+k_scan_rows(int n, int base, const long long *__restrict__ bsum, const int *__restrict__ width,
             int *__restrict__ row, int *__restrict__ col) {
   __shared__ int wsum[kThreads / 64];
   const long long t0 = (long long)blockIdx.x * kScanTile;
@@ -149,27 +155,27 @@ k_scan_rows(int n, int base, const long long *__restrict__ bsum, int *__restrict
     if (c < n) {
       row[c] = (int)off;
       col[off - base] = (int)c + base;
-      width[c] = 1;
       if (c == n - 1) row[n] = (int)(off + v[k]);
     }
     off += v[k];
   }
 }
 
-// off-diagonal columns through per-row cursors (Topology.f90:62-71); the
-// entry's tag 2 s + side locates pos(side + 1, s) after the sort; boundary
-// faces get pos = base - 1 here
+// off-diagonal columns at the places k_width gave the face in its two rows
+// (Topology.f90:62-71); the entry's tag 2 s + side locates pos(side + 1, s)
+// after the sort; boundary faces get pos = base - 1 here
 __global__ void __launch_bounds__(kThreads)
 k_fill(long long nf, int base, const int *__restrict__ faces, const int *__restrict__ row,
-       int *__restrict__ cursor, int *__restrict__ col, long long *__restrict__ tag,
+       const int2 *__restrict__ slot, int *__restrict__ col, long long *__restrict__ tag,
        int *__restrict__ pos) {
   for (long long s = blockIdx.x * (long long)kThreads + threadIdx.x; s < nf;
        s += (long long)gridDim.x * kThreads) {
     const int2 f = reinterpret_cast<const int2 *>(faces)[s];
     if (f.y > 0) {
       const int c1 = f.x - 1, c2 = f.y - 1;
-      const long long q1 = row[c1] - base + atomicAdd(&cursor[c1], 1);
-      const long long q2 = row[c2] - base + atomicAdd(&cursor[c2], 1);
+      const int2 k = slot[s];
+      const long long q1 = row[c1] - base + k.x;
+      const long long q2 = row[c2] - base + k.y;
       col[q1] = c2 + base;
       col[q2] = c1 + base;
       if (tag) {
@@ -286,12 +292,14 @@ extern "C" int tfs_topology_build_device(int n_cells, long long n_faces, const i
   const int nb = (int)((n_cells + (long long)kScanTile - 1) / kScanTile);
   int rc = TFS_OK;
   int *width = nullptr, *err = nullptr;
+  int2 *slot = nullptr;
   long long *bsum = nullptr, *total = nullptr, *tag = nullptr;
   int h_err = 0;
   long long h_total = 0;
   const hipStream_t st = nullptr;    // the calling thread's default stream
   if (hipMalloc(&width, sizeof(int) * (size_t)n_cells) != hipSuccess ||
-      hipMalloc(&bsum, sizeof(long long) * (size_t)nb + 2 * sizeof(long long)) != hipSuccess) {
+      hipMalloc(&bsum, sizeof(long long) * (size_t)nb + 2 * sizeof(long long)) != hipSuccess ||
+      hipMalloc(&slot, sizeof(int2) * (size_t)(n_faces > 0 ? n_faces : 1)) != hipSuccess) {
     (void)hipGetLastError();
     rc = TFS_ERR_NOMEM;
     goto done;
@@ -303,7 +311,7 @@ extern "C" int tfs_topology_build_device(int n_cells, long long n_faces, const i
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(kThreads), 0, st, 1LL, 0, err);
   if (n_faces > 0)
     hipLaunchKernelGGL(k_width, dim3(grid_of(n_faces)), dim3(kThreads), 0, st, n_cells, n_faces,
-                       d_faces_c, width, err);
+                       d_faces_c, width, slot, err);
   hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(kThreads), 0, st, n_cells, width, bsum);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kThreads), 0, st, nb, bsum, total);
   TFS_T_HIP(hipGetLastError());
@@ -332,7 +340,7 @@ extern "C" int tfs_topology_build_device(int n_cells, long long n_faces, const i
                      d_row, d_col);
   if (n_faces > 0)
     hipLaunchKernelGGL(k_fill, dim3(grid_of(n_faces)), dim3(kThreads), 0, st, n_faces, base,
-                       d_faces_c, d_row, width, d_col, tag, d_pos);
+                       d_faces_c, d_row, slot, d_col, tag, d_pos);
   hipLaunchKernelGGL(k_sort, dim3(grid_of(n_cells)), dim3(kThreads), 0, st, n_cells, base, d_row,
                      d_col, tag, d_dia, d_pos);
   TFS_T_HIP(hipGetLastError());
@@ -341,6 +349,7 @@ extern "C" int tfs_topology_build_device(int n_cells, long long n_faces, const i
 done:
   if (width) (void)hipFree(width);
   if (bsum) (void)hipFree(bsum);
+  if (slot) (void)hipFree(slot);
   if (tag) (void)hipFree(tag);
   return rc;
 }
