@@ -297,9 +297,13 @@ extern "C" int tfs_topology_build_device(int n_cells, long long n_faces, const i
   int h_err = 0;
   long long h_total = 0;
   const hipStream_t st = nullptr;    // the calling thread's default stream
-  if (hipMalloc(&width, sizeof(int) * (size_t)n_cells) != hipSuccess ||
-      hipMalloc(&bsum, sizeof(long long) * (size_t)nb + 2 * sizeof(long long)) != hipSuccess ||
-      hipMalloc(&slot, sizeof(int2) * (size_t)(n_faces > 0 ? n_faces : 1)) != hipSuccess) {
+  // scratch from the stream-ordered pool: no device-wide synchronisation on
+  // allocation or release
+  if (hipMallocAsync((void **)&width, sizeof(int) * (size_t)n_cells, st) != hipSuccess ||
+      hipMallocAsync((void **)&bsum, sizeof(long long) * (size_t)nb + 2 * sizeof(long long), st) !=
+          hipSuccess ||
+      hipMallocAsync((void **)&slot, sizeof(int2) * (size_t)(n_faces > 0 ? n_faces : 1), st) !=
+          hipSuccess) {
     (void)hipGetLastError();
     rc = TFS_ERR_NOMEM;
     goto done;
@@ -327,7 +331,7 @@ extern "C" int tfs_topology_build_device(int n_cells, long long n_faces, const i
     goto done;
   }
   if (d_pos) {
-    if (hipMalloc(&tag, sizeof(long long) * (size_t)h_total) != hipSuccess) {
+    if (hipMallocAsync((void **)&tag, sizeof(long long) * (size_t)h_total, st) != hipSuccess) {
       (void)hipGetLastError();
       rc = TFS_ERR_NOMEM;
       goto done;
@@ -347,9 +351,7 @@ extern "C" int tfs_topology_build_device(int n_cells, long long n_faces, const i
   TFS_T_HIP(hipStreamSynchronize(st));
   if (nnz) *nnz = h_total;
 done:
-  if (width) (void)hipFree(width);
-  if (bsum) (void)hipFree(bsum);
-  if (slot) (void)hipFree(slot);
-  if (tag) (void)hipFree(tag);
+  for (void *p : {(void *)width, (void *)bsum, (void *)slot, (void *)tag})
+    if (p) (void)hipFreeAsync(p, st);
   return rc;
 }
