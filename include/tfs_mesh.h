@@ -125,6 +125,13 @@ enum {
  * (n_cells + n_bnd_cells entries). */
 int tfs_mesh_read_cns(const char *path, int *n_cells, int *n_bnd_cells, int *n_faces,
                       int *faces_c, int *material);
+/* The records after the faces (Sources/Process/Load_Cns.f90:62-69): bcmark[b]
+ * = the boundary-condition number of boundary cell -(b+1) (the index of its
+ * name in the file; the caller maps it to a kind), copy_c[b] = its copy cell
+ * (0 where it has none).  n_bnd_cells as tfs_mesh_read_cns returned it; either
+ * output may be NULL. */
+int tfs_mesh_read_cns_marks(const char *path, int n_bnd_cells, int *bcmark, int *copy_c);
+
 /* Reads a .geo file (replaces Sources/Shared/Load_Geo.f90:23-50) for the sizes
  * of its .cns.  Outputs (each may be NULL), x, y and z stored one block after
  * the other:

@@ -297,7 +297,7 @@ class Reference:
     MESH_ARRAYS = {1: ("faces_c", True), 2: ("row", True), 3: ("col", True), 4: ("dia", True),
                    5: ("xc", False), 6: ("vol", False), 7: ("s", False), 8: ("shift", False),
                    9: ("f", False), 10: ("xf", False), 11: ("d", False), 12: ("scoef", False),
-                   13: ("val", False)}
+                   13: ("val", False), 14: ("bcmark", True), 15: ("copy_c", True)}
 
     def mesh(self, stem, dt=1.0):
         """The reference's Load_Cns + Load_Geo on <stem>.cns / <stem>.geo,

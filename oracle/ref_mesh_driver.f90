@@ -265,7 +265,8 @@
   !          6 vol (nc)              7 surfaces (3*nf)   8 shifts as read      !
   !          9 f (nf)               10 face centres (3*nf)                     !
   !         11 distances after Compute_Geometry (3*nf)  12 Scoef (nf)          !
-  !         13 val (nnz)                                                        !
+  !         13 val (nnz)  14 bcmark (nb, int)  15 CopyC (nb, int), both of    !
+  !            boundary cells -1..-nb                                           !
   !   Integer arrays go to iout, real ones to rout; returns the count copied,  !
   !   -1 if cap is too small or which is unknown.                              !
   !---------------------------------------------------------------------------!
@@ -291,6 +292,8 @@
     case(11); n = size(m_d);     if(n > cap) return; rout(1:n) = m_d
     case(12); n = size(m_scoef); if(n > cap) return; rout(1:n) = m_scoef
     case(13); n = size(m_val);   if(n > cap) return; rout(1:n) = m_val
+    case(14); n = m_nb; if(n > cap) return; iout(1:n) = bcmark(-1:-m_nb:-1)
+    case(15); n = m_nb; if(n > cap) return; iout(1:n) = CopyC(-1:-m_nb:-1)
     case default; return
   end select
   tfs_ref_mesh_get = n

@@ -81,6 +81,36 @@ extern "C" int tfs_mesh_read_cns(const char *path, int *n_cells, int *n_bnd_cell
   return TFS_MESH_OK;
 }
 
+// Load_Cns.f90:62-69: after the faces, the boundary-condition mark of every
+// boundary cell -1..-nb, then its copy cell (0 where it has none).
+extern "C" int tfs_mesh_read_cns_marks(const char *path, int n_bnd_cells, int *bcmark,
+                                       int *copy_c) {
+  if (!path || n_bnd_cells < 0) return TFS_MESH_ERR_INVALID;
+  int nc = 0, nb = 0, nf = 0;
+  const int rc = tfs_mesh_read_cns(path, &nc, &nb, &nf, nullptr, nullptr);
+  if (rc != TFS_MESH_OK) return rc;
+  if (nb != n_bnd_cells) return TFS_MESH_ERR_INVALID;
+  Records r(path);
+  if (!r.f) return TFS_MESH_ERR_IO;
+  int n_mat = 0, n_bc = 0, skip = 0;
+  // the six sizes, the names, materials and faces, as tfs_mesh_read_cns reads them
+  for (int k = 0; k < 4; ++k)
+    if (!r.read_int(&skip)) return TFS_MESH_ERR_FORMAT;
+  if (!r.read_int(&n_mat) || !r.read_int(&n_bc)) return TFS_MESH_ERR_FORMAT;
+  for (int k = 0; k < n_mat + n_bc; ++k)
+    if (!r.read(nullptr, 80)) return TFS_MESH_ERR_FORMAT;
+  if (!r.read(nullptr, sizeof(int) * (size_t)nc) || !r.read(nullptr, sizeof(int) * (size_t)nb) ||
+      !r.read(nullptr, sizeof(int) * (size_t)nf) || !r.read(nullptr, sizeof(int) * (size_t)nf))
+    return TFS_MESH_ERR_FORMAT;
+  std::vector<int> mark((size_t)nb), copy((size_t)nb);
+  if (!r.read(mark.data(), sizeof(int) * (size_t)nb) ||
+      !r.read(copy.data(), sizeof(int) * (size_t)nb))
+    return TFS_MESH_ERR_FORMAT;
+  if (bcmark) std::memcpy(bcmark, mark.data(), sizeof(int) * mark.size());
+  if (copy_c) std::memcpy(copy_c, copy.data(), sizeof(int) * copy.size());
+  return TFS_MESH_OK;
+}
+
 extern "C" int tfs_mesh_read_geo(const char *path, int n_cells, int n_bnd_cells, int n_faces,
                                  double *xc, double *vol, double *s, double *shift, double *f,
                                  double *xf) {

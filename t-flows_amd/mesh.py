@@ -83,6 +83,9 @@ def _lib():
         lib.tfs_mesh_read_cns.restype = ctypes.c_int
         lib.tfs_mesh_read_cns.argtypes = [ctypes.c_char_p] + [ctypes.POINTER(ctypes.c_int)] * 3 + \
             [ctypes.c_void_p, ctypes.c_void_p]
+        lib.tfs_mesh_read_cns_marks.restype = ctypes.c_int
+        lib.tfs_mesh_read_cns_marks.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p,
+                                                ctypes.c_void_p]
         lib.tfs_mesh_read_geo.restype = ctypes.c_int
         lib.tfs_mesh_read_geo.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int] + [ctypes.c_void_p] * 6
@@ -239,7 +242,8 @@ def read_cns(path):
     dict(n_cells, n_bnd_cells, n_faces, faces_c, material).  faces_c is the
     reference's faces_c(2, n_faces) flattened column-major (c1, c2 per face),
     1-based, c2 = -1..-n_bnd_cells for boundary faces; material lists the
-    cells, then the boundary cells."""
+    cells, then the boundary cells; bcmark / copy_c (Load_Cns.f90:62-69) the
+    boundary-condition number and the copy cell of every boundary cell."""
     lib = _lib()
     nc, nb, nf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     p = str(path).encode()
@@ -250,8 +254,12 @@ def read_cns(path):
     _mesh_check(lib.tfs_mesh_read_cns(p, ctypes.byref(nc), ctypes.byref(nb), ctypes.byref(nf),
                                       faces.ctypes.data, material.ctypes.data),
                 f"read_cns({path})")
+    bcmark = np.empty(max(nb.value, 1), np.int32)
+    copy_c = np.empty(max(nb.value, 1), np.int32)
+    _mesh_check(lib.tfs_mesh_read_cns_marks(p, nb.value, bcmark.ctypes.data, copy_c.ctypes.data),
+                f"read_cns({path})")
     return dict(n_cells=nc.value, n_bnd_cells=nb.value, n_faces=nf.value, faces_c=faces,
-                material=material)
+                material=material, bcmark=bcmark[:nb.value], copy_c=copy_c[:nb.value])
 
 
 def read_geo(path, cns):

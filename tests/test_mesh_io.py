@@ -68,7 +68,7 @@ def ingest(stem):
                 faces_c=cns["faces_c"], row=row, col=col, dia=dia, xc=geo["xc"].ravel(),
                 vol=geo["vol"], s=geo["s"].ravel(), shift=geo["shift"].ravel(), f=geo["f"],
                 xf=geo["xf"].ravel(), d=d.ravel(), scoef=scoef, val=val, pos=pos, cns=cns,
-                geo=geo)
+                geo=geo, bcmark=cns["bcmark"], copy_c=cns["copy_c"])
 
 
 _cache = {}
