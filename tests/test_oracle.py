@@ -348,7 +348,7 @@ def test_planted_factor_mistakes_are_caught(variant, caught):
 
 
 def test_cgs_count_moves_with_summation_order(oracle):
-    """Why CGS parity is judged against a spread: on this case the reference
+    """Why CGS parity is measured against a spread: on this case the reference
     recurrence with left-to-right dots reproduces the oracle exactly (84
     iterations) while reversed-order dots need 89."""
     from krylov_variants import ORDERS, cgs_iterations
