@@ -315,6 +315,25 @@ int tfs_correct_velocity_device(tfs_gradient *S, const double *d_vol, const doub
                                 double visc, double *d_flux, double *d_b, double *errmax,
                                 double *cfl_max, double *pe_max);
 
+/* The face loop of Compute_Pressure_Fractional
+ * (Sources/Process/Compute_Pressure_Fractional.f90:47-131) on the mesh of a
+ * tfs_gradient object, every material a fluid and no BUFFER boundary
+ * (TFS_ERR_INVALID otherwise): per face the density f den(c1) + (1-f)
+ * den(c2), the face velocity rebuilt from the cells' velocities and pressure
+ * gradients, flux = den (u_f . S) and a12 = den |S|^2 (f/sav(c1) +
+ * (1-f)/sav(c2)) (0 on boundary faces, where the boundary cell's velocity
+ * gives the flux); per cell b = inflow - outflow in face order.  Bitwise the
+ * reference's.  The matrix then comes from tfs_assemble_device with a12 (the
+ * boundary faces add 0).  Per face: f, flux, a12, and s in three blocks (sx,
+ * sy, sz); per cell: vol, sav, px, py, pz, b; on cells then boundary cells:
+ * u, v, w, p, den. */
+int tfs_pressure_fractional_device(tfs_gradient *S, const double *d_f, const double *d_s,
+                                   const double *d_vol, const double *d_sav, const double *d_u,
+                                   const double *d_v, const double *d_w, const double *d_px,
+                                   const double *d_py, const double *d_pz, const double *d_p,
+                                   const double *d_den, double *d_flux, double *d_a12,
+                                   double *d_b);
+
 /* ---- serving: continuous batching of solves on one GPU ------------------------
  * Not in the reference.  A case (one system) is registered with its structure
  * and values; its GPU copy is made when its first solve is admitted and stays

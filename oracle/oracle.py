@@ -70,7 +70,18 @@ class Oracle:
         lib.or_correct_velocity.restype = ctypes.c_double
         lib.or_correct_velocity.argtypes = [ctypes.c_int, ctypes.c_int, _I32P] + [_VP] * 16 + \
             [ctypes.c_double, ctypes.c_double, _VP, _VP, _VP, _VP]
+        lib.or_pressure_fractional.argtypes = [ctypes.c_int, ctypes.c_int, _I32P] + [_VP] * 15
         self.lib = lib
+
+    def pressure_fractional(self, n_cells, faces_c, f, s, vol, sav, u, v, w, px, py, pz, p, den):
+        """Compute_Pressure_Fractional's face loop: (flux, a12, b)."""
+        faces_c = _c32(faces_c)
+        nf = faces_c.size // 2
+        keep = [_c64(a).ravel() for a in (f, s, vol, sav, u, v, w, px, py, pz, p, den)]
+        flux, a12, b = np.zeros(nf), np.zeros(nf), np.zeros(n_cells)
+        self.lib.or_pressure_fractional(n_cells, nf, faces_c, *(a.ctypes.data for a in keep),
+                                        flux.ctypes.data, a12.ctypes.data, b.ctypes.data)
+        return flux, a12, b
 
     def correct_velocity(self, n_cells, faces_c, vol, sav, px, py, pz, u, v, w, pp, coef, scoef,
                          d, dt, visc, flux, den, bc=None, bou=None, in_m=None):
@@ -231,6 +242,10 @@ class Reference:
             lib.tfs_ref_mesh_correct.restype = ctypes.c_int
             lib.tfs_ref_mesh_correct.argtypes = [ctypes.c_void_p] * 10 + \
                 [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+            lib.tfs_ref_mesh_press_frac.restype = ctypes.c_int
+            lib.tfs_ref_mesh_press_frac.argtypes = [ctypes.c_void_p] * 8 + \
+                [ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double] + \
+                [ctypes.c_void_p] * 4
             lib.tfs_ref_mesh_get.restype = ctypes.c_int
             lib.tfs_ref_mesh_get.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                              ctypes.c_void_p]
@@ -363,3 +378,18 @@ class Reference:
         if rc != 0:
             raise RuntimeError(f"tfs_ref_mesh_correct returned {rc}")
         return dict(u=u, v=v, w=w, flux=flux, b=b, errmax=res[0], cfl=res[1], pe=res[2])
+
+    def mesh_press_frac(self, n_cells, n_faces, nnz, u, v, w, p, px, py, pz, sav, den, prec,
+                        stol, urf):
+        """Compute_Pressure_Fractional on the grid and matrix of the last mesh()
+        call: dict(flux, val, pp, p) -- face fluxes, A % val, the pressure
+        correction PP % n and the updated pressure P % n (cells)."""
+        keep = [_c64(a) for a in (u, v, w, p, px, py, pz, sav)]
+        flux, val = np.zeros(n_faces), np.zeros(nnz)
+        pp, pn = np.zeros(n_cells), np.zeros(n_cells)
+        rc = self.lib.tfs_ref_mesh_press_frac(*(a.ctypes.data for a in keep), float(den),
+                                              int(prec), float(stol), float(urf), flux.ctypes.data,
+                                              val.ctypes.data, pp.ctypes.data, pn.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"tfs_ref_mesh_press_frac returned {rc}")
+        return dict(flux=flux, val=val, pp=pp, p=pn)

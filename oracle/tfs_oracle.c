@@ -30,6 +30,9 @@
  *   or_assemble           : bitwise against the reference Pressure_Matrix_Fractional;
  *                           exact (integer-valued) coefficients against a dense
  *                           sum of per-face 2x2 element matrices.
+ *   or_pressure_fractional: fluxes and matrix bitwise against the reference's
+ *                           Compute_Pressure_Fractional, its right-hand side
+ *                           through the reference's pressure-correction solution.
  *   or_correct_velocity   : bitwise against the reference's Correct_Velocity
  *                           (velocities, fluxes, mass errors, errmax) on real
  *                           meshes, CFL / Peclet to the 4 digits it reports;
@@ -567,4 +570,44 @@ double or_correct_velocity(int n_cells, int n_faces, const int *faces_c, const d
   *cfl = cm;
   *pe = pm;
   return errmax;
+}
+
+/* Compute_Pressure_Fractional.f90:47-48 (b := 0), :50-131 (the face loop; the
+ * fluid-fluid density branch :55-57; interior faces :69-115 -- the face
+ * velocity, flux and A12; boundary faces :117-127 -- the boundary cell's
+ * velocity, flux only). */
+void or_pressure_fractional(int n_cells, int n_faces, const int *faces_c, const double *f,
+                            const double *s, const double *vol, const double *sav,
+                            const double *u, const double *v, const double *w, const double *px,
+                            const double *py, const double *pz, const double *p,
+                            const double *den, double *flux, double *a12, double *b) {
+  const double *sx = s, *sy = s + n_faces, *sz = s + 2 * (size_t)n_faces;
+  for (int c = 0; c < n_cells; ++c) b[c] = 0.0;
+  for (int k = 0; k < n_faces; ++k) {
+    const int c1 = faces_c[2 * k], c2 = faces_c[2 * k + 1];
+    const size_t k1 = (size_t)(c1 - 1);
+    const size_t k2 = c2 > 0 ? (size_t)(c2 - 1) : (size_t)n_cells + (size_t)(-c2 - 1);
+    const double fs = f[k];
+    const double dens = fs * den[k1] + (1.0 - fs) * den[k2];
+    if (c2 > 0) {
+      double us = fs * (u[k1] + px[k1] * vol[k1] / sav[k1]) +
+                  (1.0 - fs) * (u[k2] + px[k2] * vol[k2] / sav[k2]);
+      double vs = fs * (v[k1] + py[k1] * vol[k1] / sav[k1]) +
+                  (1.0 - fs) * (v[k2] + py[k2] * vol[k2] / sav[k2]);
+      double ws = fs * (w[k1] + pz[k1] * vol[k1] / sav[k1]) +
+                  (1.0 - fs) * (w[k2] + pz[k2] * vol[k2] / sav[k2]);
+      us = us + (p[k1] - p[k2]) * sx[k] * (fs / sav[k1] + (1.0 - fs) / sav[k2]);
+      vs = vs + (p[k1] - p[k2]) * sy[k] * (fs / sav[k1] + (1.0 - fs) / sav[k2]);
+      ws = ws + (p[k1] - p[k2]) * sz[k] * (fs / sav[k1] + (1.0 - fs) / sav[k2]);
+      flux[k] = dens * (us * sx[k] + vs * sy[k] + ws * sz[k]);
+      double a = dens * (sx[k] * sx[k] + sy[k] * sy[k] + sz[k] * sz[k]);
+      a12[k] = a * (fs / sav[k1] + (1.0 - fs) / sav[k2]);
+      b[k1] = b[k1] - flux[k];
+      b[k2] = b[k2] + flux[k];
+    } else {
+      flux[k] = dens * (u[k2] * sx[k] + v[k2] * sy[k] + w[k2] * sz[k]);
+      a12[k] = 0.0;
+      b[k1] = b[k1] - flux[k];
+    }
+  }
 }

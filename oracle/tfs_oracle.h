@@ -106,6 +106,20 @@ double or_correct_velocity(int n_cells, int n_faces, const int *faces_c, const d
                            const double *d, double dt, double visc, double *flux, double *b,
                            double *cfl, double *pe);
 
+/* The face loop of Compute_Pressure_Fractional (Sources/Process/
+ * Compute_Pressure_Fractional.f90:47-131), every material a fluid, no BUFFER
+ * faces: per face the density (f-weighted), the face velocity rebuilt from
+ * the cells' velocities and pressure gradients, the face flux and the
+ * pressure-matrix coefficient A12 (0 on boundary faces, which add nothing to
+ * the matrix); per cell b = inflow - outflow in face order.  f, s (3 blocks)
+ * per face; vol, sav, px/py/pz per cell; u/v/w, p, den on cells then
+ * boundary cells.  Outputs flux, a12 (per face), b (per cell). */
+void or_pressure_fractional(int n_cells, int n_faces, const int *faces_c, const double *f,
+                            const double *s, const double *vol, const double *sav,
+                            const double *u, const double *v, const double *w, const double *px,
+                            const double *py, const double *pz, const double *p,
+                            const double *den, double *flux, double *a12, double *b);
+
 #ifdef __cplusplus
 }
 #endif

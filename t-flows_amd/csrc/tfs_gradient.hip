@@ -250,6 +250,65 @@ k_cv_final(int nf_parts, int n_parts, const double *__restrict__ part, double *_
   block_max(pm, out + 2);
 }
 
+// ---- Compute_Pressure_Fractional's face loop -----------------------------------
+
+// Compute_Pressure_Fractional.f90:50-131 for fluid materials without BUFFER
+// faces: per face the f-weighted density, the face velocity rebuilt from the
+// cells' velocities and pressure gradients (Rhie-Chow), the flux and A12;
+// a boundary face takes its boundary cell's velocity and has A12 = 0.
+__global__ void __launch_bounds__(kThreads)
+k_pf_faces(int n, long long nf, const int2 *__restrict__ faces, const double *__restrict__ f,
+           const double *__restrict__ sv, const double *__restrict__ vol,
+           const double *__restrict__ sav, const double *__restrict__ u,
+           const double *__restrict__ v, const double *__restrict__ w,
+           const double *__restrict__ px, const double *__restrict__ py,
+           const double *__restrict__ pz, const double *__restrict__ p,
+           const double *__restrict__ den, double *__restrict__ flux, double *__restrict__ a12) {
+  for (long long s = blockIdx.x * (long long)kThreads + threadIdx.x; s < nf;
+       s += (long long)gridDim.x * kThreads) {
+    const int2 fc = faces[s];
+    const long long k1 = fc.x - 1;
+    const long long k2 = fc.y > 0 ? (long long)(fc.y - 1) : (long long)n + (-fc.y - 1);
+    const double fs = f[s], sx = sv[s], sy = sv[nf + s], sz = sv[2 * nf + s];
+    const double dens = fs * den[k1] + (1.0 - fs) * den[k2];
+    if (fc.y > 0) {
+      double us = fs * (u[k1] + px[k1] * vol[k1] / sav[k1]) +
+                  (1.0 - fs) * (u[k2] + px[k2] * vol[k2] / sav[k2]);
+      double vs = fs * (v[k1] + py[k1] * vol[k1] / sav[k1]) +
+                  (1.0 - fs) * (v[k2] + py[k2] * vol[k2] / sav[k2]);
+      double ws = fs * (w[k1] + pz[k1] * vol[k1] / sav[k1]) +
+                  (1.0 - fs) * (w[k2] + pz[k2] * vol[k2] / sav[k2]);
+      const double wsav = fs / sav[k1] + (1.0 - fs) / sav[k2];
+      const double dp = p[k1] - p[k2];
+      us = us + dp * sx * wsav;
+      vs = vs + dp * sy * wsav;
+      ws = ws + dp * sz * wsav;
+      flux[s] = dens * (us * sx + vs * sy + ws * sz);
+      const double a = dens * (sx * sx + sy * sy + sz * sz);
+      a12[s] = a * wsav;
+    } else {
+      flux[s] = dens * (u[k2] * sx + v[k2] * sy + w[k2] * sz);
+      a12[s] = 0.0;
+    }
+  }
+}
+
+// b(c) = inflow - outflow over the cell's faces in face order
+// (Compute_Pressure_Fractional.f90:112-113, :126)
+__global__ void __launch_bounds__(kThreads)
+k_pf_rhs(int n, const int *__restrict__ ptr, const long long *__restrict__ inc,
+         const double *__restrict__ flux, double *__restrict__ b) {
+  for (int c = blockIdx.x * kThreads + threadIdx.x; c < n; c += gridDim.x * kThreads) {
+    double m = 0.0;
+    for (int k = ptr[c]; k < ptr[c + 1]; ++k) {
+      const long long e = inc[k];
+      const double fl = flux[e >> 1];
+      m = (e & 1) ? m + fl : m - fl;
+    }
+    b[c] = m;
+  }
+}
+
 }  // namespace
 
 struct tfs_gradient {
@@ -259,6 +318,7 @@ struct tfs_gradient {
   long long *inc = nullptr;
   int2 *faces = nullptr, *bf = nullptr;
   long long m = 0;            // incidences (interior faces twice, boundary faces once)
+  bool has_buffer = false;    // some boundary cell is of kind TFS_BC_BUFFER
   double *d = nullptr, *g = nullptr, *xc = nullptr, *id = nullptr;
   int *other = nullptr;
   unsigned char *kind = nullptr, *flag = nullptr;
@@ -287,6 +347,7 @@ extern "C" int tfs_gradient_create(int n_cells, int n_bnd_cells, long long n_fac
   std::vector<int> cnt((size_t)n + 1, 0), seen((size_t)nb, 0);
   std::vector<unsigned char> sym((size_t)std::max(nb, 1), 0), kinds((size_t)std::max(nb, 1), 0);
   std::vector<int2> bf;
+  bool any_buffer = false;
   for (long long s = 0; s < nf; ++s) {
     const int c1 = faces_c[2 * s], c2 = faces_c[2 * s + 1];
     if (c1 < 1 || c1 > n || c2 == 0 || c2 > n || c2 < -nb) return TFS_ERR_INVALID;
@@ -299,6 +360,7 @@ extern "C" int tfs_gradient_create(int n_cells, int n_bnd_cells, long long n_fac
       const int kind = bc ? bc[-c2 - 1] : TFS_BC_WALL;
       if (kind > TFS_BC_BUFFER) return TFS_ERR_INVALID;
       kinds[(size_t)(-c2 - 1)] = (unsigned char)kind;
+      any_buffer = any_buffer || kind == TFS_BC_BUFFER;
       if (kind == TFS_BC_SYMMETRY) sym[(size_t)(-c2 - 1)] = 1;
       if (kind != TFS_BC_BUFFER && kind != TFS_BC_PRESSURE)
         bf.push_back(make_int2(c1 - 1, n + (-c2 - 1)));
@@ -337,6 +399,7 @@ extern "C" int tfs_gradient_create(int n_cells, int n_bnd_cells, long long n_fac
   S->nb = nb;
   S->nf = nf;
   S->m = (long long)m;
+  S->has_buffer = any_buffer;
   S->nbf = (long long)bf.size();
   auto up = [](void **dst, const void *src, size_t bytes) -> bool {
     if (hipMalloc(dst, std::max<size_t>(bytes, 8)) != hipSuccess) return false;
@@ -438,5 +501,28 @@ extern "C" int tfs_correct_velocity_device(tfs_gradient *S, const double *d_vol,
   if (errmax) *errmax = out[0];
   if (cfl_max) *cfl_max = out[1];
   if (pe_max) *pe_max = out[2];
+  return TFS_OK;
+}
+
+extern "C" int tfs_pressure_fractional_device(tfs_gradient *S, const double *d_f,
+                                              const double *d_s, const double *d_vol,
+                                              const double *d_sav, const double *d_u,
+                                              const double *d_v, const double *d_w,
+                                              const double *d_px, const double *d_py,
+                                              const double *d_pz, const double *d_p,
+                                              const double *d_den, double *d_flux,
+                                              double *d_a12, double *d_b) {
+  if (!S || S->has_buffer || !d_vol || !d_sav || !d_u || !d_v || !d_w || !d_px || !d_py ||
+      !d_pz || !d_p || !d_den || !d_b ||
+      (S->nf > 0 && (!d_f || !d_s || !d_flux || !d_a12)))
+    return TFS_ERR_INVALID;
+  if (S->nf > 0)
+    hipLaunchKernelGGL(k_pf_faces, dim3(grid_of(S->nf)), dim3(kThreads), 0, nullptr, S->n, S->nf,
+                       S->faces, d_f, d_s, d_vol, d_sav, d_u, d_v, d_w, d_px, d_py, d_pz, d_p,
+                       d_den, d_flux, d_a12);
+  hipLaunchKernelGGL(k_pf_rhs, dim3(grid_of(S->n)), dim3(kThreads), 0, nullptr, S->n, S->ptr,
+                     S->inc, d_flux, d_b);
+  TFS_G_HIP(hipGetLastError());
+  TFS_G_HIP(hipStreamSynchronize(nullptr));
   return TFS_OK;
 }

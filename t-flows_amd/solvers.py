@@ -685,6 +685,7 @@ class Gradient:
             lib.tfs_grad_p_device.argtypes = [_VP] + [_VP] * 4
             lib.tfs_correct_velocity_device.argtypes = [_VP] + [_VP] * 14 + \
                 [ctypes.c_double, ctypes.c_double, _VP, _VP, _DP, _DP, _DP]
+            lib.tfs_pressure_fractional_device.argtypes = [_VP] * 16
             lib._tfs_grad_typed = True
         self._lib = lib
         self.n, self.nb = int(n_cells), int(n_bnd_cells)
@@ -750,6 +751,19 @@ class Gradient:
             _dptr(b, n, "b"), ctypes.byref(em), ctypes.byref(cm), ctypes.byref(pm)),
             "tfs_correct_velocity_device")
         return em.value, cm.value, pm.value
+
+    def pressure_fractional(self, f, s, vol, sav, u, v, w, px, py, pz, p, den, flux, a12, b):
+        """Compute_Pressure_Fractional's face loop on the GPU: flux and a12 per
+        face, b per cell (GPU float64 tensors; s holds sx, sy, sz blocks)."""
+        n, na = self.n, self.n + self.nb
+        nf = flux.numel()
+        _sync()
+        _check(self._lib.tfs_pressure_fractional_device(
+            self._h, _dptr(f, nf, "f"), _dptr(s, 3 * nf, "s"), _dptr(vol, n, "vol"),
+            _dptr(sav, n, "sav"), _dptr(u, na, "u"), _dptr(v, na, "v"), _dptr(w, na, "w"),
+            _dptr(px, n, "px"), _dptr(py, n, "py"), _dptr(pz, n, "pz"), _dptr(p, na, "p"),
+            _dptr(den, na, "den"), _dptr(flux, nf, "flux"), _dptr(a12, nf, "a12"),
+            _dptr(b, n, "b")), "tfs_pressure_fractional_device")
 
     def close(self):
         if getattr(self, "_h", None):

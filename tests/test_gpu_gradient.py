@@ -146,3 +146,39 @@ def test_correct_velocity_bitwise(torch, S, oracle, meshes, name):
         assert sha(a) == ref[k], k
     assert em == want["errmax"] == float.fromhex(ref["errmax"])
     assert cm == want["cfl"] and pm == want["pe"]
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_pressure_fractional_bitwise(torch, S, oracle, meshes, name):
+    """Compute_Pressure_Fractional's face loop on the GPU (fluxes, A12, the
+    right-hand side) and the matrix assembled from its A12 on the GPU,
+    against the oracle bitwise and the reference's digests."""
+    from mesh_fixtures import CV_DEN, cv_inputs
+    cns, geo, d = meshes[name]
+    nc, nb, nf = cns["n_cells"], cns["n_bnd_cells"], cns["n_faces"]
+    print(f"pressure_fractional {name} n={nc}", flush=True)
+    cv = cv_inputs(nc, nb, nf)
+    den = np.full(nc + nb, CV_DEN)
+    flux_or, a12_or, b_or = oracle.pressure_fractional(
+        nc, cns["faces_c"], geo["f"], geo["s"], geo["vol"], cv["sav"], cv["u"], cv["v"], cv["w"],
+        cv["px"], cv["py"], cv["pz"], cv["pp"], den)
+    G = S.Gradient(nc, nb, cns["faces_c"], d)
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a, np.float64).ravel()).cuda()
+    flux = torch.zeros(nf, dtype=torch.float64, device="cuda")
+    a12 = torch.zeros(nf, dtype=torch.float64, device="cuda")
+    b = torch.zeros(nc, dtype=torch.float64, device="cuda")
+    G.pressure_fractional(dev(geo["f"]), dev(geo["s"]), dev(geo["vol"]), dev(cv["sav"]),
+                          dev(cv["u"]), dev(cv["v"]), dev(cv["w"]), dev(cv["px"]), dev(cv["py"]),
+                          dev(cv["pz"]), dev(cv["pp"]), dev(den), flux, a12, b)
+    assert np.array_equal(flux.cpu().numpy(), flux_or)
+    assert np.array_equal(a12.cpu().numpy(), a12_or)
+    assert np.array_equal(b.cpu().numpy(), b_or)
+    row, col, dia, pos = mesh.topology(nc, cns["faces_c"], base=1, with_pos=True)
+    A = S.Matrix(row - 1, col - 1, dia - 1)
+    S.Assembly(A, cns["faces_c"], pos, dia, base=1).assemble(a12)
+    val = A.get_values()
+    assert np.array_equal(val, oracle.assemble(nc, cns["faces_c"], pos, dia, len(col), a12_or))
+    want = META["meshes"][name]["pressure_fractional"]
+    assert sha(flux.cpu().numpy()) == want["flux"] and sha(val) == want["val"]
