@@ -691,12 +691,22 @@ def ingest(args):
                                                      cells[2], fsc, 0.01, 1e-3, flux, bm),
                        args.steps, args.warmup, torch.cuda.synchronize)
     c_ms = 1e3 * c_el / args.steps
+    # Compute_Pressure_Fractional's face loop (fluxes, A12, right-hand side)
+    fw = torch.from_numpy(rng.uniform(0.3, 0.7, nf)).cuda()
+    sv = torch.from_numpy(rng.standard_normal(3 * nf)).cuda()
+    den_all = torch.ones(c.n + nbc, dtype=torch.float64, device="cuda")
+    a12_pf = torch.empty(nf, dtype=torch.float64, device="cuda")
+    pf_el, _, _ = timed(lambda: grad.pressure_fractional(fw, sv, cells[0], cells[1], *vel, *comps,
+                                                         phi, den_all, flux, a12_pf, bm),
+                        args.steps, args.warmup, torch.cuda.synchronize)
+    pf_ms = 1e3 * pf_el / args.steps
     n_inc = (nnz - c.n) + nbc                     # interior faces twice, boundary once
     # per incidence: d, the other cell's slot and flags, its phi; per cell: the
     # list bounds, G, phi, three components out
     g_alg = n_inc * (24 + 4 + 1 + 8) + c.n * (4 + 48 + 8 + 24)
     log(f"  gradient (GraPhi, 3 components) {g_ms:.3f} ms ({g_alg / g_ms / 1e6:.0f} GB/s "
-        f"algorithmic); GradP {p_ms:.3f} ms; Correct_Velocity {c_ms:.3f} ms")
+        f"algorithmic); GradP {p_ms:.3f} ms; Correct_Velocity {c_ms:.3f} ms; "
+        f"pressure face loop {pf_ms:.3f} ms")
     cpu_rows = None
     if not args.no_cpu:
         try:
@@ -718,7 +728,8 @@ def ingest(args):
         "assembly": {"ms": a_ms, "algorithmic_GBs": a_alg / a_ms / 1e6,
                      "values_upload_ms_at_50GBs": upload_ms},
         "gradient": {"graphi_ms": g_ms, "graphi_algorithmic_GBs": g_alg / g_ms / 1e6,
-                     "gradp_ms": p_ms, "correct_velocity_ms": c_ms},
+                     "gradp_ms": p_ms, "correct_velocity_ms": c_ms,
+                     "pressure_fractional_faces_ms": pf_ms},
         "roofline_graphi": {"kernel": "k_graphi (GraPhi, three components)", "bound": "hbm",
                             "achieved": g_alg / g_ms / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": g_alg / g_ms / 1e6 / HBM_PEAK_GBS,

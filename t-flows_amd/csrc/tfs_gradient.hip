@@ -252,34 +252,61 @@ k_cv_final(int nf_parts, int n_parts, const double *__restrict__ part, double *_
 
 // ---- Compute_Pressure_Fractional's face loop -----------------------------------
 
+// Per cell slot, once per call, what the faces gather: the velocity with the
+// pressure-gradient term put back, u + p_x vol / sav (the raw velocity for a
+// boundary cell, which the reference's boundary faces read), sav, p and the
+// density -- six values side by side, so a face reads two 48-byte records
+// instead of twenty scattered arrays.  Each value is the one the reference's
+// face expression forms, so the result stays bitwise.
+__global__ void __launch_bounds__(kThreads)
+k_pf_cells(int n, long long na, const double *__restrict__ vol, const double *__restrict__ sav,
+           const double *__restrict__ u, const double *__restrict__ v,
+           const double *__restrict__ w, const double *__restrict__ px,
+           const double *__restrict__ py, const double *__restrict__ pz,
+           const double *__restrict__ p, const double *__restrict__ den,
+           double *__restrict__ rec) {
+  for (long long c = blockIdx.x * (long long)kThreads + threadIdx.x; c < na;
+       c += (long long)gridDim.x * kThreads) {
+    double *r = rec + 6 * c;
+    if (c < n) {
+      r[0] = u[c] + px[c] * vol[c] / sav[c];
+      r[1] = v[c] + py[c] * vol[c] / sav[c];
+      r[2] = w[c] + pz[c] * vol[c] / sav[c];
+      r[3] = sav[c];
+    } else {
+      r[0] = u[c];
+      r[1] = v[c];
+      r[2] = w[c];
+      r[3] = 0.0;
+    }
+    r[4] = p[c];
+    r[5] = den[c];
+  }
+}
+
 // Compute_Pressure_Fractional.f90:50-131 for fluid materials without BUFFER
 // faces: per face the f-weighted density, the face velocity rebuilt from the
 // cells' velocities and pressure gradients (Rhie-Chow), the flux and A12;
 // a boundary face takes its boundary cell's velocity and has A12 = 0.
 __global__ void __launch_bounds__(kThreads)
 k_pf_faces(int n, long long nf, const int2 *__restrict__ faces, const double *__restrict__ f,
-           const double *__restrict__ sv, const double *__restrict__ vol,
-           const double *__restrict__ sav, const double *__restrict__ u,
-           const double *__restrict__ v, const double *__restrict__ w,
-           const double *__restrict__ px, const double *__restrict__ py,
-           const double *__restrict__ pz, const double *__restrict__ p,
-           const double *__restrict__ den, double *__restrict__ flux, double *__restrict__ a12) {
+           const double *__restrict__ sv, const double *__restrict__ rec,
+           double *__restrict__ flux, double *__restrict__ a12) {
   for (long long s = blockIdx.x * (long long)kThreads + threadIdx.x; s < nf;
        s += (long long)gridDim.x * kThreads) {
     const int2 fc = faces[s];
     const long long k1 = fc.x - 1;
     const long long k2 = fc.y > 0 ? (long long)(fc.y - 1) : (long long)n + (-fc.y - 1);
+    const double *r1 = rec + 6 * k1, *r2 = rec + 6 * k2;
     const double fs = f[s], sx = sv[s], sy = sv[nf + s], sz = sv[2 * nf + s];
-    const double dens = fs * den[k1] + (1.0 - fs) * den[k2];
+    const double dens = fs * r1[5] + (1.0 - fs) * r2[5];
     if (fc.y > 0) {
-      double us = fs * (u[k1] + px[k1] * vol[k1] / sav[k1]) +
-                  (1.0 - fs) * (u[k2] + px[k2] * vol[k2] / sav[k2]);
-      double vs = fs * (v[k1] + py[k1] * vol[k1] / sav[k1]) +
-                  (1.0 - fs) * (v[k2] + py[k2] * vol[k2] / sav[k2]);
-      double ws = fs * (w[k1] + pz[k1] * vol[k1] / sav[k1]) +
-                  (1.0 - fs) * (w[k2] + pz[k2] * vol[k2] / sav[k2]);
-      const double wsav = fs / sav[k1] + (1.0 - fs) / sav[k2];
-      const double dp = p[k1] - p[k2];
+      const double sav1 = r1[3], sav2 = r2[3];
+      double us = fs * r1[0] + (1.0 - fs) * r2[0];
+      double vs = fs * r1[1] + (1.0 - fs) * r2[1];
+      double ws = fs * r1[2] + (1.0 - fs) * r2[2];
+      const double wsav = fs / sav1 + (1.0 - fs) / sav2;
+      const double dp = r1[4] - r2[4];
       us = us + dp * sx * wsav;
       vs = vs + dp * sy * wsav;
       ws = ws + dp * sz * wsav;
@@ -287,7 +314,7 @@ k_pf_faces(int n, long long nf, const int2 *__restrict__ faces, const double *__
       const double a = dens * (sx * sx + sy * sy + sz * sz);
       a12[s] = a * wsav;
     } else {
-      flux[s] = dens * (u[k2] * sx + v[k2] * sy + w[k2] * sz);
+      flux[s] = dens * (r2[0] * sx + r2[1] * sy + r2[2] * sz);
       a12[s] = 0.0;
     }
   }
@@ -323,13 +350,15 @@ struct tfs_gradient {
   int *other = nullptr;
   unsigned char *kind = nullptr, *flag = nullptr;
   double *part = nullptr;     // 3 kParts partial maxima + 3 results
+  double *rec = nullptr;      // 6 values per cell slot for the pressure face loop
 };
 
 extern "C" int tfs_gradient_destroy(tfs_gradient *S) {
   if (!S) return TFS_OK;
   for (void *p : {(void *)S->ptr, (void *)S->inc, (void *)S->faces, (void *)S->bf, (void *)S->d,
                   (void *)S->g, (void *)S->xc, (void *)S->kind,
-                  (void *)S->part, (void *)S->id, (void *)S->other, (void *)S->flag})
+                  (void *)S->part, (void *)S->id, (void *)S->other, (void *)S->flag,
+                  (void *)S->rec})
     if (p) (void)hipFree(p);
   delete S;
   return TFS_OK;
@@ -417,6 +446,7 @@ extern "C" int tfs_gradient_create(int n_cells, int n_bnd_cells, long long n_fac
             up((void **)&S->id, id.data(), sizeof(double) * id.size()) &&
             up((void **)&S->other, other.data(), sizeof(int) * other.size()) &&
             up((void **)&S->flag, flag.data(), flag.size()) &&
+            up((void **)&S->rec, nullptr, 6 * sizeof(double) * na) &&
             (!xc || up((void **)&S->xc, xc, 3 * sizeof(double) * na));
   if (!ok) {
     (void)hipGetLastError();
@@ -516,10 +546,12 @@ extern "C" int tfs_pressure_fractional_device(tfs_gradient *S, const double *d_f
       !d_pz || !d_p || !d_den || !d_b ||
       (S->nf > 0 && (!d_f || !d_s || !d_flux || !d_a12)))
     return TFS_ERR_INVALID;
+  const long long na = (long long)S->n + S->nb;
+  hipLaunchKernelGGL(k_pf_cells, dim3(grid_of(na)), dim3(kThreads), 0, nullptr, S->n, na, d_vol,
+                     d_sav, d_u, d_v, d_w, d_px, d_py, d_pz, d_p, d_den, S->rec);
   if (S->nf > 0)
     hipLaunchKernelGGL(k_pf_faces, dim3(grid_of(S->nf)), dim3(kThreads), 0, nullptr, S->n, S->nf,
-                       S->faces, d_f, d_s, d_vol, d_sav, d_u, d_v, d_w, d_px, d_py, d_pz, d_p,
-                       d_den, d_flux, d_a12);
+                       S->faces, d_f, d_s, S->rec, d_flux, d_a12);
   hipLaunchKernelGGL(k_pf_rhs, dim3(grid_of(S->n)), dim3(kThreads), 0, nullptr, S->n, S->ptr,
                      S->inc, d_flux, d_b);
   TFS_G_HIP(hipGetLastError());
