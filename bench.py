@@ -76,6 +76,13 @@ def cpu_name():
     return os.popen("grep -m1 'model name' /proc/cpuinfo").read().split(":")[-1].strip()
 
 
+CPU_COMPILER = {
+    "reference": "the reference's own Fortran (Solvers_Mod, Matrix_Mod) built by oracle/Makefile "
+                 "with amdflang -O3 -fdefault-real-8 (its FAST flags), sequential build",
+    "port": "oracle/tfs_oracle.c built with gcc -O2 -ffp-contract=off",
+}
+
+
 def cpu_solver():
     """The reference's own solver (oracle/_ref, compiled from its sources with
     its FAST flags) if built, else the C restatement; both 1 thread."""
@@ -102,27 +109,35 @@ def cpu_full_solves(case, solve, tol, reps, x_gpu):
     import numpy as np
     kind, run = cpu_solver()
     times, out = [], None
+    run(case, solve["method"], solve["prec"], solve["cap"], tol)  # warm-up
     for _ in range(reps):
         out, sec = run(case, solve["method"], solve["prec"], solve["cap"], tol)
         times.append(sec)
     med = float(np.median(times))
     err = float(np.linalg.norm(x_gpu - out["x"]) / np.linalg.norm(out["x"]))
     return dict(value=1.0 / med, unit="solves/s", cores=1, kind=kind,
-                sample=f"{reps} full solves of the same system (median {med:.3f} s), "
-                       f"{out['niter']} iterations, 1 thread on {cpu_name()}",
+                sample=f"{reps} full solves of the same system after a warm-up (median "
+                       f"{med:.3f} s), {out['niter']} iterations, 1 thread on {cpu_name()}",
+                compiler=CPU_COMPILER.get(kind),
                 niter=out["niter"], seconds_per_solve=med), err
 
 
-def cpu_iteration_sample(case, solve, iters, gpu_niter):
-    """Seconds per iteration of the reference solver from a solve capped at
-    `iters` iterations (tol 0), scaled to the GPU solve's iteration count."""
+def cpu_iteration_sample(case, solve, iters, gpu_niter, reps=1):
+    """Seconds per iteration of the reference solver from solves capped at
+    `iters` iterations (tol 0; one warm-up, then the median of `reps`),
+    scaled to the GPU solve's iteration count."""
+    import numpy as np
     kind, run = cpu_solver()
-    _, sec = run(case, solve["method"], solve["prec"], iters, 0.0)
+    if reps > 1:
+        run(case, solve["method"], solve["prec"], iters, 0.0)  # warm-up
+    secs = [run(case, solve["method"], solve["prec"], iters, 0.0)[1] for _ in range(reps)]
+    sec = float(np.median(secs))
     per_solve = sec / (iters + 1) * gpu_niter
     return dict(value=1.0 / per_solve, unit="solves/s", cores=1, kind=kind,
-                sample=f"one solve capped at {iters} iterations ({sec:.2f} s incl. Prec_Form), "
-                       f"scaled to {gpu_niter} iterations, 1 thread on {cpu_name()}",
-                seconds_per_solve=per_solve)
+                sample=f"solves capped at {iters} iterations (median {sec:.2f} s of {reps} after "
+                       f"a warm-up, incl. Prec_Form), scaled to {gpu_niter} iterations, "
+                       f"1 thread on {cpu_name()}",
+                compiler=CPU_COMPILER.get(kind), seconds_per_solve=per_solve)
 
 
 def timed(step, steps, warmup, sync, label=None):
@@ -185,11 +200,11 @@ def spmv_roofline(A, x, y, n, nnz, reps=50, inloop_ms=None, traffic_profile=None
 
 
 def sweep_bytes(n, ntri):
-    """Algorithmic HBM bytes of one preconditioner sweep over a triangle of
-    ntri entries: col + val per entry, row pointer, right-hand-side index,
-    factor, right-hand side, result, and the sentinel stored for the next
-    sweep (the dependencies' own results are re-read on chip, not counted)."""
-    return 12 * ntri + 4 * (n + 1) + (4 + 8 + 8 + 8 + 8) * n
+    """Algorithmic HBM bytes of one FP64 preconditioner sweep over a triangle
+    of ntri entries: col + val per entry (12), and per row the right-hand
+    side, the factor, the result and a row pointer's share -- the forward +
+    backward pair is 12 (nnz - N) + 64 N with the scale pass fused."""
+    return 12 * ntri + 32 * n
 
 
 def sweep_roofline(A, c, inloop_us, traffic_profile=None):
@@ -318,8 +333,23 @@ def single_solve(args, k):
     return out
 
 
+def fp32_spmv_bytes(n, nnz):
+    """Algorithmic bytes of one FP32 SpMV (FP32 values and vectors, int32
+    indices, x counted once): 8 nnz + 12 N + 4."""
+    return 8 * nnz + 12 * n + 4
+
+
+def fp32_sweep_bytes(n, nnz):
+    """Algorithmic bytes of one FP32 preconditioner sweep: half of the
+    forward + backward pair 8 (nnz - N) + 32 N (the FP64 pair's
+    12 (nnz - N) + 64 N with values and vectors in FP32)."""
+    return 0.5 * (8 * (nnz - n) + 32 * n)
+
+
 def mixed_vs_fp64(args):
-    """Workload 3: FP32-inner refinement against the plain FP64 solve."""
+    """Workload 3, the headline: the FP32-inner refinement timed over `steps`
+    solves of the Poisson system (the value), the plain FP64 Cg drop-in timed
+    beside it outside the steps region."""
     import numpy as np
     import torch
     from tflows_amd import cases, mesh, solvers
@@ -330,8 +360,11 @@ def mixed_vs_fp64(args):
     c = mesh.make_case(**kw)
     log(f"  n={c.n} nnz={c.nnz} ({time.perf_counter() - t0:.1f}s)")
     tol = solve["rtol"] * float(np.linalg.norm(c.b))
+    t0 = time.perf_counter()
     A = solvers.Matrix(c.row, c.col, c.dia, c.val)
-    A.set_refinement(solvers.TFS_REFINE_FCG if args.refine == "fcg" else solvers.TFS_REFINE_DEFECT)
+    log(f"  matrix layouts built and uploaded ({time.perf_counter() - t0:.1f}s)")
+    A.set_refinement({"reliable": solvers.TFS_REFINE_RELIABLE, "fcg": solvers.TFS_REFINE_FCG,
+                      "defect": solvers.TFS_REFINE_DEFECT}[args.refine])
     if args.inner_block:
         A.set_inner_block(args.inner_block)
     info = A.info()
@@ -343,63 +376,96 @@ def mixed_vs_fp64(args):
     def step64():
         r.copy_(b)
         x.zero_()
-        A.values_changed()
+        A.values_changed()  # Prec_Form runs inside the solve, as in the reference
         return solvers.Cg(A, x, r, solve["prec"], solve["cap"], tol)
 
-    def step32(x_rtol=0.0):
+    def step32():
         r.copy_(b)
         x.zero_()
-        A.values_changed()
-        return solvers.solve_mixed(A, x, r, solve["prec"], 200, tol, args.inner_rtol, x_rtol)
+        A.values_changed()  # the FP32 copies and factor are rebuilt in the solve
+        return solvers.solve_mixed(A, x, r, solve["prec"], 200, tol, args.inner_rtol)
 
-    e64, lat64, res64 = timed(step64, args.steps, args.warmup, torch.cuda.synchronize, "fp64")
-    x64 = x.cpu().numpy()
-    log(f"  fp64: {1e3 * e64 / args.steps:.1f} ms/solve, niter={res64[0]}, fin={res64[2]:.3e}")
-    e32, lat32, res32 = timed(step32, args.steps, args.warmup, torch.cuda.synchronize, "mixed")
+    lib = solvers._lib()
+    t0 = time.perf_counter()
+    step32()  # builds the reordered inner copy (host order + device layouts)
+    torch.cuda.synchronize()
+    log(f"  inner copy built, first solve {time.perf_counter() - t0:.1f}s")
+    for k in range(max(0, args.warmup - 1)):
+        step32()
+    torch.cuda.synchronize()
+    # per-class kernel times of one solve; the dominant class is then timed
+    # with HIP events around each of its launches through the timed region
+    bd = breakdown(lib, step32)
+    dom_kc = 1 if bd["sweeps"]["ms_per_solve"] > bd["spmv"]["ms_per_solve"] else 0
+    lib.tfs_timing_start(dom_kc)
+    e32, lat32, res32 = timed(step32, args.steps, 0, torch.cuda.synchronize, "mixed")
+    dom_ms, dom_n = ctypes.c_double(), ctypes.c_longlong()
+    lib.tfs_timing_stop(ctypes.byref(dom_ms), ctypes.byref(dom_n))
+    dom_launch_ms = dom_ms.value / max(dom_n.value, 1)
     x32 = x.cpu().numpy()
     steps_out, inner, _, fin32 = res32
+    log(f"  mixed: {1e3 * e32 / args.steps:.1f} ms/solve, {steps_out} residual replacements, "
+        f"{inner} FP32 iterations, fin={fin32:.3e} (tol {tol:.3e}); breakdown {bd}")
+    # the plain FP64 drop-in on the same system (outside the steps region)
+    e64, lat64, res64 = timed(step64, args.fp64_reps, 1, torch.cuda.synchronize, "fp64")
+    x64 = x.cpu().numpy()
+    bd64 = breakdown(lib, step64)
+    ms64 = 1e3 * e64 / args.fp64_reps
+    ms32 = 1e3 * e32 / args.steps
     err = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
-    speedup = e64 / e32
-    log(f"  mixed: {1e3 * e32 / args.steps:.1f} ms/solve, {steps_out} refinement steps, "
-        f"{inner} inner iterations, fin={fin32:.3e}; speedup {speedup:.2f}x, rel-L2 vs FP64 "
-        f"{err:.2e}")
-    # the same with the correction criterion (FP64-level accuracy)
-    x_rtol = solve["rtol"]
-    ex, _, resx = timed(lambda: step32(x_rtol), args.steps, 0, torch.cuda.synchronize, "mixed+x")
-    errx = float(np.linalg.norm(x.cpu().numpy() - x64) / np.linalg.norm(x64))
-    log(f"  mixed, ||e|| <= {x_rtol:g} ||x||: {1e3 * ex / args.steps:.1f} ms/solve, "
-        f"{resx[0]} steps, {resx[1]} inner; speedup {e64 / ex:.2f}x, rel-L2 vs FP64 {errx:.2e}")
-    lib = solvers._lib()
-    bd = breakdown(lib, step64)
+    speedup = ms64 / ms32
+    log(f"  fp64: {ms64:.1f} ms/solve, niter={res64[0]}, fin={res64[2]:.3e}; speedup "
+        f"{speedup:.2f}x, rel-L2 mixed vs FP64 {err:.2e}")
+    if dom_kc == 0:
+        algo, kname = fp32_spmv_bytes(c.n, c.nnz), \
+            "k_ell_diff (FP32 sliced-ELL SpMV, difference form)"
+        prof = "k_ell_diff_poisson256_counters"
+        per = "8 nnz + 12 N + 4 (SpMV, FP32 values and vectors)"
+    else:
+        algo, kname = fp32_sweep_bytes(c.n, c.nnz), \
+            "k_sweep_syncfree<float> (FP32 block multi-colour DIC sweeps)"
+        prof = "k_sweep_f32_poisson256_counters"
+        per = "(8 (nnz - N) + 32 N) / 2 per sweep (FP32 DIC forward + backward pair)"
+    achieved = algo / (dom_launch_ms * 1e-3) / 1e9
+    roof = {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(prof), "algorithmic_bytes": algo,
+            "algorithmic_bytes_formula": per, "launch_us": 1e3 * dom_launch_ms,
+            "launches_timed": dom_n.value, "timing": "HIP events around every launch of the "
+            "kernel class in the timed solves, on the matrix's stream"}
     y = torch.empty_like(b)
-    roof = spmv_roofline(A, x, y, c.n, c.nnz)
+    roof64 = spmv_roofline(A, x, y, c.n, c.nnz)
     cpu = None
     if not args.no_cpu:
-        log("timing the CPU baseline (iteration sample)...")
-        cpu = cpu_iteration_sample(c, solve, args.cpu_iters, res64[0])
-        log(f"  cpu {cpu['value']:.4f} solves/s ({cpu['kind']})")
+        log(f"timing the CPU baseline ({args.cpu_reps} iteration samples after a warm-up)...")
+        cpu = cpu_iteration_sample(c, solve, args.cpu_iters, res64[0], reps=args.cpu_reps)
+        log(f"  cpu {cpu['value']:.5f} solves/s ({cpu['kind']})")
     lat = np.array(lat32) * 1e3
     out = line(args, args.steps / e32, e32,
-               f"Poisson {kw['nx']}^3 stretched to aspect 100, FP32-inner PCG "
-               f"(DIC) in FP64 refinement ({args.refine} outer loop) to rel 1e-10, plain FP64 "
-               f"Cg beside it", {
+               f"Poisson {kw['nx']}^3 stretched to aspect 100 (tanh beta "
+               f"{kw['beta']:.4f}), zero-mean RHS, cell 1 pinned: FP32-inner PCG (DIC, "
+               f"block multi-colour inner order) with reliable FP64 updates to rel 1e-10; "
+               f"plain FP64 Cg + DIC beside it", {
                    "latency_ms": {"p50": float(np.percentile(lat, 50)),
                                   "p99": float(np.percentile(lat, 99))},
                    "roofline": roof,
-                   "fp64": {"solves_per_s": args.steps / e64, "ms_per_solve": 1e3 * e64 / args.steps,
-                            "niter": res64[0], "fin_res": res64[2],
-                            "breakdown_ms_per_solve": {k2: v["ms_per_solve"] for k2, v in bd.items()}},
-                   "mixed": {"refinement_steps": steps_out, "inner_iterations": inner,
-                             "fin_res": fin32, "inner_rtol": args.inner_rtol,
-                             "outer": args.refine,
-                             "inner_order": "coloured" if A.info()["inner_colours"] else
-                             "reference", "inner_colours": A.info()["inner_colours"],
-                             "rel_l2_vs_fp64": err},
-                   "mixed_x_rtol": {"x_rtol": x_rtol, "ms_per_solve": 1e3 * ex / args.steps,
-                                    "refinement_steps": resx[0], "inner_iterations": resx[1],
-                                    "speedup_vs_fp64": e64 / ex, "rel_l2_vs_fp64": errx},
-                   "speedup_vs_fp64": speedup, "cpu_baseline": cpu})
-    out["dtype"] = "f32 inner / f64 outer"
+                   "roofline_fp64_spmv": roof64,
+                   "speedup_vs_fp64": speedup, "rel_l2_vs_fp64": err,
+                   "fp64": {"solves_per_s": 1e3 / ms64, "ms_per_solve": ms64,
+                            "repetitions": args.fp64_reps, "niter": res64[0],
+                            "fin_res": res64[2],
+                            "breakdown_ms_per_solve": {k2: v["ms_per_solve"]
+                                                       for k2, v in bd64.items()}},
+                   "mixed": {"residual_replacements": steps_out, "fp32_iterations": inner,
+                             "fin_res": fin32, "outer": args.refine,
+                             "inner_rtol": args.inner_rtol,
+                             "inner_block": A.info()["inner_block"],
+                             "inner_colours": A.info()["inner_colours"],
+                             "inner_levels": A.info()["inner_levels"],
+                             "breakdown_ms_per_solve": {k2: v["ms_per_solve"]
+                                                        for k2, v in bd.items()}},
+                   "cpu_baseline": cpu})
+    out["dtype"] = "f32 storage, f64 arithmetic (inner); f64 residual and solution"
     out["config"].update(n_cells=c.n, nnz=c.nnz, rtol=solve["rtol"], tol=tol,
                          levels_lower=info["levels_lower"], levels_upper=info["levels_upper"])
     return out
@@ -825,11 +891,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", type=int, default=2, choices=[1, 2, 3, 4, 5])
+    ap.add_argument("--workload", type=int, default=3, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--ingest", action="store_true",
                     help="instead of a solve workload: the device topology and assembly "
                          "on the 128^3 momentum mesh")
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--fp64-reps", type=int, default=2,
+                    help="workload 3: timed FP64 solves beside the refinement (after one warm-up)")
     ap.add_argument("--cpu-iters", type=int, default=10,
                     help="workload 3: iterations in the CPU sample")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
@@ -840,7 +908,7 @@ def main():
                     help="inner FP32 tolerance relative to the outer residual (default 1e-2 "
                          "for workload 3, 1e-3 for the batch of workload 4: the fastest "
                          "measured for each)")
-    ap.add_argument("--refine", choices=["fcg", "defect"], default="fcg",
+    ap.add_argument("--refine", choices=["reliable", "fcg", "defect"], default="reliable",
                     help="workloads 3-4: outer loop of the refinement")
     ap.add_argument("--inner-block", type=int, default=0,
                     help="workloads 3-4: rows per colour block of the inner order (0: default)")
@@ -856,7 +924,7 @@ def main():
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.inner_rtol is None:
-        args.inner_rtol = 1e-3 if args.workload == 4 else 1e-2
+        args.inner_rtol = 1e-3 if args.workload == 4 else 0.0  # 0: the solver's default
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               f"(the solver path runs on a single MI355X)", file=sys.stderr)

@@ -97,6 +97,13 @@ int tfs_layout_colour(int n, const int *row, const int *col, int *perm, int *col
 /* The same with runs of `block` consecutive rows coloured as units. */
 int tfs_layout_block_colour(int n, const int *row, const int *col, int block, int *perm,
                             int *colour);
+/* The block multi-colour order of the FP32 inner solves: blocks of at most
+ * `block` rows grown breadth-first (seeds in index order, neighbours in
+ * ascending index), the block graph coloured greedily in growth order, perm =
+ * rows by colour, then block, then discovery order; colour[i] and block_of[i]
+ * per row.  Returns the number of colours. */
+int tfs_layout_block_colour_bfs(int n, const int *row, const int *col, int block, int *perm,
+                                int *colour, int *block_of);
 /* The layouts the solver builds for a matrix (0-based CRS): s_orig = caller
  * row at each S (reverse Cuthill-McKee) position; level of each caller row in
  * the lower dependency DAG of the caller's numbering (forward sweep) and in

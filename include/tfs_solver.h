@@ -115,6 +115,8 @@ typedef struct {
   int inner_colours;    /* colours of the inner order (0 until first built)  */
   int spmv_rows;        /* SpMV shape: rows per row block (one wavefront)    */
   int spmv_waves;       /*             wavefronts per workgroup              */
+  int inner_block;      /* rows per block of the inner order (1 = points)    */
+  int inner_levels;     /* depth of the inner forward sweep (0 until built)  */
 } tfs_matrix_info;
 int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info);
 
@@ -128,17 +130,24 @@ int tfs_solve_device(tfs_matrix *A, int method, int prec, double *d_x, double *d
 
 /* ---- FP32-inner solve inside FP64 iterative refinement -------------------------
  * Not in the reference: values and inner Krylov vectors stream in FP32 while
- * the outer residual b - A x, the solution update and every dot product stay
- * FP64 on the device.  r = b - A x;  repeat { solve A z = r by FP32 PCG (the
- * prec factor recomputed in FP32 from FP32 values) to ||r_inner|| <
- * inner_rtol ||r||;  update x and r from z (see tfs_matrix_set_refinement) }
- * until ||r|| < tol.
- * niter: in = cap on refinement steps; out = steps taken (cap+1 if tol was not
- * reached, which also happens when FP32 stops reducing the defect);
- * inner_iters = FP32 PCG iterations over all steps; ini_res = ||b - A x0||;
- * fin_res = ||b - A x|| (the true FP64 residual, left in r1).
+ * the residual b - A x, the solution and every dot product stay FP64 on the
+ * device.  The outer loop is set by tfs_matrix_set_refinement; by default
+ * (TFS_REFINE_RELIABLE) one FP32 PCG recurrence (the prec factor recomputed
+ * in FP32 from FP32 values) runs until its residual has fallen by
+ * inner_rtol (0.1 if <= 0) against the last true residual, x += the FP64 sum
+ * of its steps, r = b - A x in FP64, and the recurrence continues from that
+ * r -- until ||r|| < tol (a segment that has not reduced its residual by
+ * inner_rtol within 20000 FP32 iterations ends the solve, not converged).
+ * niter: in = cap on refinement steps (residual replacements); out = steps
+ * taken (cap+1 if tol was not reached, which also happens when FP32 stops
+ * reducing the true residual); inner_iters = FP32 PCG iterations over all
+ * steps; ini_res = ||b - A x0||; fin_res = ||b - A x|| (the true FP64
+ * residual, left in r1).
  * x_rtol > 0 adds a stopping condition on the solution: the last update of
- * x must also satisfy ||dx|| <= x_rtol ||x||.  With the flexible-CG outer
+ * x must also satisfy ||dx|| <= x_rtol ||x||.  With reliable updates the FP32
+ * recurrence cannot see residual components below FP32 resolution (the error
+ * along the smallest eigenvectors), so once it has reached tol the
+ * flexible-CG loop continues from its x to meet the criterion.  With the flexible-CG outer
  * loop (the default) this gives the FP64 solver's accuracy at the same tol
  * (x_rtol = tol / ||b||); defect correction's update omits the error it
  * cannot resolve in FP32 (along the smallest eigenvectors when
@@ -170,20 +179,29 @@ int tfs_matrix_set_spmv_shape(tfs_matrix *A, int rows_per_block, int waves_per_g
  *                        the refinement still converges to the FP64 tol. */
 enum { TFS_INNER_REFERENCE = 0, TFS_INNER_COLOURED = 1 };
 int tfs_matrix_set_inner_order(tfs_matrix *A, int order);
-/* Block multi-colouring of the inner order: runs of `rows` consecutive rows
- * (of the RCM order) are coloured as units, keeping their couplings inside
- * the factor (a stronger preconditioner than 1-row colouring, with at most
- * `rows` sweep levels per colour).  1 = plain colouring. */
+/* Block multi-colouring of the inner order: blocks of at most `rows` rows
+ * grown breadth-first over the pattern are coloured as units, keeping their
+ * couplings inside the factor (a stronger preconditioner than 1-row
+ * colouring, with a block's internal depth of sweep levels per colour).
+ * 1 = plain colouring. */
 int tfs_matrix_set_inner_block(tfs_matrix *A, int rows);
 /* Outer loop of the refinement:
- *   TFS_REFINE_FCG     (default) flexible CG in FP64 with the FP32 inner solve
- *                      as its (variable) preconditioner: x and r are updated
- *                      along A-orthogonalised directions, so convergence does
- *                      not need cond(A) * 2^-24 < 1;
+ *   TFS_REFINE_RELIABLE (default) one FP32 CG recurrence with reliable FP64
+ *                      updates: its directions are summed into x in FP64
+ *                      every iteration, and whenever its residual has fallen
+ *                      by inner_rtol the FP64 residual b - A x replaces it
+ *                      (direction and rho kept, so nothing restarts).  The
+ *                      FP32 operator is applied in difference form
+ *                      (sum a_ij (x_j - x_i) + rowsum_i x_i), whose rounding
+ *                      does not grow with cond(A): the iteration count stays
+ *                      that of FP64 CG with the same preconditioner;
+ *   TFS_REFINE_FCG     flexible CG in FP64 with each FP32 inner solve (to
+ *                      inner_rtol, restarted per step) as its variable
+ *                      preconditioner;
  *   TFS_REFINE_DEFECT  classical defect correction x += z, r = b - A x, which
  *                      contracts by about cond(A) * 2^-24 per step.
- * Both stop on the true FP64 residual. */
-enum { TFS_REFINE_FCG = 0, TFS_REFINE_DEFECT = 1 };
+ * All stop on the true FP64 residual. */
+enum { TFS_REFINE_FCG = 0, TFS_REFINE_DEFECT = 1, TFS_REFINE_RELIABLE = 2 };
 int tfs_matrix_set_refinement(tfs_matrix *A, int outer);
 
 /* ---- batched solves ----------------------------------------------------------------

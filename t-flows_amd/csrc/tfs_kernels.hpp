@@ -215,8 +215,8 @@ __global__ void __launch_bounds__(kRedThreads)
 k_pupdate(int n, const T *__restrict__ z, T *__restrict__ p, const KState *st) {
   if (solve_done(st)) return;
   const bool first = st->iter == 0;
-  const T beta = (T)st->beta;
-  TFS_GRID_STRIDE(i, n) p[i] = first ? z[i] : z[i] + beta * p[i];
+  const double beta = st->beta;
+  TFS_GRID_STRIDE(i, n) p[i] = first ? z[i] : (T)((double)z[i] + beta * (double)p[i]);
 }
 
 // x += alpha p; r -= alpha q; partial ||r||^2  (Cg.f90:159-172)
@@ -354,6 +354,123 @@ k_add_correction(int n, double *__restrict__ x, const float *__restrict__ e,
   TFS_GRID_STRIDE(i, n) {
     const int j = idx ? idx[i] : i;  // idx: a permutation, so no two lanes share a j
     x[j] = x[j] + (double)e[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 inner operator of the reliable-update refinement, in difference form:
+//   y_i = s_i x_i + sum_{j != i} a_ij (x_j - x_i),   s_i = sum_j a_ij,
+// with a_ij rounded to FP32 and s_i summed in FP64 and then rounded.  The
+// rounding of each product then scales with the local difference x_j - x_i
+// rather than with x_j, so on smooth vectors -- the slow modes of a Krylov
+// solve of a diffusion operator -- the FP32 operator's error stays at about
+// eps32 relative to A x instead of eps32 * cond(A).  That is what lets an FP32
+// Krylov process with FP64 residual replacement keep the FP64 iteration count
+// on ill-conditioned Poisson systems (in the usual sum-of-products form the
+// true residual stalls once eps32 * cond(A) ~ 1).
+// Storage: sliced ELL, 64 rows per slice (one row per lane), the slice's width
+// its longest off-diagonal run, entries column-major inside the slice (so
+// every col/val load of a wave is one contiguous 256-byte run); padding
+// entries carry the row's own column (or 0 past the last row) and value 0.
+// Arithmetic is FP64 (free in a bandwidth-bound kernel); only storage is FP32.
+constexpr int kEllWaves = 4;  // slices per workgroup (one wavefront each)
+constexpr int kEllBatch = 8;  // entries per lane loaded before their gathers
+
+template <bool DOT>
+__global__ void __launch_bounds__(kEllWaves * kWave)
+k_ell_diff(int n, int nslice, const long long *__restrict__ off, const int *__restrict__ col,
+           const float *__restrict__ val, const float *__restrict__ rsum,
+           const float *__restrict__ x, float *__restrict__ y, const KState *st,
+           double *partials) {
+  if (st && solve_done(st)) return;
+  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  double red = 0.0;
+  for (int s = blockIdx.x * kEllWaves + wv; s < nslice; s += gridDim.x * kEllWaves) {
+    const int su = __builtin_amdgcn_readfirstlane(s);
+    const long long e0 = off[su];
+    const int w = (int)((off[su + 1] - e0) / kWave);
+    const int i = su * kWave + lane;
+    const bool live = i < n;
+    const double xi = live ? (double)x[i] : 0.0;
+    double acc = live ? (double)rsum[i] * xi : 0.0;
+    const long long base = e0 + lane;
+    for (int j0 = 0; j0 < w; j0 += kEllBatch) {
+      int c[kEllBatch];
+      float a[kEllBatch], xv[kEllBatch];
+#pragma unroll
+      for (int k = 0; k < kEllBatch; ++k)
+        if (j0 + k < w) {
+          c[k] = col[base + (long long)(j0 + k) * kWave];
+          a[k] = val[base + (long long)(j0 + k) * kWave];
+        }
+#pragma unroll
+      for (int k = 0; k < kEllBatch; ++k)
+        if (j0 + k < w) xv[k] = x[c[k]];
+#pragma unroll
+      for (int k = 0; k < kEllBatch; ++k)
+        if (j0 + k < w) acc = acc + (double)a[k] * ((double)xv[k] - xi);
+    }
+    if (live) {
+      const float yi = (float)acc;
+      y[i] = yi;
+      if (DOT) red += xi * (double)yi;
+    }
+  }
+  if (DOT) store_partial<kEllWaves * kWave>(red, partials);
+}
+
+// The sliced-ELL values (FP32) from the FP64 caller values: src[e] is the
+// caller entry of slot e, or -1 for padding.
+__global__ void __launch_bounds__(kRedThreads)
+k_ell_values(long long len, const int *__restrict__ src, const double *__restrict__ val,
+             float *__restrict__ eval) {
+  for (long long e = (long long)blockIdx.x * kRedThreads + threadIdx.x; e < len;
+       e += (long long)gridDim.x * kRedThreads) {
+    const int k = src[e];
+    eval[e] = k >= 0 ? (float)val[k] : 0.0f;
+  }
+}
+
+// Row sums of the operator, summed in FP64 over the caller's entries of each
+// S row, rounded once.
+__global__ void __launch_bounds__(kRedThreads)
+k_row_sums(int n, const int *__restrict__ s_row, const int *__restrict__ s_src,
+           const double *__restrict__ val, float *__restrict__ rsum) {
+  TFS_GRID_STRIDE(k, n) {
+    double s = 0.0;
+    for (int e = s_row[k]; e < s_row[k + 1]; ++e) s += val[s_src[e]];
+    rsum[k] = (float)s;
+  }
+}
+
+// Reliable-update iteration tail: c += alpha p (the FP32 direction folded into
+// an FP64 correction every iteration -- an FP32 running correction would
+// carry a rounding error that A amplifies by its largest eigenvalue),
+// r -= alpha q (FP32 recurrence residual), partial ||r||^2.
+__global__ void __launch_bounds__(kRedThreads)
+k_axpy_mixed(int n, double *__restrict__ c, float *__restrict__ r, const float *__restrict__ p,
+             const float *__restrict__ q, const KState *st, double *partials) {
+  if (solve_done(st)) return;
+  const double alpha = st->alpha;
+  double s = 0.0;
+  TFS_GRID_STRIDE(i, n) {
+    c[i] = c[i] + alpha * (double)p[i];
+    const float ri = (float)((double)r[i] - alpha * (double)q[i]);
+    r[i] = ri;
+    s += (double)ri * (double)ri;
+  }
+  store_partial<kRedThreads>(s, partials);
+}
+
+// x[idx[i]] += c[i]; c[i] = 0 (the correction of one reliable-update segment
+// moved into the solution; idx is a permutation or null).
+__global__ void __launch_bounds__(kRedThreads)
+k_fold_correction(int n, double *__restrict__ x, double *__restrict__ c,
+                  const int *__restrict__ idx) {
+  TFS_GRID_STRIDE(i, n) {
+    const int j = idx ? idx[i] : i;
+    x[j] = x[j] + c[i];
+    c[i] = 0.0;
   }
 }
 
@@ -620,8 +737,18 @@ struct Tri {
 
 enum SweepOp { OP_FACTOR = 0, OP_FWD, OP_BWD };
 
-template <typename T>
-__device__ __forceinline__ T tiny() { return (T)kTiny; }
+// One term of a row's sum, in FP64 whatever the storage type: the factor
+// subtracts d_j a_ij^2 (Prec_Form.f90:36), the sweeps a_ij x_j
+// (Prec_Solve.f90:49,63).  For FP64 storage this is the reference's own
+// arithmetic; FP32 storage (the refinement's inner solves) is widened here,
+// which keeps the inner preconditioner a fixed linear operator to FP32
+// storage rounding -- with FP32 arithmetic the sweeps' rounding made the FP32
+// CG recurrence lose up to 40 % more iterations than FP64 on red-black order.
+template <int OP, typename T>
+__device__ __forceinline__ double sweep_term(double s, T a, T v) {
+  const double av = (double)a, vv = (double)v;
+  return OP == OP_FACTOR ? s - vv * av * av : s - av * vv;
+}
 
 // One row of a level (all its dependencies are in earlier levels).  Three
 // memory rounds: the row pointers and what the position indexes; the first
@@ -632,7 +759,7 @@ template <typename T, int OP>
 __device__ __forceinline__ void sweep_row(const Tri<T> &Tr, T *d, const T *b, T *x, int p) {
   T *out = OP == OP_FACTOR ? d : x;
   const int e0 = Tr.ptr[p], e1 = Tr.ptr[p + 1];
-  T s, dp = T(0);
+  double s, dp = 0.0;  // FP64 arithmetic whatever the storage type
   int src = 0;
   if (OP == OP_FACTOR) {
     s = Tr.diag[p];
@@ -653,15 +780,12 @@ __device__ __forceinline__ void sweep_row(const Tri<T> &Tr, T *d, const T *b, T 
 #pragma unroll
   for (int k = 0; k < kPrefetch; ++k)
     if (e0 + k < e1) v[k] = out[c[k]];
-  if (OP == OP_BWD) s = s / (dp + tiny<T>());
+  if (OP == OP_BWD) s = s / (dp + kTiny);
 #pragma unroll
   for (int k = 0; k < kPrefetch; ++k)
-    if (e0 + k < e1) s = OP == OP_FACTOR ? s - v[k] * a[k] * a[k] : s - a[k] * v[k];
-  for (int e = e0 + kPrefetch; e < e1; ++e) {
-    const T w = out[Tr.col[e]], av = Tr.val[e];
-    s = OP == OP_FACTOR ? s - w * av * av : s - av * w;
-  }
-  out[p] = OP == OP_FACTOR ? T(1) / s : s * dp;
+    if (e0 + k < e1) s = sweep_term<OP>(s, a[k], v[k]);
+  for (int e = e0 + kPrefetch; e < e1; ++e) s = sweep_term<OP>(s, Tr.val[e], out[Tr.col[e]]);
+  out[p] = (T)(OP == OP_FACTOR ? 1.0 / s : s * dp);
 }
 
 template <typename T, int OP>
@@ -715,7 +839,7 @@ template <typename T>
 struct SweepRow {
   int p, e0, len, src, used;
   bool done;
-  T s, dp;
+  double s, dp;  // FP64 arithmetic whatever the storage type
   int pc[kPrefetch];  // the first kPrefetch entries: column positions ...
   T pv[kPrefetch];    // ... and values
 };
@@ -727,7 +851,7 @@ __device__ __forceinline__ void sweep_load1(SweepRow<T> &r, int p, int n, const 
   r.p = p;
   r.done = p >= n;
   r.e0 = r.len = r.used = r.src = 0;
-  r.s = r.dp = T(0);
+  r.s = r.dp = 0.0;
   if (r.done) return;
   r.e0 = Tr.ptr[p];
   r.len = Tr.ptr[p + 1] - r.e0;
@@ -778,7 +902,7 @@ __device__ __forceinline__ bool sweep_pass(SweepRow<T> &r, const Tri<T> &Tr, T *
       if (Sentinel<T>::is(v[k])) {
         blocked = true;
       } else {
-        r.s = OP == OP_FACTOR ? r.s - v[k] * r.pv[k] * r.pv[k] : r.s - r.pv[k] * v[k];
+        r.s = sweep_term<OP>(r.s, r.pv[k], v[k]);
         r.used = k + 1;
       }
     }
@@ -788,13 +912,12 @@ __device__ __forceinline__ bool sweep_pass(SweepRow<T> &r, const Tri<T> &Tr, T *
     if (Sentinel<T>::is(w)) {
       blocked = true;
     } else {
-      const T a = Tr.val[e];
-      r.s = OP == OP_FACTOR ? r.s - w * a * a : r.s - a * w;
+      r.s = sweep_term<OP>(r.s, Tr.val[e], w);
       r.used = e - r.e0 + 1;
     }
   }
   if (blocked) return false;
-  coherent_store(&out[r.p], OP == OP_FACTOR ? T(1) / r.s : r.s * r.dp);
+  coherent_store(&out[r.p], (T)(OP == OP_FACTOR ? 1.0 / r.s : r.s * r.dp));
   r.done = true;
   return true;
 }
@@ -802,7 +925,7 @@ __device__ __forceinline__ bool sweep_pass(SweepRow<T> &r, const Tri<T> &Tr, T *
 template <typename T, int OP>
 __device__ __forceinline__ void sweep_give_up(SweepRow<T> &r, T *out, int *err) {
   coherent_store(err, 1);
-  if (!r.done) coherent_store(&out[r.p], OP == OP_FACTOR ? T(1) / r.s : r.s * r.dp);
+  if (!r.done) coherent_store(&out[r.p], (T)(OP == OP_FACTOR ? 1.0 / r.s : r.s * r.dp));
   r.done = true;
 }
 
@@ -820,7 +943,7 @@ __device__ __forceinline__ void sweep_give_up(SweepRow<T> &r, T *out, int *err) 
 // device-coherent polls, which bypass the XCD's L2.
 template <typename T, int OP>
 __device__ __forceinline__ void sweep_row(SweepRow<T> &r, const Tri<T> &Tr, T *out, int *err) {
-  if (OP == OP_BWD && !r.done) r.s = r.s / (r.dp + tiny<T>());
+  if (OP == OP_BWD && !r.done) r.s = r.s / (r.dp + kTiny);
   unsigned int spins = 0;
   sweep_pass<T, OP>(r, Tr, out, true);
   while (__any(!r.done)) {

@@ -234,11 +234,94 @@ int colour_order(int n, const int *row, const int *col, std::vector<int> &perm,
   return ncol;
 }
 
+int block_colour_order(int n, const int *row, const int *col, int block, std::vector<int> &perm,
+                       std::vector<int> *colour_out, std::vector<int> *block_out) {
+  // symmetrised adjacency, each row's neighbours ascending and distinct
+  std::vector<int> deg(n + 1, 0);
+  for (int i = 0; i < n; ++i)
+    for (int k = row[i]; k < row[i + 1]; ++k)
+      if (col[k] != i) {
+        deg[i + 1]++;
+        deg[col[k] + 1]++;
+      }
+  for (int i = 0; i < n; ++i) deg[i + 1] += deg[i];
+  std::vector<int> adj(deg[n]), fill(deg.begin(), deg.end() - 1);
+  for (int i = 0; i < n; ++i)
+    for (int k = row[i]; k < row[i + 1]; ++k)
+      if (col[k] != i) {
+        adj[fill[i]++] = col[k];
+        adj[fill[col[k]]++] = i;
+      }
+  for (int i = 0; i < n; ++i) std::sort(adj.begin() + deg[i], adj.begin() + deg[i + 1]);
+  block = std::max(1, block);
+  // blocks grown breadth-first from the lowest unassigned vertex
+  std::vector<int> blk(n, -1), order;  // order: vertices block by block, discovery order
+  std::vector<int> bstart(1, 0);
+  order.reserve(n);
+  for (int s = 0; s < n; ++s) {
+    if (blk[s] >= 0) continue;
+    const int b = (int)bstart.size() - 1;
+    size_t head = order.size();
+    const size_t first = head;
+    blk[s] = b;
+    order.push_back(s);
+    while (head < order.size() && (int)(order.size() - first) < block) {
+      const int v = order[head++];
+      for (int e = deg[v]; e < deg[v + 1] && (int)(order.size() - first) < block; ++e) {
+        const int w = adj[e];
+        if (blk[w] < 0) {
+          blk[w] = b;
+          order.push_back(w);
+        }
+      }
+    }
+    bstart.push_back((int)order.size());
+  }
+  const int nb = (int)bstart.size() - 1;
+  // greedy colouring of the block graph, blocks in growth order
+  std::vector<int> bcol(nb, -1), seen;
+  int ncol = 0;
+  for (int B = 0; B < nb; ++B) {
+    for (int k = bstart[B]; k < bstart[B + 1]; ++k) {
+      const int v = order[k];
+      for (int e = deg[v]; e < deg[v + 1]; ++e) {
+        const int Bj = blk[adj[e]];
+        if (Bj != B && bcol[Bj] >= 0) seen[bcol[Bj]] = B;
+      }
+    }
+    int c = 0;
+    while (c < ncol && seen[c] == B) ++c;
+    if (c == ncol) {
+      ++ncol;
+      seen.push_back(-1);
+    }
+    bcol[B] = c;
+  }
+  // colour, then block, then discovery order
+  std::vector<int> cstart(ncol + 1, 0);
+  for (int B = 0; B < nb; ++B) cstart[bcol[B] + 1] += bstart[B + 1] - bstart[B];
+  for (int c = 0; c < ncol; ++c) cstart[c + 1] += cstart[c];
+  perm.assign(n, 0);
+  for (int B = 0; B < nb; ++B)
+    for (int k = bstart[B]; k < bstart[B + 1]; ++k) perm[cstart[bcol[B]]++] = order[k];
+  if (colour_out) {
+    colour_out->assign(n, 0);
+    for (int i = 0; i < n; ++i) (*colour_out)[i] = bcol[blk[i]];
+  }
+  if (block_out) *block_out = std::move(blk);
+  return ncol;
+}
+
 void build_layout(int n, const int *row, const int *col, const int *dia, int spmv_rows,
-                  int spmv_cap, int small_level, Layout &out) {
+                  int spmv_cap, int small_level, Layout &out, bool identity_s) {
   out.n = n;
   out.nnz = n > 0 ? row[n] : 0;
-  out.s_orig = rcm(n, row, col);
+  if (identity_s) {
+    out.s_orig.resize(n);
+    for (int i = 0; i < n; ++i) out.s_orig[i] = i;
+  } else {
+    out.s_orig = rcm(n, row, col);
+  }
   out.s_of_orig.assign(n, 0);
   for (int k = 0; k < n; ++k) out.s_of_orig[out.s_orig[k]] = k;
   out.s_row.assign(n + 1, 0);

@@ -64,8 +64,25 @@ struct Layout {
 int colour_order(int n, const int *row, const int *col, std::vector<int> &perm,
                  std::vector<int> *colour = nullptr, int block = 1);
 
+// Algebraic block multi-colour order (Iwashita, Nakashima & Takahashi,
+// "Algebraic Block Multi-Color Ordering Method for Parallel Multi-Threaded
+// Sparse Triangular Solver in ICCG Method", IPDPS 2012): blocks of at most
+// `block` rows grown breadth-first over the symmetrised pattern (seeds in
+// index order, a block's rows in the order the search reached them, neighbours
+// in ascending index), the block graph coloured greedily in block order, and
+// the rows listed by colour, then block, then discovery order.  An incomplete
+// factor in this order couples a row only to earlier rows of its own block and
+// to rows of blocks of earlier colours: a sweep is one pass per colour whose
+// blocks are independent, each block walked in its own (short) level order.
+// `perm` maps new position -> vertex; `colour` and `block_of` are per vertex
+// (block numbers in growth order).  Returns the number of colours.
+int block_colour_order(int n, const int *row, const int *col, int block, std::vector<int> &perm,
+                       std::vector<int> *colour = nullptr, std::vector<int> *block_of = nullptr);
+
 // row/col/dia are 0-based, validated (diagonal present, columns ascending).
+// identity_s keeps the caller's order as the S order (for a matrix whose rows
+// are already in the order its kernels want, e.g. a block multi-colour copy).
 void build_layout(int n, const int *row, const int *col, const int *dia, int spmv_rows,
-                  int spmv_cap, int small_level, Layout &out);
+                  int spmv_cap, int small_level, Layout &out, bool identity_s = false);
 
 }  // namespace tfs

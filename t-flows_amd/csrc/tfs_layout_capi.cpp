@@ -45,3 +45,17 @@ extern "C" int tfs_layout_summary(int n, const int *row, const int *col, const i
   *bandwidth = L.bandwidth;
   return 0;
 }
+
+extern "C" int tfs_layout_block_colour_bfs(int n, const int *row, const int *col, int block,
+                                           int *perm, int *colour, int *block_of) {
+  if (n < 0 || block < 1 || (n > 0 && (!row || !col || !perm || !colour || !block_of)))
+    return -1;
+  std::vector<int> p, c, b;
+  const int ncol = tfs::block_colour_order(n, row, col, block, p, &c, &b);
+  for (int i = 0; i < n; ++i) {
+    perm[i] = p[i];
+    colour[i] = c[i];
+    block_of[i] = b[i];
+  }
+  return ncol;
+}

@@ -172,18 +172,26 @@ struct tfs_matrix {
   // P a greedy colouring of the S-order graph (built on first use)
   int inner_order = TFS_INNER_COLOURED;
   int inner_block = kInnerBlock;  // rows coloured as one unit in the inner order
-  int refine = TFS_REFINE_FCG;   // outer loop of the refinement
+  int refine = TFS_REFINE_RELIABLE;  // outer loop of the refinement
   tfs_matrix *inner = nullptr;
   int *inner_emap = nullptr;     // inner's caller-order entry k <- entry of val
   int *inner_from_s = nullptr;   // S position of this matrix for inner's S position i
   bool inner_ready = false;      // inner's values match val
   int inner_colours = 0;
+  // The FP32 operator of the reliable-update refinement (on the inner copy):
+  // sliced ELL of the off-diagonals in S order plus FP32 row sums.
+  int ell_nslice = 0;
+  long long ell_len = 0;
+  long long *ell_off = nullptr;
+  int *ell_col = nullptr, *ell_src = nullptr;
+  float *ell_val = nullptr, *ell_rsum = nullptr;
+  bool ell_ready = false;
 };
 
 namespace {
 
 int matrix_create(int n, long long nnz, const int *row, const int *col, const int *dia, int base,
-                  tfs_matrix **out, Layout *keep);
+                  tfs_matrix **out, Layout *keep, bool identity_s = false);
 
 int dev_alloc(tfs_matrix *A, void **p, size_t bytes) {
   if (bytes == 0) bytes = 8;
@@ -353,6 +361,7 @@ void values_changed(tfs_matrix *A) {
   A->v64.ready = A->v32.ready = false;
   A->v64.factor_prec = A->v32.factor_prec = -1;
   A->inner_ready = false;
+  A->ell_ready = false;
 }
 
 template <typename T>
@@ -659,10 +668,83 @@ int solve_device(tfs_matrix *A, int method, int prec, double *x, double *r1, int
   return status_of(h);
 }
 
-// The colour-ordered copy used by the FP32 inner solves: P A P^T with P a
-// greedy colouring of this matrix's S-order graph, built from the device
-// copy of the S layout (the caller's arrays are not kept).  Its own layouts
-// follow (RCM for its SpMV; L/U levels of the colour order: one per colour).
+// The sliced ELL of the reliable-update refinement's FP32 operator (see
+// k_ell_diff), from the host layout of a matrix: the off-diagonal entries of
+// each S row, 64 rows per slice.
+int build_ell(tfs_matrix *A, const Layout &Ly) {
+  const int n = A->n, ns = (n + kWave - 1) / kWave;
+  std::vector<long long> off((size_t)ns + 1, 0);
+  for (int sl = 0; sl < ns; ++sl) {
+    int w = 0;
+    for (int i = sl * kWave; i < std::min(n, (sl + 1) * kWave); ++i)
+      w = std::max(w, Ly.s_row[i + 1] - Ly.s_row[i] - 1);
+    off[(size_t)sl + 1] = off[(size_t)sl] + (long long)w * kWave;
+  }
+  const long long len = off[(size_t)ns];
+  std::vector<int> ecol((size_t)len), esrc((size_t)len, -1);
+  for (int sl = 0; sl < ns; ++sl) {
+    const int w = (int)((off[(size_t)sl + 1] - off[(size_t)sl]) / kWave);
+    for (int lane = 0; lane < kWave; ++lane) {
+      const int i = sl * kWave + lane;
+      int j = 0;
+      if (i < n)
+        for (int e = Ly.s_row[i]; e < Ly.s_row[i + 1]; ++e)
+          if (Ly.s_col[e] != i) {
+            const size_t q = (size_t)(off[(size_t)sl] + (long long)j * kWave + lane);
+            ecol[q] = Ly.s_col[e];
+            esrc[q] = Ly.s_src[e];
+            ++j;
+          }
+      for (; j < w; ++j) ecol[(size_t)(off[(size_t)sl] + (long long)j * kWave + lane)] = i < n ? i : 0;
+    }
+  }
+  A->ell_nslice = ns;
+  A->ell_len = len;
+  TFS_TRY(upload(A, &A->ell_off, off));
+  TFS_TRY(upload(A, &A->ell_col, ecol));
+  TFS_TRY(upload(A, &A->ell_src, esrc));
+  TFS_TRY(dev_alloc(A, (void **)&A->ell_val, sizeof(float) * (size_t)std::max(len, 1LL)));
+  TFS_TRY(dev_alloc(A, (void **)&A->ell_rsum, sizeof(float) * (size_t)std::max(n, 1)));
+  A->ell_ready = false;
+  return 0;
+}
+
+// The ELL values and row sums from the caller-order FP64 values.
+int prepare_ell(tfs_matrix *A) {
+  if (A->ell_ready) return 0;
+  if (A->ell_len > 0)
+    hipLaunchKernelGGL(k_ell_values, dim3(grid_for(A->ell_len, kRedThreads, 8192)),
+                       dim3(kRedThreads), 0, cur_stream(), A->ell_len, A->ell_src, A->val,
+                       A->ell_val);
+  if (A->n > 0)
+    hipLaunchKernelGGL(k_row_sums, dim3(vgrid(A->n)), dim3(kRedThreads), 0, cur_stream(), A->n,
+                       A->s_row, A->s_src, A->val, A->ell_rsum);
+  TFS_HIP(hipGetLastError());
+  A->ell_ready = true;
+  return 0;
+}
+
+inline int ell_grid(const tfs_matrix *A) { return grid_for(A->ell_nslice, kEllWaves); }
+
+// q = A32 p in difference form; DOT: partial sums of (p, q) per workgroup.
+template <bool DOT>
+void launch_ell(tfs_matrix *A, const float *x, float *y, const KState *st) {
+  tm_begin(TFS_KC_SPMV);
+  hipLaunchKernelGGL(k_ell_diff<DOT>, dim3(ell_grid(A)), dim3(kEllWaves * kWave), 0, cur_stream(),
+                     A->n, A->ell_nslice, A->ell_off, A->ell_col, A->ell_val, A->ell_rsum, x, y,
+                     st, A->partials);
+  tm_end(TFS_KC_SPMV);
+}
+
+// The reordered copy used by the FP32 inner solves: P A P^T, P the block
+// multi-colour order of this matrix's S-order graph (blocks of inner_block
+// rows grown breadth-first, tfs_layout.hpp), or with inner_block 1 the greedy
+// point colouring (red-black on a hex mesh), or the caller's own order
+// (TFS_INNER_REFERENCE, used by the reliable-update loop).  Built from the device copy of
+// the S layout (the caller's arrays are not kept).  The copy keeps its rows in
+// that order for the Krylov vectors and the SpMV (identity S order); its L/U
+// levels follow the order (one per colour for the point colouring, each
+// colour's block-internal depth for blocks).
 int build_inner(tfs_matrix *A) {
   if (A->inner) return 0;
   const int n = A->n;
@@ -676,7 +758,16 @@ int build_inner(tfs_matrix *A) {
                          cur_stream()));
   TFS_HIP(hipStreamSynchronize(cur_stream()));
   std::vector<int> perm;
-  const int ncol = colour_order(n, row.data(), col.data(), perm, nullptr, A->inner_block);
+  int ncol = 0;
+  if (A->inner_order == TFS_INNER_REFERENCE) {
+    // the caller's order (the reference factor, in FP32)
+    perm.resize(n);
+    TFS_HIP(hipMemcpy(perm.data(), A->s_of_orig, sizeof(int) * n, hipMemcpyDeviceToHost));
+  } else {
+    ncol = A->inner_block > 1
+               ? block_colour_order(n, row.data(), col.data(), A->inner_block, perm)
+               : colour_order(n, row.data(), col.data(), perm, nullptr, 1);
+  }
   std::vector<int> pos(n);
   for (int i = 0; i < n; ++i) pos[perm[i]] = i;
   std::vector<int> prow(n + 1, 0), pcol((size_t)nnz), pdia(n, -1), emap((size_t)nnz);
@@ -697,11 +788,15 @@ int build_inner(tfs_matrix *A) {
   }
   Layout Ly;
   tfs_matrix *C = nullptr;
-  TFS_TRY(matrix_create(n, nnz, prow.data(), pcol.data(), pdia.data(), 0, &C, &Ly));
+  TFS_TRY(matrix_create(n, nnz, prow.data(), pcol.data(), pdia.data(), 0, &C, &Ly, true));
   std::vector<int> from_s(n);
   for (int i = 0; i < n; ++i) from_s[i] = perm[Ly.s_orig[i]];
   int rc = upload(A, &A->inner_emap, emap);
   if (rc == 0) rc = upload(A, &A->inner_from_s, from_s);
+  if (rc == 0) {
+    StreamScope scope(C->stream);
+    rc = build_ell(C, Ly);
+  }
   if (rc < 0) {
     tfs_matrix_destroy(C);
     return rc;
@@ -714,6 +809,188 @@ int build_inner(tfs_matrix *A) {
   A->inner_ready = false;
   A->bytes += C->bytes;
   return 0;
+}
+
+// The inner copy's values from this matrix's (caller-order gather through
+// the entry map), marking the copy's layouts stale.
+int refresh_inner(tfs_matrix *A) {
+  tfs_matrix *I = A->inner;
+  if (A->inner_ready) return 0;
+  gather((int)A->nnz, A->val, A->inner_emap, I->val);
+  values_changed(I);
+  A->inner_ready = true;
+  TFS_HIP(hipGetLastError());
+  return 0;
+}
+
+// One iteration of the reliable-update refinement's FP32 CG on the inner copy
+// I (Cg.f90:97-178 with FP32 vectors): z = M^-1 r (FP32 factor and sweeps),
+// rho = (r, z), p = z + beta p, q = A32 p in difference form with (p, q),
+// c += alpha p into the FP64 correction, r -= alpha q with ||r||.
+void reliable_iteration(tfs_matrix *I, int prec, const Vecs<float> &F, double *c, KState *st) {
+  const int n = I->n, g = vgrid(n);
+  if (prec == 2) dic_apply<float>(I, F.r, F.yL, F.zU, st);
+  const int mode = prec == 2 ? 2 : prec == 1 ? 1 : 0;
+  float *z = prec == 0 ? F.r : F.z;
+  hipLaunchKernelGGL(k_rho<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.r, F.z,
+                     vals<float>(I).s_diag, F.zU, I->s_to_u, mode, st, I->partials);
+  finish(g, I->partials, EPI_RHO, st);
+  hipLaunchKernelGGL(k_pupdate<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, z, F.p, st);
+  launch_ell<true>(I, F.p, F.q, st);
+  finish(ell_grid(I), I->partials, EPI_ALPHA, st);
+  hipLaunchKernelGGL(k_axpy_mixed, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, c, F.r, F.p,
+                     F.q, st, I->partials);
+  finish(g, I->partials, EPI_NORM, st);
+}
+
+int push_state(tfs_matrix *A, int k) {
+  TFS_HIP(hipMemcpyAsync(A->d_st + k, A->h_st + k, sizeof(KState), hipMemcpyHostToDevice,
+                         cur_stream()));
+  return 0;
+}
+
+// FP32 CG with reliable FP64 updates (TFS_REFINE_RELIABLE; after the reliable
+// updates of van der Vorst & Ye, "Residual replacement strategies for Krylov
+// subspace iterative methods for the convergence of true residuals", SIAM J.
+// Sci. Comput. 22 (2000), as used for mixed-precision CG by Clark et al.,
+// Comput. Phys. Commun. 181 (2010)).  One CG recurrence runs on the inner copy
+// in FP32 -- factor, sweeps, SpMV (difference form) and the vectors r, z, p,
+// q -- while its directions are folded every iteration into an FP64
+// correction c.  Whenever the FP32 recurrence residual has fallen by `delta`
+// against the last true residual, the segment ends: x += c (FP64), r = b - A x
+// with the FP64 matrix, and the recurrence continues from that true residual
+// with its direction p and rho kept (no restart, so no Krylov information is
+// lost).  It stops when the true residual is below tol.  niter counts the
+// residual replacements, inner_iters the CG iterations.
+int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *niter,
+                       int *inner_iters, double tol, double inner_rtol, double x_rtol,
+                       double *ini_res, double *fin_res);
+
+// FP32 iterations one segment of the reliable-update loop may take to reduce
+// its residual by delta; a segment that needs more ends the solve as not
+// converged (as FP64 CG would fail on such a system, e.g. an unpreconditioned
+// nonsymmetric one).
+constexpr int kSegmentCap = 20000;
+
+int solve_reliable_device(tfs_matrix *A, int prec, double *x, double *r1, int *niter,
+                          int *inner_iters, double tol, double delta, double x_rtol,
+                          double *ini_res, double *fin_res) {
+  const int n = A->n, cap = *niter;
+  if (!(delta > 0.0)) delta = 0.1;
+  delta = std::min(0.5, std::max(1e-6, delta));
+  TFS_TRY(build_inner(A));
+  tfs_matrix *I = A->inner;
+  TFS_TRY(refresh_inner(A));
+  const int *to_s = A->inner_from_s;  // I's S position -> A's S position
+  TFS_TRY(ensure_vec64(A, 8));
+  TFS_TRY(ensure_vec32(I, 7));
+  TFS_TRY(prepare_values<double>(A));
+  TFS_TRY(prec_form<float>(I, prec));
+  TFS_TRY(prepare_ell(I));
+  const Vecs<double> D = vecs_of(A->vec64, n);
+  double *bS = D.r2, *c = D.z;  // c: the FP64 correction, in I's S order
+  const Vecs<float> F = vecs_of(I->vec32, n);
+  KState *si = I->d_st;
+  KState *ho = A->h_st + 1, *hi = I->h_st;
+  const int g = vgrid(n);
+  gather(n, x, A->s_orig, D.x);
+  gather(n, r1, A->s_orig, bS);
+  auto true_residual = [&]() -> int {
+    TFS_TRY(init_state(A, 1, tol, cap, 0.0));
+    gather(n, bS, (const int *)nullptr, D.r);
+    launch_spmv<double, SPMV_RESID>(A, D.x, D.r, nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_sumsq<double>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.r,
+                       (int)EPI_BNRM, A->d_st + 1, A->partials);
+    finish(g, A->partials, EPI_BNRM, A->d_st + 1);
+    TFS_HIP(hipGetLastError());
+    return read_state(A, 1);
+  };
+  TFS_TRY(true_residual());
+  const double err0 = ho->bnrm2;
+  double err = err0, best = err0;
+  hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, c, 0.0);
+  int updates = 0, stalls = 0, status = TFS_OK;
+  bool started = false;
+  while (err >= tol) {
+    if (updates >= cap) {
+      status = TFS_NOT_CONVERGED;
+      break;
+    }
+    // the recurrence continues from the true residual, in FP32
+    gather(n, D.r, to_s, F.r);
+    const double seg_tol = std::max(delta * err, 0.5 * tol);
+    if (!started) {
+      TFS_TRY(init_state(I, 0, seg_tol, kSegmentCap, err0));
+      hi->err = err;
+    } else {
+      hi->done = 0;
+      hi->status = 0;
+      hi->tol = seg_tol;
+      hi->err = err;
+      hi->cap = hi->iter + kSegmentCap;
+      hi->rho_old = hi->rho;  // the last iteration's rho (kept on its exit)
+    }
+    TFS_TRY(push_state(I, 0));
+    started = true;
+    TFS_TRY(iterate(I, hi->cap, [&] { reliable_iteration(I, prec, F, c, si); }));
+    if (hi->status == kStBreakdown) {
+      status = TFS_ERR_BREAKDOWN;
+      break;
+    }
+    // residual replacement: x += c; r = b - A x in FP64
+    ++updates;
+    hipLaunchKernelGGL(k_fold_correction, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x, c,
+                       to_s);
+    TFS_TRY(true_residual());
+    const double e_new = ho->bnrm2;
+    if (!std::isfinite(e_new)) {
+      err = e_new;
+      status = TFS_ERR_BREAKDOWN;
+      break;
+    }
+    err = e_new;
+    if (hi->status == kStNotConverged) {
+      if (err >= tol) status = TFS_NOT_CONVERGED;
+      break;
+    }
+    // no progress of the true residual over three segments: FP32 can no
+    // longer reduce it
+    stalls = e_new >= best ? stalls + 1 : 0;
+    best = std::min(best, e_new);
+    if (stalls >= 3) {
+      if (err >= tol) status = TFS_NOT_CONVERGED;
+      break;
+    }
+  }
+  *inner_iters = started ? hi->iter : 0;
+  scatter(n, D.x, A->s_orig, x);
+  TFS_HIP(hipGetLastError());
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
+  TFS_TRY(check_sweep(A));
+  TFS_TRY(check_sweep(I));
+  *ini_res = err0;
+  if (status == TFS_OK && updates > 0 && x_rtol > 0.0 && updates < cap) {
+    // The FP32 recurrence cannot see the part of the residual below FP32
+    // resolution (error along the smallest eigenvectors).  For the solution
+    // criterion the flexible-CG loop, whose directions are FP64, continues
+    // from x (b back in r1) until its last update is below x_rtol ||x||.
+    scatter(n, bS, A->s_orig, r1);
+    TFS_HIP(hipGetLastError());
+    int steps = cap - updates, inner = 0;
+    double ir = 0.0, fr = 0.0;
+    A->refine = TFS_REFINE_FCG;
+    const int rc = solve_mixed_device(A, prec, x, r1, &steps, &inner, tol, 0.0, x_rtol, &ir, &fr);
+    A->refine = TFS_REFINE_RELIABLE;
+    *inner_iters += inner;
+    *fin_res = fr;
+    *niter = rc == TFS_NOT_CONVERGED ? cap + 1 : updates + steps;
+    return rc;
+  }
+  scatter(n, D.r, A->s_orig, r1);
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
+  *fin_res = err;
+  *niter = status == TFS_NOT_CONVERGED ? cap + 1 : updates;
+  return status;
 }
 
 // FP32-inner PCG inside an FP64 outer loop (iterative refinement).
@@ -742,6 +1019,10 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
     return TFS_OK;
   }
   if (!x || !r1) return TFS_ERR_INVALID;
+  if (A->refine == TFS_REFINE_RELIABLE)
+    return solve_reliable_device(A, prec, x, r1, niter, inner_iters, tol, inner_rtol, x_rtol,
+                                 ini_res, fin_res);
+  if (!(inner_rtol > 0.0)) inner_rtol = 1e-2;
   inner_rtol = std::min(0.5, std::max(1e-6, inner_rtol));
   // The inner solves run on I: this matrix, or its colour-ordered copy when
   // the factor's order matters (prec 2).  The diagonal preconditioners do not
@@ -752,11 +1033,7 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
   if (coloured) {
     TFS_TRY(build_inner(A));
     I = A->inner;
-    if (!A->inner_ready) {
-      gather((int)A->nnz, A->val, A->inner_emap, I->val);
-      values_changed(I);
-      A->inner_ready = true;
-    }
+    TFS_TRY(refresh_inner(A));
   }
   const int *to_s = coloured ? A->inner_from_s : nullptr;  // I's S position -> A's
   TFS_TRY(ensure_vec64(A, 8));
@@ -1123,7 +1400,7 @@ namespace {
 // touching the device, so a bad structure is rejected without a GPU; then
 // builds the layouts and uploads them.  `keep` receives the host layout.
 int matrix_create(int n, long long nnz, const int *row, const int *col, const int *dia, int base,
-                  tfs_matrix **out, Layout *keep) {
+                  tfs_matrix **out, Layout *keep, bool identity_s) {
   if (!out) return TFS_ERR_INVALID;
   *out = nullptr;
   if (n < 0 || nnz < n || nnz > 0x7ffffffeLL || (base != 0 && base != 1)) return TFS_ERR_INVALID;
@@ -1147,7 +1424,8 @@ int matrix_create(int n, long long nnz, const int *row, const int *col, const in
 
   Layout local;
   Layout &Ly = keep ? *keep : local;
-  build_layout(n, hr.data(), hc.data(), hd.data(), kSpmvRows, kSpmvCap, kSmallLevel, Ly);
+  build_layout(n, hr.data(), hc.data(), hd.data(), kSpmvRows, kSpmvCap, kSmallLevel, Ly,
+               identity_s);
   tfs_matrix *A = new tfs_matrix;
   A->n = n;
   A->nnz = nnz;
@@ -1239,6 +1517,8 @@ extern "C" int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info) {
   info->inner_colours = A->inner_colours;
   info->spmv_rows = A->spmv_rows;
   info->spmv_waves = A->spmv_waves;
+  info->inner_block = A->inner_block;
+  info->inner_levels = A->inner ? A->inner->L.nlev : 0;
   return TFS_OK;
 }
 
@@ -1262,33 +1542,42 @@ extern "C" int tfs_matrix_set_spmv_shape(tfs_matrix *A, int rows_per_block, int 
   return set_spmv_shape(A, s_row, rows_per_block, waves_per_group);
 }
 
+namespace {
+// Drops the inner copy (rebuilt on the next refinement solve).
+int drop_inner(tfs_matrix *A) {
+  if (!A->inner) return 0;
+  StreamScope scope(A->stream);
+  TFS_HIP(hipStreamSynchronize(A->stream));
+  A->bytes -= A->inner->bytes;
+  tfs_matrix_destroy(A->inner);
+  A->inner = nullptr;
+  TFS_TRY(dev_free(A, A->inner_emap, sizeof(int) * (size_t)A->nnz));
+  TFS_TRY(dev_free(A, A->inner_from_s, sizeof(int) * (size_t)A->n));
+  A->inner_emap = A->inner_from_s = nullptr;
+  A->inner_colours = 0;
+  return 0;
+}
+}  // namespace
+
 extern "C" int tfs_matrix_set_inner_block(tfs_matrix *A, int rows) {
   if (!A || rows < 1 || rows > 4096) return TFS_ERR_INVALID;
   if (rows == A->inner_block) return TFS_OK;
-  StreamScope scope(A->stream);
   A->inner_block = rows;
-  if (A->inner) {  // rebuilt on the next refinement solve
-    A->bytes -= A->inner->bytes;
-    tfs_matrix_destroy(A->inner);
-    A->inner = nullptr;
-    TFS_TRY(dev_free(A, A->inner_emap, sizeof(int) * (size_t)A->nnz));
-    TFS_TRY(dev_free(A, A->inner_from_s, sizeof(int) * (size_t)A->n));
-    A->inner_emap = A->inner_from_s = nullptr;
-    A->inner_colours = 0;
-  }
-  return TFS_OK;
+  return drop_inner(A);
 }
 
 extern "C" int tfs_matrix_set_refinement(tfs_matrix *A, int outer) {
-  if (!A || (outer != TFS_REFINE_FCG && outer != TFS_REFINE_DEFECT)) return TFS_ERR_INVALID;
+  if (!A || (outer != TFS_REFINE_FCG && outer != TFS_REFINE_DEFECT && outer != TFS_REFINE_RELIABLE))
+    return TFS_ERR_INVALID;
   A->refine = outer;
   return TFS_OK;
 }
 
 extern "C" int tfs_matrix_set_inner_order(tfs_matrix *A, int order) {
   if (!A || (order != TFS_INNER_REFERENCE && order != TFS_INNER_COLOURED)) return TFS_ERR_INVALID;
+  if (order == A->inner_order) return TFS_OK;
   A->inner_order = order;
-  return TFS_OK;
+  return drop_inner(A);
 }
 
 extern "C" int tfs_solve_device(tfs_matrix *A, int method, int prec, double *d_x, double *d_r1,

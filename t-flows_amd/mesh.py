@@ -80,6 +80,9 @@ def _lib():
         lib.tfs_layout_block_colour.restype = ctypes.c_int
         lib.tfs_layout_block_colour.argtypes = [ctypes.c_int, _I32P, _I32P, ctypes.c_int, _I32P,
                                                 _I32P]
+        lib.tfs_layout_block_colour_bfs.restype = ctypes.c_int
+        lib.tfs_layout_block_colour_bfs.argtypes = [ctypes.c_int, _I32P, _I32P, ctypes.c_int,
+                                                    _I32P, _I32P, _I32P]
         lib.tfs_mesh_read_cns.restype = ctypes.c_int
         lib.tfs_mesh_read_cns.argtypes = [ctypes.c_char_p] + [ctypes.POINTER(ctypes.c_int)] * 3 + \
             [ctypes.c_void_p, ctypes.c_void_p]
@@ -208,6 +211,19 @@ def layout_colour(row, col, block=1):
     if k < 0:
         raise ValueError("tfs_layout_colour failed")
     return k, perm, colour
+
+
+def layout_block_colour_bfs(row, col, block):
+    """Block multi-colour order of the FP32 inner solves (blocks of at most
+    `block` rows grown breadth-first): (ncolours, perm, colour, block_of)."""
+    row = np.ascontiguousarray(row, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    n = len(row) - 1
+    perm, colour, block_of = (np.empty(n, np.int32) for _ in range(3))
+    k = _lib().tfs_layout_block_colour_bfs(n, row, col, int(block), perm, colour, block_of)
+    if k < 0:
+        raise ValueError("tfs_layout_block_colour_bfs failed")
+    return k, perm, colour, block_of
 
 
 def layout_summary(row, col, dia):

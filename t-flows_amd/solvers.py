@@ -37,7 +37,7 @@ TFS_ERR_NO_DEVICE = -5
 
 CG, BICG, CGS = 0, 1, 2
 TFS_INNER_REFERENCE, TFS_INNER_COLOURED = 0, 1
-TFS_REFINE_FCG, TFS_REFINE_DEFECT = 0, 1
+TFS_REFINE_FCG, TFS_REFINE_DEFECT, TFS_REFINE_RELIABLE = 0, 1, 2
 TFS_SWEEP_LEVELS, TFS_SWEEP_STATIC, TFS_SWEEP_DYNAMIC, TFS_SWEEP_COOPERATIVE = 0, 1, 2, 3
 _METHODS = {"cg": CG, "bicg": BICG, "cgs": CGS}
 
@@ -59,7 +59,8 @@ class _Info(ctypes.Structure):
                 ("levels_lower", ctypes.c_int), ("levels_upper", ctypes.c_int),
                 ("row_blocks", ctypes.c_int), ("device_bytes", ctypes.c_longlong),
                 ("inner_colours", ctypes.c_int), ("spmv_rows", ctypes.c_int),
-                ("spmv_waves", ctypes.c_int)]
+                ("spmv_waves", ctypes.c_int), ("inner_block", ctypes.c_int),
+                ("inner_levels", ctypes.c_int)]
 
 
 def _lib():
@@ -183,7 +184,8 @@ class Matrix:
         return dict(n=i.n, nnz=i.nnz, levels_lower=i.levels_lower,
                     levels_upper=i.levels_upper, row_blocks=i.row_blocks,
                     device_bytes=i.device_bytes, inner_colours=i.inner_colours,
-                    spmv_rows=i.spmv_rows, spmv_waves=i.spmv_waves)
+                    spmv_rows=i.spmv_rows, spmv_waves=i.spmv_waves,
+                    inner_block=i.inner_block, inner_levels=i.inner_levels)
 
     def tune_spmv(self):
         """Times the SpMV shapes and keeps the fastest; returns (rows, waves)."""
@@ -203,7 +205,8 @@ class Matrix:
                "tfs_matrix_set_sweep_schedule")
 
     def set_refinement(self, outer):
-        """TFS_REFINE_FCG (default) or TFS_REFINE_DEFECT for solve_mixed."""
+        """TFS_REFINE_RELIABLE (default), TFS_REFINE_FCG or TFS_REFINE_DEFECT for
+        solve_mixed."""
         _check(self._lib.tfs_matrix_set_refinement(self._h, int(outer)),
                "tfs_matrix_set_refinement")
 
@@ -339,12 +342,16 @@ def solve(method, A, x, r1, prec, niter, tol, ini_res=0.0):
     return _solve(_METHODS.get(method, method), A, x, r1, prec, niter, tol, ini_res)
 
 
-def solve_mixed(A, x, r1, prec, niter, tol, inner_rtol=1e-2, x_rtol=0.0):
+def solve_mixed(A, x, r1, prec, niter, tol, inner_rtol=0.0, x_rtol=0.0):
     """FP32-inner PCG inside FP64 iterative refinement (GPU tensors only).
 
     Returns (steps, inner_iterations, ini_res, fin_res); fin_res is the true
-    FP64 residual ||b - A x||, which is also left in r1.  x_rtol > 0 also
-    requires the last correction to satisfy ||e|| <= x_rtol ||x||."""
+    FP64 residual ||b - A x||, which is also left in r1.  inner_rtol: the
+    FP32 recurrence's residual drop per residual replacement (reliable
+    updates, default 0.1) or each inner solve's tolerance (flexible CG and
+    defect correction, default 1e-2); 0 picks the default.  x_rtol > 0 (the
+    flexible-CG and defect-correction loops) also requires the last
+    correction to satisfy ||e|| <= x_rtol ||x||."""
     it = ctypes.c_int(int(niter))
     inner = ctypes.c_int(0)
     ir = ctypes.c_double(0.0)

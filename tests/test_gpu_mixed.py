@@ -13,8 +13,14 @@ the exact solution (sparse LU) is printed for every case and bounded twice:
     tolerance: within 10x of the oracle's own error and <= 1e-8 wherever the
     oracle's is (defect correction's update does not contain the error it
     cannot resolve, so there the bound stays that of the residual).
-The FP32 factor is the reference recurrence in FP32: it agrees with the FP64
-oracle factor to FP32 rounding (rel 1e-5).
+The reliable-update loop (the default) runs one FP32 CG recurrence with FP64
+residual replacement: in the reference order it takes about the oracle's
+iteration count; its error meets the residual bound above, and with the
+correction criterion (the last segment's FP64 correction <= 1e-10 ||x||) the
+oracle's own accuracy.
+The FP32 factor is the reference recurrence on FP32 values (FP64 arithmetic,
+FP32 storage): it agrees with the FP64 oracle factor to FP32 rounding (rel
+1e-5).
 """
 import numpy as np
 import pytest
@@ -47,8 +53,11 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("outer", ["fcg", "defect"])
-@pytest.mark.parametrize("criterion", ["residual", "correction"])
+OUTER_CRITERIA = [("reliable", "residual"), ("reliable", "correction"), ("fcg", "residual"),
+                  ("fcg", "correction"), ("defect", "residual"), ("defect", "correction")]
+
+
+@pytest.mark.parametrize("outer,criterion", OUTER_CRITERIA)
 @pytest.mark.parametrize("order", ["reference", "coloured"])
 @pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("prec", [0, 1, 2])
@@ -60,24 +69,39 @@ def test_refinement_reaches_fp64_solution(torch, S, oracle, name, prec, order, c
     ref = oracle.solve("cg", c.row, c.col, c.dia, c.val, c.b, prec=prec, niter=20000, tol=tol)
     A = S.Matrix(c.row, c.col, c.dia, c.val)
     A.set_inner_order(S.TFS_INNER_COLOURED if order == "coloured" else S.TFS_INNER_REFERENCE)
-    A.set_refinement(S.TFS_REFINE_FCG if outer == "fcg" else S.TFS_REFINE_DEFECT)
+    A.set_refinement({"reliable": S.TFS_REFINE_RELIABLE, "fcg": S.TFS_REFINE_FCG,
+                      "defect": S.TFS_REFINE_DEFECT}[outer])
     x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
     r = torch.from_numpy(c.b.copy()).cuda()
     x_rtol = 1e-10 if criterion == "correction" else 0.0
-    steps, inner, ini, fin = S.solve_mixed(A, x, r, prec, 50, tol, inner_rtol=1e-4, x_rtol=x_rtol)
+    rtol = 0.1 if outer == "reliable" else 1e-4
+    steps, inner, ini, fin = S.solve_mixed(A, x, r, prec, 50, tol, inner_rtol=rtol, x_rtol=x_rtol)
+    if outer == "reliable" and ref["niter"] > 20000:
+        # CG itself does not converge here (unpreconditioned nonsymmetric
+        # momentum matrix): neither does its FP32 recurrence, and the solve
+        # says so within its segment cap
+        print(f"  oracle Cg did not converge; reliable: steps {steps} inner {inner}", flush=True)
+        assert A.status == S.TFS_NOT_CONVERGED and steps == 51 and inner <= 20000 * 50
+        return
     assert A.status == S.TFS_OK and fin < tol, (steps, inner, fin, tol)
     assert 1 <= steps <= 30
     assert abs(ini - np.linalg.norm(c.b)) <= 1e-12 * np.linalg.norm(c.b)
     xs = exact_solution(c.row, c.col, c.val, c.b)
     err, err_ref = rel_l2(x.cpu().numpy(), xs), rel_l2(ref["x"], xs)
-    print(f"  steps {steps} inner {inner}: rel-L2 vs exact {err:.2e} (oracle {err_ref:.2e}), "
-          f"vs oracle {rel_l2(x.cpu().numpy(), ref['x']):.2e}", flush=True)
-    if criterion == "correction" and outer == "fcg":
+    print(f"  steps {steps} inner {inner} (oracle {ref['niter']}): rel-L2 vs exact {err:.2e} "
+          f"(oracle {err_ref:.2e}), vs oracle {rel_l2(x.cpu().numpy(), ref['x']):.2e}", flush=True)
+    if outer == "reliable" and order == "reference" and criterion == "residual":
+        # one CG recurrence whose preconditioner is the oracle's in FP32:
+        # about the oracle's iteration count
+        assert inner <= 1.5 * ref["niter"] + 5, (inner, ref["niter"])
+    if criterion == "correction" and outer in ("fcg", "reliable"):
         assert err <= refinement_bar(ref["x"], xs)
     else:
-        # defect correction: its correction misses what FP32 cannot resolve
-        # (the error along the smallest eigenvectors), so the update criterion
-        # does not bound the error; the stated bound is that of the residual
+        # the residual alone does not fix the error along the smallest
+        # eigenvectors, which FP32 resolves less well than FP64 at the same
+        # residual; and defect correction's update misses what FP32 cannot
+        # resolve, so there the update criterion does not bound the error
+        # either: the stated bound is that of the residual
         assert err <= max(1e-6, refinement_bar(ref["x"], xs))
     # the residual left in r1 is the true FP64 residual
     true_r = torch.from_numpy(c.b.copy()).cuda()
@@ -118,6 +142,7 @@ def test_coloured_inner_order(torch, S):
     c = mesh.make_case(**CASES["poisson_stretched20"])
     tol = 1e-10 * float(np.linalg.norm(c.b))
     A = S.Matrix(c.row, c.col, c.dia, c.val)
+    A.set_inner_block(1)  # point colouring
     out = {}
     for order in (S.TFS_INNER_REFERENCE, S.TFS_INNER_COLOURED):
         A.set_inner_order(order)
