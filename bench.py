@@ -230,9 +230,9 @@ def sweep_roofline(A, c, inloop_us, traffic_profile=None):
     }
 
 
-def breakdown(lib, step):
+def breakdown(lib, step, classes=((0, "spmv"), (1, "sweeps"), (2, "factor"))):
     out = {}
-    for kc, name in ((0, "spmv"), (1, "sweeps"), (2, "factor")):
+    for kc, name in classes:
         lib.tfs_timing_start(kc)
         step()
         ms, nl = ctypes.c_double(), ctypes.c_longlong()
@@ -395,8 +395,8 @@ def mixed_vs_fp64(args):
     torch.cuda.synchronize()
     # per-class kernel times of one solve; the dominant class is then timed
     # with HIP events around each of its launches through the timed region
-    bd = breakdown(lib, step32)
-    dom_kc = 1 if bd["sweeps"]["ms_per_solve"] > bd["spmv"]["ms_per_solve"] else 0
+    bd = breakdown(lib, step32, ((4, "spmv_fp32"), (1, "sweeps"), (2, "factor"), (0, "spmv_fp64")))
+    dom_kc = 1 if bd["sweeps"]["ms_per_solve"] > bd["spmv_fp32"]["ms_per_solve"] else 4
     lib.tfs_timing_start(dom_kc)
     e32, lat32, res32 = timed(step32, args.steps, 0, torch.cuda.synchronize, "mixed")
     dom_ms, dom_n = ctypes.c_double(), ctypes.c_longlong()
@@ -416,16 +416,19 @@ def mixed_vs_fp64(args):
     speedup = ms64 / ms32
     log(f"  fp64: {ms64:.1f} ms/solve, niter={res64[0]}, fin={res64[2]:.3e}; speedup "
         f"{speedup:.2f}x, rel-L2 mixed vs FP64 {err:.2e}")
-    if dom_kc == 0:
-        algo, kname = fp32_spmv_bytes(c.n, c.nnz), \
-            "k_ell_diff (FP32 sliced-ELL SpMV, difference form)"
+    # algorithmic bytes of the class over the timed solves, per launch
+    if dom_kc == 4:
+        units, per_unit = inner, fp32_spmv_bytes(c.n, c.nnz)
+        kname = "k_ell_diff (FP32 sliced-ELL SpMV, difference form)"
         prof = "k_ell_diff_poisson256_counters"
-        per = "8 nnz + 12 N + 4 (SpMV, FP32 values and vectors)"
+        per = "8 nnz + 12 N + 4 per SpMV (FP32 values and vectors), one SpMV per FP32 iteration"
     else:
-        algo, kname = fp32_sweep_bytes(c.n, c.nnz), \
-            "k_sweep_syncfree<float> (FP32 block multi-colour DIC sweeps)"
-        prof = "k_sweep_f32_poisson256_counters"
-        per = "(8 (nnz - N) + 32 N) / 2 per sweep (FP32 DIC forward + backward pair)"
+        units, per_unit = 2 * inner, fp32_sweep_bytes(c.n, c.nnz)
+        kname = "k_colour_sweep (FP32 point-coloured DIC sweeps, one launch per colour)"
+        prof = "k_colour_sweep_poisson256_counters"
+        per = ("(8 (nnz - N) + 32 N) / 2 per sweep (FP32 DIC forward + backward pair), two "
+               "sweeps per FP32 iteration, spread over the launches of the colours")
+    algo = per_unit * units * args.steps / max(dom_n.value, 1)  # bytes per launch
     achieved = algo / (dom_launch_ms * 1e-3) / 1e9
     roof = {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

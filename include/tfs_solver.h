@@ -417,7 +417,8 @@ enum {
   TFS_KC_SPMV = 0,    /* k_spmv in every mode                                  */
   TFS_KC_SWEEP = 1,   /* forward / backward preconditioner sweeps (per level) */
   TFS_KC_FACTOR = 2,  /* Prec_Form level kernels                               */
-  TFS_KC_VECTOR = 3   /* reserved: vector update / reduction kernels           */
+  TFS_KC_VECTOR = 3,  /* reserved: vector update / reduction kernels           */
+  TFS_KC_SPMV32 = 4   /* k_ell_diff, the refinement's FP32 operator            */
 };
 /* From now on every launch of `kclass` on the library stream is bracketed by
  * a HIP event pair (up to 4096 pairs between stream synchronisations). */

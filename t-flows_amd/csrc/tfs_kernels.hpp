@@ -419,6 +419,53 @@ k_ell_diff(int n, int nslice, const long long *__restrict__ off, const int *__re
   if (DOT) store_partial<kEllWaves * kWave>(red, partials);
 }
 
+// Incomplete-Cholesky sweeps of a point-coloured inner order (rows of one
+// colour are independent: every triangle entry of a row lies in another
+// colour), one launch per colour over that colour's ELL slices (64 rows,
+// entries column-major, as k_ell_diff):
+//   forward   x_i = (b_i - sum_{j<i} a_ij x_j) d_i              (Prec_Solve.f90:46-53)
+//   backward  x_i = (b_i / (d_i + TINY) - sum_{j>i} a_ij x_j) d_i  (:55-67, scale fused)
+// in FP64 arithmetic on FP32 storage.  Vectors are in the inner copy's S
+// order (the colour order), so b, d and x stream contiguously.  Padding
+// entries have column -1 (x there may hold anything, even a NaN).
+template <bool BWD>
+__global__ void __launch_bounds__(kEllWaves * kWave)
+k_colour_sweep(int s0, int s1, const int *__restrict__ row0, const long long *__restrict__ off,
+               const int *__restrict__ col, const float *__restrict__ val,
+               const float *__restrict__ d, const float *__restrict__ b, float *__restrict__ x,
+               int row_end, const KState *st) {
+  if (st && solve_done(st)) return;
+  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  for (int s = s0 + blockIdx.x * kEllWaves + wv; s < s1; s += gridDim.x * kEllWaves) {
+    const int su = __builtin_amdgcn_readfirstlane(s);
+    const long long e0 = off[su];
+    const int w = (int)((off[su + 1] - e0) / kWave);
+    const int i = row0[su] + lane;
+    const bool live = i < row_end;
+    const double di = live ? (double)d[i] : 0.0;
+    double acc = live ? (double)b[i] : 0.0;
+    if (BWD) acc = acc / (di + kTiny);
+    const long long base = e0 + lane;
+    for (int j0 = 0; j0 < w; j0 += kEllBatch) {
+      int c[kEllBatch];
+      float a[kEllBatch], xv[kEllBatch];
+#pragma unroll
+      for (int k = 0; k < kEllBatch; ++k)
+        if (j0 + k < w) {
+          c[k] = col[base + (long long)(j0 + k) * kWave];
+          a[k] = val[base + (long long)(j0 + k) * kWave];
+        }
+#pragma unroll
+      for (int k = 0; k < kEllBatch; ++k)
+        if (j0 + k < w) xv[k] = c[k] >= 0 ? x[c[k]] : 0.0f;  // -1: padding
+#pragma unroll
+      for (int k = 0; k < kEllBatch; ++k)
+        if (j0 + k < w) acc = acc - (double)a[k] * (double)xv[k];
+    }
+    if (live) x[i] = (float)(acc * di);
+  }
+}
+
 // The sliced-ELL values (FP32) from the FP64 caller values: src[e] is the
 // caller entry of slot e, or -1 for padding.
 __global__ void __launch_bounds__(kRedThreads)

@@ -186,6 +186,16 @@ struct tfs_matrix {
   int *ell_col = nullptr, *ell_src = nullptr;
   float *ell_val = nullptr, *ell_rsum = nullptr;
   bool ell_ready = false;
+  // Point-coloured incomplete-Cholesky sweeps of the inner copy: per colour,
+  // sliced ELL of the strictly-lower (L) and strictly-upper (U) entries.
+  int csw_colours = 0;
+  std::vector<int> csw_cstart, csw_sbeg;  // host: first row / first slice of each colour
+  int *csw_row0 = nullptr;                // first row of each slice
+  long long csw_len[2] = {0, 0};
+  long long *csw_off[2] = {nullptr, nullptr};
+  int *csw_col[2] = {nullptr, nullptr}, *csw_src[2] = {nullptr, nullptr};
+  float *csw_val[2] = {nullptr, nullptr};
+  bool csw_ready = false;
 };
 
 namespace {
@@ -362,6 +372,7 @@ void values_changed(tfs_matrix *A) {
   A->v64.factor_prec = A->v32.factor_prec = -1;
   A->inner_ready = false;
   A->ell_ready = false;
+  A->csw_ready = false;
 }
 
 template <typename T>
@@ -709,6 +720,96 @@ int build_ell(tfs_matrix *A, const Layout &Ly) {
   return 0;
 }
 
+// The point-coloured sweep layout (k_colour_sweep) of a matrix whose rows
+// are already in colour order: built only if its lower levels are exactly
+// its colours in S order (true of a greedy point colouring, where a row's
+// level is its colour), so that the L-order factor is also in S order.
+int build_colour_sweeps(tfs_matrix *A, const Layout &Ly) {
+  const int n = A->n;
+  for (int p = 0; p < n; ++p)
+    if (Ly.L.orig[p] != p || Ly.s_orig[p] != p) return 0;  // not a point colouring
+  const int nc = Ly.L.nlev;
+  std::vector<int> cstart(Ly.L.lev_ptr.begin(), Ly.L.lev_ptr.end()), sbeg(1, 0), row0;
+  for (int c = 0; c < nc; ++c) {
+    for (int r = cstart[c]; r < cstart[c + 1]; r += kWave) row0.push_back(r);
+    sbeg.push_back((int)row0.size());
+  }
+  const int ns = (int)row0.size();
+  for (int t = 0; t < 2; ++t) {  // 0: lower entries, 1: upper entries
+    std::vector<long long> off((size_t)ns + 1, 0);
+    auto take = [&](int i, int e) { return t == 0 ? Ly.s_col[e] < i : Ly.s_col[e] > i; };
+    for (int c = 0; c < nc; ++c)
+      for (int sl = sbeg[c]; sl < sbeg[c + 1]; ++sl) {
+        int w = 0;
+        for (int i = row0[sl]; i < std::min(row0[sl] + kWave, cstart[c + 1]); ++i) {
+          int k = 0;
+          for (int e = Ly.s_row[i]; e < Ly.s_row[i + 1]; ++e) k += take(i, e);
+          w = std::max(w, k);
+        }
+        off[(size_t)sl + 1] = off[(size_t)sl] + (long long)w * kWave;
+      }
+    const long long len = off[(size_t)ns];
+    std::vector<int> ecol((size_t)len, -1), esrc((size_t)len, -1);
+    for (int c = 0; c < nc; ++c)
+      for (int sl = sbeg[c]; sl < sbeg[c + 1]; ++sl)
+        for (int lane = 0; lane < kWave; ++lane) {
+          const int i = row0[sl] + lane;
+          if (i >= cstart[c + 1]) break;
+          int j = 0;
+          for (int e = Ly.s_row[i]; e < Ly.s_row[i + 1]; ++e)
+            if (take(i, e)) {
+              const size_t q = (size_t)(off[(size_t)sl] + (long long)j * kWave + lane);
+              ecol[q] = Ly.s_col[e];
+              esrc[q] = Ly.s_src[e];
+              ++j;
+            }
+        }
+    A->csw_len[t] = len;
+    TFS_TRY(upload(A, &A->csw_off[t], off));
+    TFS_TRY(upload(A, &A->csw_col[t], ecol));
+    TFS_TRY(upload(A, &A->csw_src[t], esrc));
+    TFS_TRY(dev_alloc(A, (void **)&A->csw_val[t], sizeof(float) * (size_t)std::max(len, 1LL)));
+  }
+  TFS_TRY(upload(A, &A->csw_row0, row0));
+  A->csw_cstart = cstart;
+  A->csw_sbeg = sbeg;
+  A->csw_colours = nc;
+  A->csw_ready = false;
+  return 0;
+}
+
+int prepare_colour_sweeps(tfs_matrix *A) {
+  if (A->csw_ready || A->csw_colours == 0) return 0;
+  for (int t = 0; t < 2; ++t)
+    if (A->csw_len[t] > 0)
+      hipLaunchKernelGGL(k_ell_values, dim3(grid_for(A->csw_len[t], kRedThreads, 8192)),
+                         dim3(kRedThreads), 0, cur_stream(), A->csw_len[t], A->csw_src[t], A->val,
+                         A->csw_val[t]);
+  TFS_HIP(hipGetLastError());
+  A->csw_ready = true;
+  return 0;
+}
+
+// z := M^-1 r for prec 2 on a point-coloured matrix: the forward sweep into y,
+// colour by colour, then the backward sweep into z, colours in reverse; all
+// vectors in S order (FP32), factor d (its L order is its S order).
+void colour_sweeps(tfs_matrix *A, const float *r, float *y, float *z, const KState *st) {
+  const float *d = A->v32.dL;
+  for (int k = 0; k < 2 * A->csw_colours; ++k) {
+    const bool bwd = k >= A->csw_colours;
+    const int c = bwd ? 2 * A->csw_colours - 1 - k : k;
+    const int s0 = A->csw_sbeg[c], s1 = A->csw_sbeg[c + 1];
+    if (s1 == s0) continue;
+    const int t = bwd ? 1 : 0;
+    tm_begin(TFS_KC_SWEEP);
+    hipLaunchKernelGGL(bwd ? k_colour_sweep<true> : k_colour_sweep<false>,
+                       dim3(grid_for(s1 - s0, kEllWaves, 8192)), dim3(kEllWaves * kWave), 0,
+                       cur_stream(), s0, s1, A->csw_row0, A->csw_off[t], A->csw_col[t],
+                       A->csw_val[t], d, bwd ? y : r, bwd ? z : y, A->csw_cstart[c + 1], st);
+    tm_end(TFS_KC_SWEEP);
+  }
+}
+
 // The ELL values and row sums from the caller-order FP64 values.
 int prepare_ell(tfs_matrix *A) {
   if (A->ell_ready) return 0;
@@ -729,11 +830,11 @@ inline int ell_grid(const tfs_matrix *A) { return grid_for(A->ell_nslice, kEllWa
 // q = A32 p in difference form; DOT: partial sums of (p, q) per workgroup.
 template <bool DOT>
 void launch_ell(tfs_matrix *A, const float *x, float *y, const KState *st) {
-  tm_begin(TFS_KC_SPMV);
+  tm_begin(TFS_KC_SPMV32);
   hipLaunchKernelGGL(k_ell_diff<DOT>, dim3(ell_grid(A)), dim3(kEllWaves * kWave), 0, cur_stream(),
                      A->n, A->ell_nslice, A->ell_off, A->ell_col, A->ell_val, A->ell_rsum, x, y,
                      st, A->partials);
-  tm_end(TFS_KC_SPMV);
+  tm_end(TFS_KC_SPMV32);
 }
 
 // The reordered copy used by the FP32 inner solves: P A P^T, P the block
@@ -793,10 +894,9 @@ int build_inner(tfs_matrix *A) {
   for (int i = 0; i < n; ++i) from_s[i] = perm[Ly.s_orig[i]];
   int rc = upload(A, &A->inner_emap, emap);
   if (rc == 0) rc = upload(A, &A->inner_from_s, from_s);
-  if (rc == 0) {
-    StreamScope scope(C->stream);
-    rc = build_ell(C, Ly);
-  }
+  if (rc == 0) rc = build_ell(C, Ly);
+  if (rc == 0 && A->inner_order == TFS_INNER_COLOURED && A->inner_block == 1)
+    rc = build_colour_sweeps(C, Ly);
   if (rc < 0) {
     tfs_matrix_destroy(C);
     return rc;
@@ -829,8 +929,12 @@ int refresh_inner(tfs_matrix *A) {
 // c += alpha p into the FP64 correction, r -= alpha q with ||r||.
 void reliable_iteration(tfs_matrix *I, int prec, const Vecs<float> &F, double *c, KState *st) {
   const int n = I->n, g = vgrid(n);
-  if (prec == 2) dic_apply<float>(I, F.r, F.yL, F.zU, st);
-  const int mode = prec == 2 ? 2 : prec == 1 ? 1 : 0;
+  const bool csw = prec == 2 && I->csw_colours > 0;
+  if (csw)
+    colour_sweeps(I, F.r, F.r2, F.z, st);  // z in S order (forward result in slot r2)
+  else if (prec == 2)
+    dic_apply<float>(I, F.r, F.yL, F.zU, st);
+  const int mode = csw ? 3 : prec == 2 ? 2 : prec == 1 ? 1 : 0;
   float *z = prec == 0 ? F.r : F.z;
   hipLaunchKernelGGL(k_rho<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.r, F.z,
                      vals<float>(I).s_diag, F.zU, I->s_to_u, mode, st, I->partials);
@@ -883,10 +987,11 @@ int solve_reliable_device(tfs_matrix *A, int prec, double *x, double *r1, int *n
   TFS_TRY(refresh_inner(A));
   const int *to_s = A->inner_from_s;  // I's S position -> A's S position
   TFS_TRY(ensure_vec64(A, 8));
-  TFS_TRY(ensure_vec32(I, 7));
+  TFS_TRY(ensure_vec32(I, 8));
   TFS_TRY(prepare_values<double>(A));
   TFS_TRY(prec_form<float>(I, prec));
   TFS_TRY(prepare_ell(I));
+  if (prec == 2) TFS_TRY(prepare_colour_sweeps(I));
   const Vecs<double> D = vecs_of(A->vec64, n);
   double *bS = D.r2, *c = D.z;  // c: the FP64 correction, in I's S order
   const Vecs<float> F = vecs_of(I->vec32, n);
@@ -1827,7 +1932,7 @@ extern "C" int tfs_sweep_time(tfs_matrix *A, int reps, double *ms) {
 }
 
 extern "C" int tfs_timing_start(int kclass) {
-  if (kclass < TFS_KC_SPMV || kclass > TFS_KC_VECTOR) return TFS_ERR_INVALID;
+  if (kclass < TFS_KC_SPMV || kclass > TFS_KC_SPMV32) return TFS_ERR_INVALID;
   TFS_TRY(ensure_init());
   TFS_HIP(hipDeviceSynchronize());
   if (g_tm.ev.empty()) {
