@@ -391,6 +391,14 @@ int tfs_server_stats(tfs_server *S, long long *hbm_in_use, long long *hbm_peak,
 int tfs_spmv_device(tfs_matrix *A, const double *d_x, double *d_y);
 /* r := r - A x  (Residual.f90:22-27, term by term in row order). */
 int tfs_residual_device(tfs_matrix *A, const double *d_x, double *d_r);
+/* The same without allocating device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its two
+ * work vectors live in a library scratch array of 64 MiB held on each GPU the
+ * library runs on: matrices of up to 4,194,304 rows (more: TFS_ERR_INVALID).
+ * The matrix must exist (its device layouts are allocated at creation); calls
+ * that overlap on one GPU share the scratch array and race. */
+int tfs_residual_device_noalloc(tfs_matrix *A, const double *d_x, double *d_r);
 /* Prec_Form: builds (or reuses) the factor for prec. */
 int tfs_prec_form(tfs_matrix *A, int prec);
 /* Prec_Solve: x := M^-1 b with the current factor (tfs_prec_form first). */

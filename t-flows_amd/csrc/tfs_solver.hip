@@ -30,12 +30,24 @@
 
 using namespace tfs;
 
+// Scratch of tfs_residual_device_noalloc: the two S-order vectors of the
+// residual (x, r), up to kScratchRows rows each -- 64 MiB held on every GPU
+// the library runs on until the process ends.  Host code must not pass
+// `scratch` itself to a kernel (on the host it names a stand-in, not device
+// memory): its device address is looked up once per device at set-up
+// (ensure_init) and kept in Lib::scratch.
+constexpr int kScratchRows = 1 << 22;
+static __device__ __align__(256) double scratch[2 * (size_t)kScratchRows];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
 
 struct Lib {
   std::mutex mu;
   bool ready = false;
   int device = 0;
+  double *scratch[kMaxDevices] = {};  // device address of `scratch`, per device
 };
 Lib g_lib;
 
@@ -232,8 +244,12 @@ int ensure_init() {
   if (g_lib.ready) return 0;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return TFS_ERR_NO_DEVICE;
-  if (g_lib.device >= ndev) return TFS_ERR_INVALID;
+  if (g_lib.device >= ndev || g_lib.device >= kMaxDevices) return TFS_ERR_INVALID;
   TFS_HIP(hipSetDevice(g_lib.device));
+  // this device's copy of the allocation-free entry point's scratch array
+  void *sp = nullptr;
+  TFS_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  g_lib.scratch[g_lib.device] = static_cast<double *>(sp);
   g_lib.ready = true;
   return 0;
 }
@@ -1433,6 +1449,9 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
   TFS_HIP(hipMemset(A->d_st, 0, 2 * sizeof(KState)));
   TFS_HIP(hipMemset(A->val, 0, sizeof(double) * (size_t)std::max(A->nnz, 1LL)));
   TFS_HIP(hipHostMalloc((void **)&A->h_st, 2 * sizeof(KState), hipHostMallocDefault));
+  // the FP64 value layouts every FP64 path uses, allocated here so that the
+  // allocation-free entry point finds them
+  TFS_TRY(alloc_values<double>(A));
   // Persistent sweeps: at most as many workgroups as can be co-resident for
   // the heaviest sweep kernel (the static schedule needs them all resident).
   int dev = 0, n_cu = 0, per_cu = 8;
@@ -1751,6 +1770,36 @@ extern "C" int tfs_residual_device(tfs_matrix *A, const double *d_x, double *d_r
   gather(A->n, d_r, A->s_orig, V.r);
   launch_spmv<double, SPMV_RESID>(A, V.x, V.r, nullptr, nullptr, nullptr, nullptr);
   scatter(A->n, V.r, A->s_orig, d_r);
+  TFS_HIP(hipGetLastError());
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
+  return TFS_OK;
+}
+
+// tfs_residual_device without allocating: the two S-order vectors live in
+// this device's `scratch` array (the FP64 value layouts exist from creation;
+// the values are copied into them if they changed, as in the variant that
+// allocates).  Every scratch element the call reads it has written first, so
+// nothing needs clearing; calls that overlap on one GPU share the array and
+// race.
+extern "C" int tfs_residual_device_noalloc(tfs_matrix *A, const double *d_x, double *d_r) {
+  if (!A) return TFS_ERR_INVALID;
+  StreamScope scope(A->stream);
+  if (A->n > 0 && (!d_x || !d_r)) return TFS_ERR_INVALID;
+  if (A->n == 0) return TFS_OK;
+  if (A->n > kScratchRows) {
+    std::fprintf(stderr,
+                 "tfs_residual_device_noalloc: %d rows exceed the scratch capacity of %d rows\n",
+                 A->n, kScratchRows);
+    return TFS_ERR_INVALID;
+  }
+  double *base = g_lib.scratch[g_lib.device];
+  if (!base || !A->v64.allocated) return TFS_ERR_DEVICE;
+  TFS_TRY(prepare_values<double>(A));
+  double *xS = base, *rS = base + kScratchRows;
+  gather(A->n, d_x, A->s_orig, xS);
+  gather(A->n, d_r, A->s_orig, rS);
+  launch_spmv<double, SPMV_RESID>(A, xS, rS, nullptr, nullptr, nullptr, nullptr);
+  scatter(A->n, rS, A->s_orig, d_r);
   TFS_HIP(hipGetLastError());
   TFS_HIP(hipStreamSynchronize(cur_stream()));
   return TFS_OK;
