@@ -350,7 +350,7 @@ struct tfs_gradient {
   int *other = nullptr;
   unsigned char *kind = nullptr, *flag = nullptr;
   double *part = nullptr;     // 3 kParts partial maxima + 3 results
-  double *rec = nullptr;      // 6 values per cell slot for the pressure face loop
+  double *rec = nullptr;      // 6 values per cell slot for the pressure face loop (lazy)
 };
 
 extern "C" int tfs_gradient_destroy(tfs_gradient *S) {
@@ -446,7 +446,6 @@ extern "C" int tfs_gradient_create(int n_cells, int n_bnd_cells, long long n_fac
             up((void **)&S->id, id.data(), sizeof(double) * id.size()) &&
             up((void **)&S->other, other.data(), sizeof(int) * other.size()) &&
             up((void **)&S->flag, flag.data(), flag.size()) &&
-            up((void **)&S->rec, nullptr, 6 * sizeof(double) * na) &&
             (!xc || up((void **)&S->xc, xc, 3 * sizeof(double) * na));
   if (!ok) {
     (void)hipGetLastError();
@@ -547,6 +546,14 @@ extern "C" int tfs_pressure_fractional_device(tfs_gradient *S, const double *d_f
       (S->nf > 0 && (!d_f || !d_s || !d_flux || !d_a12)))
     return TFS_ERR_INVALID;
   const long long na = (long long)S->n + S->nb;
+  // the per-cell records (48 bytes per cell slot) on the first call only:
+  // meshes that never run this face loop do not hold them
+  if (!S->rec && hipMalloc((void **)&S->rec, std::max<size_t>(6 * sizeof(double) * na, 8)) !=
+                     hipSuccess) {
+    (void)hipGetLastError();
+    S->rec = nullptr;
+    return TFS_ERR_NOMEM;
+  }
   hipLaunchKernelGGL(k_pf_cells, dim3(grid_of(na)), dim3(kThreads), 0, nullptr, S->n, na, d_vol,
                      d_sav, d_u, d_v, d_w, d_px, d_py, d_pz, d_p, d_den, S->rec);
   if (S->nf > 0)

@@ -2003,6 +2003,9 @@ extern "C" int tfs_timing_stop(double *ms, long long *launches) {
   g_tm.on = false;
   if (ms) *ms = g_tm.ms;
   if (launches) *launches = g_tm.launches;
+  // the events go back now: none outlives the measurement into process exit
+  for (hipEvent_t e : g_tm.ev) (void)hipEventDestroy(e);
+  g_tm.ev.clear();
   return TFS_OK;
 }
 

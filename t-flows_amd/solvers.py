@@ -20,7 +20,9 @@ the reference there is no exception for non-convergence; the last status is
 kept in ``A.status``.  Invalid arguments and device failures raise.
 There is no CPU fallback: without the HIP library this module raises.
 """
+import atexit
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -63,6 +65,30 @@ class _Info(ctypes.Structure):
                 ("inner_levels", ctypes.c_int)]
 
 
+_LIVE = []  # weak references to the library objects, in creation order
+
+
+def _track(obj):
+    _LIVE.append(weakref.ref(obj))
+
+
+def _close_all():
+    """Destroys the library objects still alive, newest first (an inner or
+    dependent object before what it was made from).  Registered with atexit
+    after torch is imported, so it runs before torch's exit handlers, before
+    the interpreter's teardown (whose garbage collection would call the
+    destroys in no particular order) and before the C runtime's exit handlers
+    shut the HIP runtime down: every stream, event and allocation goes back
+    while everything it belongs to is still up."""
+    while _LIVE:
+        obj = _LIVE.pop()()
+        if obj is not None:
+            try:
+                obj.close()
+            except Exception:  # pragma: no cover -- best effort at exit
+                pass
+
+
 def _lib():
     # torch first: its HIP runtime then serves this library too (same soname),
     # so device pointers from torch tensors are valid here.
@@ -70,6 +96,7 @@ def _lib():
     lib = _native.solver_lib()
     if getattr(lib, "_tfs_typed", False):
         return lib
+    atexit.register(_close_all)
     lib.tfs_status_string.restype = ctypes.c_char_p
     lib.tfs_status_string.argtypes = [ctypes.c_int]
     lib.tfs_init.argtypes = [ctypes.c_int]
@@ -137,6 +164,17 @@ def _dptr(t, n, what):
     return t.data_ptr()
 
 
+def _check_count(t, n, what, exact=True):
+    """Size check before any device access: a per-face or per-cell array
+    whose length differs from the mesh's would make the kernels read or write
+    past its end (or with the wrong stride)."""
+    if t is None:
+        return
+    m = t.numel() if hasattr(t, "numel") else np.asarray(t).size
+    if (m != n) if exact else (m < n):
+        raise ValueError(f"{what}: {m} entries, the mesh needs {'exactly ' if exact else ''}{n}")
+
+
 def _sync():
     import torch
     torch.cuda.synchronize()
@@ -156,6 +194,7 @@ class Matrix:
         _check(lib.tfs_matrix_create(n, nnz, row, col, dia, base, ctypes.byref(h)),
                "tfs_matrix_create")
         self._h = h
+        _track(self)
         self._lib = lib
         self.n = n
         self.nnz = nnz
@@ -426,6 +465,7 @@ class Batch:
                                     ptrs(rows), ptrs(cols), ptrs(dias), base, ctypes.byref(h)),
                "tfs_batch_create")
         self._h = h
+        _track(self)
         for s, sysd in enumerate(systems):
             if len(sysd) > 3 and sysd[3] is not None:
                 self.set_values(s, sysd[3])
@@ -532,6 +572,7 @@ class Server:
         _check(lib.tfs_server_create(int(hbm_budget_bytes), int(workers), ctypes.byref(h)),
                "tfs_server_create")
         self._h = h
+        _track(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -612,6 +653,7 @@ class Assembly:
                                        np.ascontiguousarray(dia, np.int32), base, ctypes.byref(h)),
                "tfs_assembly_create")
         self._h = h
+        _track(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -697,6 +739,7 @@ class Gradient:
         self._lib = lib
         self.n, self.nb = int(n_cells), int(n_bnd_cells)
         faces_c = np.ascontiguousarray(faces_c, np.int32)
+        self.nf = faces_c.size // 2  # the mesh's face count: every per-face array has it
         d = np.ascontiguousarray(d, np.float64).ravel()
         self._keep = [None if xc is None else np.ascontiguousarray(xc, np.float64).ravel(),
                       None if bc is None else np.ascontiguousarray(bc, np.uint8)]
@@ -706,6 +749,7 @@ class Gradient:
                                        None if bc is None else self._keep[1].ctypes.data,
                                        ctypes.byref(h)), "tfs_gradient_create")
         self._h = h
+        _track(self)
 
     def matrix(self):
         """G(1:6, c) as an (n_cells, 6) host array."""
@@ -734,8 +778,13 @@ class Gradient:
         returns (errmax, cfl_max, pe_max).  coef per face = A % val(A % pos(1, s));
         den per cell = DENc(material(c)); in_m (uint8 per face) marks the
         faces of the material whose CFL / Peclet numbers are reported."""
-        n, na = self.n, self.n + self.nb
-        nf = flux.numel()
+        n, na, nf = self.n, self.n + self.nb, self.nf
+        for t, what in ((coef, "coef"), (scoef, "scoef"), (flux, "flux"), (in_m, "in_m")):
+            _check_count(t, nf, what)
+        for t, m, what in ((u, na, "u"), (v, na, "v"), (w, na, "w"), (pp, na, "pp"),
+                           (vol, n, "vol"), (sav, n, "sav"), (px, n, "px"), (py, n, "py"),
+                           (pz, n, "pz"), (den, n, "den"), (b, n, "b"), (bou, self.nb, "bou")):
+            _check_count(t, m, what, exact=False)
 
         def opt(t, m, what, dtype=None):
             if t is None:
@@ -762,8 +811,14 @@ class Gradient:
     def pressure_fractional(self, f, s, vol, sav, u, v, w, px, py, pz, p, den, flux, a12, b):
         """Compute_Pressure_Fractional's face loop on the GPU: flux and a12 per
         face, b per cell (GPU float64 tensors; s holds sx, sy, sz blocks)."""
-        n, na = self.n, self.n + self.nb
-        nf = flux.numel()
+        n, na, nf = self.n, self.n + self.nb, self.nf
+        for t, what in ((f, "f"), (flux, "flux"), (a12, "a12")):
+            _check_count(t, nf, what)
+        _check_count(s, 3 * nf, "s (sx, sy, sz blocks)")
+        for t, m, what in ((u, na, "u"), (v, na, "v"), (w, na, "w"), (p, na, "p"),
+                           (den, na, "den"), (vol, n, "vol"), (sav, n, "sav"), (px, n, "px"),
+                           (py, n, "py"), (pz, n, "pz"), (b, n, "b")):
+            _check_count(t, m, what, exact=False)
         _sync()
         _check(self._lib.tfs_pressure_fractional_device(
             self._h, _dptr(f, nf, "f"), _dptr(s, 3 * nf, "s"), _dptr(vol, n, "vol"),

@@ -3,7 +3,10 @@ T-FlowS calls it: a Fortran program whose Matrix_Type comes from the
 reference's own Matrix_Mod_Topology (oracle/_ref/shim/shim_driver, built by
 `make -C oracle shim` where the reference tree exists) calls Cg / Bicg / Cgs
 of the drop-in for every preconditioner; its outputs are checked against the
-oracle with the parity bar of the direct solves."""
+oracle with the parity bar of the direct solves.  It then solves the same
+system with the values assembled on the device (Solvers_Assemble_On_Device)
+and handed over from a device copy (Solvers_Values_From_Device): both must
+give the host-upload solve bit for bit."""
 import json
 import pathlib
 import subprocess
@@ -45,7 +48,11 @@ def test_fortran_drop_in(oracle, tmp_path, kw):
     assert res.returncode == 0
     raw = (tmp_path / "out.bin").read_bytes()
     rec = 4 + 16 + 8 * c.n
-    assert len(raw) == 9 * rec
+    assert len(raw) == 11 * rec
+    # values assembled on the device, and handed over from a device copy: the
+    # same solve as with the host upload (Cg, prec 2 is record 2), bit for bit
+    for k in (9, 10):
+        assert raw[k * rec:(k + 1) * rec] == raw[2 * rec:3 * rec], k
     tol = 1e-10 * float(np.linalg.norm(c.b))
     k = 0
     for method in ("cg", "bicg", "cgs"):
