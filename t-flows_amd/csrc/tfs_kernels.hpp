@@ -33,6 +33,8 @@ constexpr int kSpmvRows = 64;            // rows per SpMV row block (one per lan
 constexpr int kSpmvCap = 512;            // entries per row block staged in LDS
 constexpr int kRedThreads = 256;         // threads of the vector kernels
 constexpr int kMaxGrid = 2048;           // bound of every partial-sum grid
+constexpr int kSweepGrid = 2048;         // workgroup cap of one coloured-sweep launch
+constexpr int kMaxPartials = 32768;      // partial-sum slots per matrix
 constexpr int kLevelThreads = 256;       // threads per block, one level launch
 constexpr int kSmallThreads = 1024;      // one workgroup walking small levels
 constexpr int kSmallLevel = 2048;        // a level this small joins a walked run
@@ -419,23 +421,29 @@ k_ell_diff(int n, int nslice, const long long *__restrict__ off, const int *__re
   if (DOT) store_partial<kEllWaves * kWave>(red, partials);
 }
 
-// Incomplete-Cholesky sweeps of a point-coloured inner order (rows of one
-// colour are independent: every triangle entry of a row lies in another
-// colour), one launch per colour over that colour's ELL slices (64 rows,
-// entries column-major, as k_ell_diff):
-//   forward   x_i = (b_i - sum_{j<i} a_ij x_j) d_i              (Prec_Solve.f90:46-53)
-//   backward  x_i = (b_i / (d_i + TINY) - sum_{j>i} a_ij x_j) d_i  (:55-67, scale fused)
-// in FP64 arithmetic on FP32 storage.  Vectors are in the inner copy's S
-// order (the colour order), so b, d and x stream contiguously.  Padding
-// entries have column -1 (x there may hold anything, even a NaN).
-template <bool BWD>
+// Incomplete-Cholesky preconditioner z = M^-1 r of a point-coloured inner
+// order (rows of one colour are independent: every triangle entry of a row
+// lies in another colour), over per-colour ELL slices (64 rows, entries
+// column-major, as k_ell_diff), in FP64 arithmetic on FP32 storage:
+//   forward   y_i = (r_i - sum_{j<i} a_ij y_j) d_i                (Prec_Solve.f90:46-53)
+//   backward  z_i = (y_i / (d_i + TINY) - sum_{j>i} a_ij z_j) d_i   (:55-67, scale fused)
+// The first colour has no lower entries (its forward pass is y = r d, a
+// plain stream) and the last colour no upper ones: its z comes out of its
+// forward launch (LAST), so red-black takes three launches.  (Forming the
+// first colour's y where it is gathered instead of storing it doubles the
+// gathers of the next colour: slower at 256^3.)  Every launch that writes z
+// adds its rows' part of rho = (r, z) (Cg.f90:108-112) to its own partial
+// sums.
+template <bool BWD, bool LAST>
 __global__ void __launch_bounds__(kEllWaves * kWave)
 k_colour_sweep(int s0, int s1, const int *__restrict__ row0, const long long *__restrict__ off,
                const int *__restrict__ col, const float *__restrict__ val,
-               const float *__restrict__ d, const float *__restrict__ b, float *__restrict__ x,
-               int row_end, const KState *st) {
+               const float *__restrict__ d, const float *__restrict__ r, float *__restrict__ y,
+               float *__restrict__ z, int row_end, const KState *st, double *partials) {
   if (st && solve_done(st)) return;
   const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  const float *__restrict__ xin = BWD ? z : y;  // the dependencies
+  double rho = 0.0;
   for (int s = s0 + blockIdx.x * kEllWaves + wv; s < s1; s += gridDim.x * kEllWaves) {
     const int su = __builtin_amdgcn_readfirstlane(s);
     const long long e0 = off[su];
@@ -443,12 +451,19 @@ k_colour_sweep(int s0, int s1, const int *__restrict__ row0, const long long *__
     const int i = row0[su] + lane;
     const bool live = i < row_end;
     const double di = live ? (double)d[i] : 0.0;
-    double acc = live ? (double)b[i] : 0.0;
-    if (BWD) acc = acc / (di + kTiny);
+    const double ri = live ? (double)r[i] : 0.0;
+    double acc;
+    if (BWD) {
+      const double yi = live ? (double)y[i] : 0.0;
+      acc = yi / (di + kTiny);
+    } else {
+      acc = ri;
+    }
     const long long base = e0 + lane;
     for (int j0 = 0; j0 < w; j0 += kEllBatch) {
       int c[kEllBatch];
-      float a[kEllBatch], xv[kEllBatch];
+      float a[kEllBatch];
+      double xv[kEllBatch];
 #pragma unroll
       for (int k = 0; k < kEllBatch; ++k)
         if (j0 + k < w) {
@@ -457,13 +472,26 @@ k_colour_sweep(int s0, int s1, const int *__restrict__ row0, const long long *__
         }
 #pragma unroll
       for (int k = 0; k < kEllBatch; ++k)
-        if (j0 + k < w) xv[k] = c[k] >= 0 ? x[c[k]] : 0.0f;  // -1: padding
+        if (j0 + k < w) {
+          const int cj = c[k];  // -1: padding (the slot may hold anything)
+          xv[k] = cj < 0 ? 0.0 : (double)xin[cj];
+        }
 #pragma unroll
       for (int k = 0; k < kEllBatch; ++k)
-        if (j0 + k < w) acc = acc - (double)a[k] * (double)xv[k];
+        if (j0 + k < w) acc = acc - (double)a[k] * xv[k];
     }
-    if (live) x[i] = (float)(acc * di);
+    if (live) {
+      const double out = acc * di;
+      if (BWD || LAST) {
+        const float zi = (float)(LAST && !BWD ? (out / (di + kTiny)) * di : out);
+        z[i] = zi;
+        rho += ri * (double)zi;
+      } else {
+        y[i] = (float)out;
+      }
+    }
   }
+  if (BWD || LAST) store_partial<kEllWaves * kWave>(rho, partials);
 }
 
 // The sliced-ELL values (FP32) from the FP64 caller values: src[e] is the

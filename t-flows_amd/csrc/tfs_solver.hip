@@ -806,24 +806,38 @@ int prepare_colour_sweeps(tfs_matrix *A) {
   return 0;
 }
 
-// z := M^-1 r for prec 2 on a point-coloured matrix: the forward sweep into y,
-// colour by colour, then the backward sweep into z, colours in reverse; all
-// vectors in S order (FP32), factor d (its L order is its S order).
-void colour_sweeps(tfs_matrix *A, const float *r, float *y, float *z, const KState *st) {
+// z := M^-1 r for prec 2 on a point-coloured matrix (k_colour_sweep): the
+// forward launches of colours 0..K-1 into y (the last writes z instead), then
+// the backward launches of colours K-2..0 into z; all vectors in S order
+// (FP32), factor d (its L order is its S order).  Each launch that writes z
+// leaves per-workgroup partial sums of (r, z) at its own offset; returns
+// their total count, for the rho reduction.
+int colour_sweeps(tfs_matrix *A, const float *r, float *y, float *z, const KState *st) {
   const float *d = A->v32.dL;
-  for (int k = 0; k < 2 * A->csw_colours; ++k) {
-    const bool bwd = k >= A->csw_colours;
-    const int c = bwd ? 2 * A->csw_colours - 1 - k : k;
-    const int s0 = A->csw_sbeg[c], s1 = A->csw_sbeg[c + 1];
-    if (s1 == s0) continue;
-    const int t = bwd ? 1 : 0;
+  const int K = A->csw_colours;
+  // every launch that writes z keeps its own partial slots: at most
+  // kMaxPartials over the K such launches
+  static const int grid_cap = [] {
+    const char *e = std::getenv("TFS_CSW_GRID");  // launch-shape experiments
+    return e && std::atoi(e) > 0 ? std::atoi(e) : kSweepGrid;
+  }();
+  const int cap = std::max(1, std::min(grid_cap, kMaxPartials / std::max(1, K)));
+  int parts = 0;
+  auto launch = [&](int c, bool bwd, bool last) {
+    const int s0 = A->csw_sbeg[c], s1 = A->csw_sbeg[c + 1], t = bwd ? 1 : 0;
+    const int g = grid_for(std::max(1, s1 - s0), kEllWaves, cap);
+    auto kern = bwd ? k_colour_sweep<true, false>
+                    : last ? k_colour_sweep<false, true> : k_colour_sweep<false, false>;
     tm_begin(TFS_KC_SWEEP);
-    hipLaunchKernelGGL(bwd ? k_colour_sweep<true> : k_colour_sweep<false>,
-                       dim3(grid_for(s1 - s0, kEllWaves, 8192)), dim3(kEllWaves * kWave), 0,
-                       cur_stream(), s0, s1, A->csw_row0, A->csw_off[t], A->csw_col[t],
-                       A->csw_val[t], d, bwd ? y : r, bwd ? z : y, A->csw_cstart[c + 1], st);
+    hipLaunchKernelGGL(kern, dim3(g), dim3(kEllWaves * kWave), 0, cur_stream(), s0, s1,
+                       A->csw_row0, A->csw_off[t], A->csw_col[t], A->csw_val[t], d, r, y, z,
+                       A->csw_cstart[c + 1], st, A->partials + parts);
     tm_end(TFS_KC_SWEEP);
-  }
+    if (bwd || last) parts += g;
+  };
+  for (int c = 0; c < K; ++c) launch(c, false, c == K - 1);
+  for (int c = K - 2; c >= 0; --c) launch(c, true, false);
+  return parts;
 }
 
 // The ELL values and row sums from the caller-order FP64 values.
@@ -946,15 +960,17 @@ int refresh_inner(tfs_matrix *A) {
 void reliable_iteration(tfs_matrix *I, int prec, const Vecs<float> &F, double *c, KState *st) {
   const int n = I->n, g = vgrid(n);
   const bool csw = prec == 2 && I->csw_colours > 0;
-  if (csw)
-    colour_sweeps(I, F.r, F.r2, F.z, st);  // z in S order (forward result in slot r2)
-  else if (prec == 2)
-    dic_apply<float>(I, F.r, F.yL, F.zU, st);
-  const int mode = csw ? 3 : prec == 2 ? 2 : prec == 1 ? 1 : 0;
   float *z = prec == 0 ? F.r : F.z;
-  hipLaunchKernelGGL(k_rho<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.r, F.z,
-                     vals<float>(I).s_diag, F.zU, I->s_to_u, mode, st, I->partials);
-  finish(g, I->partials, EPI_RHO, st);
+  if (csw) {
+    // z in S order (forward result in slot r2), with the rho partial sums
+    finish(colour_sweeps(I, F.r, F.r2, F.z, st), I->partials, EPI_RHO, st);
+  } else {
+    if (prec == 2) dic_apply<float>(I, F.r, F.yL, F.zU, st);
+    const int mode = prec == 2 ? 2 : prec == 1 ? 1 : 0;
+    hipLaunchKernelGGL(k_rho<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.r, F.z,
+                       vals<float>(I).s_diag, F.zU, I->s_to_u, mode, st, I->partials);
+    finish(g, I->partials, EPI_RHO, st);
+  }
   hipLaunchKernelGGL(k_pupdate<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, z, F.p, st);
   launch_ell<true>(I, F.p, F.q, st);
   finish(ell_grid(I), I->partials, EPI_ALPHA, st);
@@ -1441,7 +1457,7 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
   TFS_TRY(dev_alloc(A, (void **)&A->stage_x, nb));
   TFS_TRY(dev_alloc(A, (void **)&A->stage_r, nb));
   TFS_TRY(dev_alloc(A, (void **)&A->d_st, 2 * sizeof(KState)));
-  TFS_TRY(dev_alloc(A, (void **)&A->partials, sizeof(double) * kMaxGrid));
+  TFS_TRY(dev_alloc(A, (void **)&A->partials, sizeof(double) * kMaxPartials));
   TFS_TRY(dev_alloc(A, (void **)&A->sweep_err, sizeof(int)));
   TFS_HIP(hipMemset(A->sweep_err, 0, sizeof(int)));
   TFS_TRY(dev_alloc(A, (void **)&A->sweep_ctr, 2 * sizeof(unsigned int)));
