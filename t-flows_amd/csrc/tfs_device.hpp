@@ -68,34 +68,6 @@ __device__ __forceinline__ double block_sum(double v, double *lds) {
   return t;
 }
 
-// Grid reduction epilogue.  Every workgroup stores its partial; the one that
-// arrives last sums all partials in index order (a fixed tree, so the result
-// does not depend on arrival order) and returns true with the total in *tot
-// (thread 0).  The counter is left at zero for the next use.
-template <int NT>
-__device__ __forceinline__ bool grid_sum_last(double part, double *partials, unsigned int *ctr,
-                                              double *tot, double *lds) {
-  __shared__ int s_last;
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = part;
-    __threadfence();
-    const unsigned int prev = atomicAdd(ctr, 1u);
-    s_last = prev == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  double v = 0.0;
-  for (unsigned int k = threadIdx.x; k < gridDim.x; k += NT)
-    v += __hip_atomic_load(&partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  v = block_sum<NT>(v, lds);
-  if (threadIdx.x == 0) {
-    *tot = v;
-    *ctr = 0u;
-  }
-  return true;
-}
-
 __device__ __forceinline__ bool solve_done(const KState *st) {
   return __builtin_amdgcn_readfirstlane(st->done) != 0;
 }
