@@ -485,7 +485,8 @@ def batch(args):
     ns = [c.n for c in sysl]
     log(f"  cells {min(ns)}..{max(ns)}, total {sum(ns)}")
     B = solvers.Batch([(c.row, c.col, c.dia, c.val) for c in sysl])
-    B.set_refinement(solvers.TFS_REFINE_FCG if args.refine == "fcg" else solvers.TFS_REFINE_DEFECT)
+    B.set_refinement({"reliable": solvers.TFS_REFINE_RELIABLE, "fcg": solvers.TFS_REFINE_FCG,
+                      "defect": solvers.TFS_REFINE_DEFECT}[args.refine])
     if args.inner_block:
         B.set_inner_block(args.inner_block)
     if args.inner_order:
@@ -926,8 +927,8 @@ def main():
                     help="first run a plain device copy of this many MiB, 5 times (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
-    if args.inner_rtol is None:
-        args.inner_rtol = 1e-3 if args.workload == 4 else 0.0  # 0: the solver's default
+    if args.inner_rtol is None:  # 0: the solver's default (reliable updates: 0.1)
+        args.inner_rtol = 1e-3 if args.workload == 4 and args.refine == "fcg" else 0.0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               f"(the solver path runs on a single MI355X)", file=sys.stderr)

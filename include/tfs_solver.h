@@ -232,7 +232,10 @@ int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *const *d_x,
 int tfs_batch_set_inner_order(tfs_batch *B, int order);
 int tfs_batch_set_inner_block(tfs_batch *B, int rows);
 /* Outer loop of the batch's refinement, as tfs_matrix_set_refinement
- * (default TFS_REFINE_FCG: flexible CG with per-system scalars). */
+ * (default TFS_REFINE_RELIABLE: per system one FP32 CG recurrence on the
+ * colour-ordered copy with reliable FP64 updates, in rounds in which every
+ * unconverged system reduces its residual by inner_rtol; with
+ * TFS_INNER_REFERENCE the batch runs flexible CG instead). */
 int tfs_batch_set_refinement(tfs_batch *B, int outer);
 /* spmv_rows / spmv_waves: the SpMV shape the batch's tuner picked at creation. */
 int tfs_batch_get_info(const tfs_batch *B, int *nsys, int *n_total, long long *nnz_total,

@@ -645,6 +645,25 @@ k_axpy_norm_seg(Tiles tl, T *__restrict__ x, T *__restrict__ r, const T *__restr
   store_partial<kRedThreads>(s, partials);
 }
 
+// The reliable-update iteration tail per system (k_axpy_mixed over tiles):
+// c += alpha_s p (FP64 correction), r -= alpha_s q (FP32 recurrence residual),
+// per-tile partial ||r||^2.
+__global__ void __launch_bounds__(kRedThreads)
+k_axpy_mixed_seg(Tiles tl, double *__restrict__ c, float *__restrict__ r,
+                 const float *__restrict__ p, const float *__restrict__ q, const KState *st,
+                 const KState *all, double *partials) {
+  if (tile_skip(tl, st, all)) return;
+  const double alpha = st[tl.sys[blockIdx.x]].alpha;
+  double s = 0.0;
+  TFS_TILE_LOOP(i, tl) {
+    c[i] = c[i] + alpha * (double)p[i];
+    const float ri = (float)((double)r[i] - alpha * (double)q[i]);
+    r[i] = ri;
+    s += (double)ri * (double)ri;
+  }
+  store_partial<kRedThreads>(s, partials);
+}
+
 // out = in on the tiles of systems still running (the batched residual is
 // formed in a scratch vector and kept only where the solve continues).
 template <typename T>

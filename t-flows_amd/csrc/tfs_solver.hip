@@ -2044,15 +2044,16 @@ struct tfs_batch {
   KState *d_st = nullptr;
   KState *h_st = nullptr;
   double *partials = nullptr;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};  // [0] FP64 iteration, [1] FP32
-  int graph_prec[2] = {-1, -1};
+  // [0] FP64 iteration, [1] FP32 (flexible-CG inner solves), [2] FP32 reliable updates
+  hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+  int graph_prec[3] = {-1, -1, -1};
   bool use_graph = true;
   int graph_launches = 0;
   // FP32 inner solves of the refinement on a colour-ordered copy (each
   // system's rows grouped by colour, systems in order), built on first use
   int inner_order = TFS_INNER_COLOURED;
   int inner_block = kInnerBlock;
-  int refine = TFS_REFINE_FCG;  // outer loop of the refinement
+  int refine = TFS_REFINE_RELIABLE;  // outer loop of the refinement
   tfs_batch *inner = nullptr;
   int *inner_emap = nullptr;    // inner's entry k <- entry of A->val
   int *inner_from_s = nullptr;  // S position of A for inner's S position i
@@ -2095,14 +2096,12 @@ void batch_cg_iteration(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KS
   finish_seg(B, EPI_NORM, st, all);
 }
 
-// Captures kGraphIters batched iterations into a graph (once per precision
-// and preconditioner).  If the capture fails -- e.g. under the cooperative
-// schedule, whose launch is not capturable -- it is retried with per-level
-// sweeps, and without a graph as a last resort.
-template <typename T>
-int batch_graph(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KState *all,
-                hipGraphExec_t *exec) {
-  const int slot = sizeof(T) == 8 ? 0 : 1;
+// Captures kGraphIters batched iterations (`one()`) into the graph of `slot`
+// (once per slot and preconditioner).  If the capture fails -- e.g. under the
+// cooperative schedule, whose launch is not capturable -- it is retried with
+// per-level sweeps, and without a graph as a last resort.
+template <typename F>
+int batch_graph_fn(tfs_batch *B, int slot, int prec, F &&one, hipGraphExec_t *exec) {
   *exec = nullptr;
   if (!B->use_graph) return 0;
   if (B->graph[slot] && B->graph_prec[slot] == prec) {
@@ -2117,7 +2116,7 @@ int batch_graph(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KState *al
     const bool was_persistent = B->A->persistent;
     hipGraph_t g = nullptr;
     if (hipStreamBeginCapture(cur_stream(), hipStreamCaptureModeRelaxed) != hipSuccess) break;
-    for (int k = 0; k < kGraphIters; ++k) batch_cg_iteration<T>(B, prec, V, st, all);
+    for (int k = 0; k < kGraphIters; ++k) one();
     const hipError_t e1 = hipStreamEndCapture(cur_stream(), &g);
     hipError_t e2 = hipErrorUnknown;
     if (e1 == hipSuccess && g) e2 = hipGraphInstantiate(&B->graph[slot], g, nullptr, nullptr, 0);
@@ -2139,6 +2138,13 @@ int batch_graph(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KState *al
   return 0;
 }
 
+template <typename T>
+int batch_graph(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KState *all,
+                hipGraphExec_t *exec) {
+  return batch_graph_fn(B, sizeof(T) == 8 ? 0 : 1, prec,
+                        [&] { batch_cg_iteration<T>(B, prec, V, st, all); }, exec);
+}
+
 int batch_read(tfs_batch *B) {
   TFS_HIP(hipMemcpyAsync(B->h_st, B->d_st, sizeof(KState) * (2 * B->nsys + 2),
                          hipMemcpyDeviceToHost, cur_stream()));
@@ -2152,25 +2158,30 @@ int batch_write(tfs_batch *B) {
   return 0;
 }
 
-// Replays iterations until every system is done (states read between chunks).
-template <typename T>
-int batch_iterate(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KState *all, int cap) {
+// Replays iterations (`one()`, captured in the graph of `slot`) until every
+// system is done (states read between chunks).
+template <typename F>
+int batch_iterate_fn(tfs_batch *B, int slot, int prec, KState *st, KState *all, int cap,
+                     F &&one) {
   hipGraphExec_t exec = nullptr;
-  TFS_TRY(batch_graph<T>(B, prec, V, st, all, &exec));
+  TFS_TRY(batch_graph_fn(B, slot, prec, one, &exec));
   const int step = exec ? kGraphIters : 1;
   const KState *ha = B->h_st + (all - B->d_st);
   int queued = 0, chunk = step;
   std::vector<int> it_prev(B->nsys, 0);
   std::vector<double> err_prev(B->nsys, 0.0);
   const KState *hs = B->h_st + (st - B->d_st);
-  for (int s = 0; s < B->nsys; ++s) err_prev[s] = hs[s].err;
+  for (int s = 0; s < B->nsys; ++s) {
+    err_prev[s] = hs[s].err;
+    it_prev[s] = hs[s].iter;
+  }
   while (!ha->done) {
     for (int k = 0; k < chunk; k += step) {
       if (exec) {
         TFS_HIP(hipGraphLaunch(exec, cur_stream()));
         B->graph_launches++;
       } else {
-        batch_cg_iteration<T>(B, prec, V, st, all);
+        one();
       }
     }
     queued += chunk;
@@ -2196,6 +2207,12 @@ int batch_iterate(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KState *
     chunk = std::max(step, std::min(32, next) / step * step);
   }
   return 0;
+}
+
+template <typename T>
+int batch_iterate(tfs_batch *B, int prec, const Vecs<T> &V, KState *st, KState *all, int cap) {
+  return batch_iterate_fn(B, sizeof(T) == 8 ? 0 : 1, prec, st, all, cap,
+                          [&] { batch_cg_iteration<T>(B, prec, V, st, all); });
 }
 
 // caller per-system vectors <-> combined S-order vector
@@ -2348,6 +2365,7 @@ int build_batch_inner(tfs_batch *B) {
   }
   Layout Ly;
   int rc = batch_build(C, n, nnz, prow.data(), pcol.data(), pdia.data(), Ly);
+  if (rc == 0) rc = build_ell(C->A, Ly);  // the reliable-update loop's FP32 operator
   if (rc < 0) {
     tfs_batch_destroy(C);
     return rc;
@@ -2514,6 +2532,165 @@ extern "C" int tfs_batch_solve_device(tfs_batch *B, int prec, double *const *d_x
 
 // FP32-inner PCG in FP64 iterative refinement for every system at once
 // (see solve_mixed_device); steps / inner iterations are per system.
+namespace {
+
+// One reliable-update iteration of every system on the batch's inner copy I
+// (reliable_iteration per system): z = M^-1 r, rho, p, q = A32 p in
+// difference form, (p, q), c += alpha p, r -= alpha q, ||r||.
+void batch_reliable_iteration(tfs_batch *I, int prec, const Vecs<float> &F, double *c,
+                              KState *st, KState *all) {
+  tfs_matrix *A = I->A;
+  const Tiles tl = tiles_of(I);
+  const int nt = I->ntiles;
+  if (prec == 2) dic_apply<float>(A, F.r, F.yL, F.zU, all);
+  const int mode = prec == 2 ? 2 : prec == 1 ? 1 : 0;
+  float *z = prec == 0 ? F.r : F.z;
+  hipLaunchKernelGGL(k_rho_seg<float>, dim3(nt), dim3(kRedThreads), 0, cur_stream(), tl, F.r, F.z,
+                     vals<float>(A).s_diag, F.zU, A->s_to_u, mode, st, all, I->partials);
+  finish_seg(I, EPI_RHO, st, all);
+  hipLaunchKernelGGL(k_pupdate_seg<float>, dim3(nt), dim3(kRedThreads), 0, cur_stream(), tl, z,
+                     F.p, st, all);
+  launch_ell<false>(A, F.p, F.q, all);
+  hipLaunchKernelGGL(k_dot_seg<float>, dim3(nt), dim3(kRedThreads), 0, cur_stream(), tl, F.p, F.q,
+                     st, all, I->partials);
+  finish_seg(I, EPI_ALPHA, st, all);
+  hipLaunchKernelGGL(k_axpy_mixed_seg, dim3(nt), dim3(kRedThreads), 0, cur_stream(), tl, c, F.r,
+                     F.p, F.q, st, all, I->partials);
+  finish_seg(I, EPI_NORM, st, all);
+}
+
+// The batch's reliable-update refinement (TFS_REFINE_RELIABLE, see
+// solve_reliable_device): every system runs one FP32 CG recurrence on the
+// colour-ordered inner copy, its directions summed into an FP64 correction;
+// in rounds, each unconverged system runs until its recurrence residual has
+// fallen by delta (its tiles idle once it has), then every such system's
+// correction goes into x and its true FP64 residual replaces the recurrence's.
+// Systems are independent: a system's path does not depend on the others.
+int batch_solve_reliable(tfs_batch *B, int prec, double *const *d_x, double *const *d_r1, int cap,
+                         const double *tol, double delta, int *steps, int *inner_iters,
+                         double *ini_res, double *fin_res, int *status) {
+  if (!(delta > 0.0)) delta = 0.1;
+  delta = std::min(0.5, std::max(1e-6, delta));
+  tfs_matrix *A = B->A;
+  const int n = A->n, ns = B->nsys;
+  TFS_TRY(build_batch_inner(B));
+  tfs_batch *I = B->inner;
+  if (!A->inner_ready) {
+    gather((int)A->nnz, A->val, B->inner_emap, I->A->val);
+    values_changed(I->A);
+    A->inner_ready = true;
+  }
+  const int *to_s = B->inner_from_s;  // I's S position -> B's
+  TFS_TRY(ensure_vec64(A, 8));
+  TFS_TRY(ensure_vec32(I->A, 7));
+  TFS_TRY(prepare_values<double>(A));
+  TFS_TRY(prec_form<float>(I->A, prec));
+  TFS_TRY(prepare_ell(I->A));
+  const Vecs<double> D = vecs_of(A->vec64, n);
+  double *bS = D.r2, *c = D.z;  // c: FP64 correction in I's S order
+  const Vecs<float> F = vecs_of(I->A->vec32, n);
+  KState *so = B->d_st + ns, *allo = B->d_st + 2 * ns + 1;
+  KState *si = I->d_st, *alli = I->d_st + 2 * ns;
+  KState *hi = I->h_st, *ho = B->h_st + ns;
+  const Tiles tl = tiles_of(B);
+  const int g = vgrid(n);
+  TFS_TRY(batch_in(B, d_x, A->stage_x, D.x));
+  TFS_TRY(batch_in(B, d_r1, A->stage_r, bS));  // b
+  std::vector<int> caps(ns, cap), stalls(ns, 0), updates(ns, 0), st_out(ns, TFS_OK);
+  std::vector<double> err(ns), err0(ns), best(ns);
+  auto true_residual = [&]() -> int {
+    batch_init_states(B, ns, caps.data(), tol);
+    TFS_TRY(batch_write(B));
+    gather(n, bS, (const int *)nullptr, D.r);
+    launch_spmv<double, SPMV_RESID>(A, D.x, D.r, nullptr, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_sumsq_seg<double>, dim3(B->ntiles), dim3(kRedThreads), 0, cur_stream(),
+                       tl, D.r, (int)EPI_BNRM, so, allo, B->partials);
+    finish_seg(B, EPI_BNRM, so, allo);
+    TFS_HIP(hipGetLastError());
+    return batch_read(B);
+  };
+  TFS_TRY(true_residual());
+  for (int s = 0; s < ns; ++s) err[s] = err0[s] = best[s] = ho[s].bnrm2;
+  hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, c, 0.0);
+  std::vector<char> active(ns, 0), started(ns, 0);
+  for (;;) {
+    int nact = 0;
+    for (int s = 0; s < ns; ++s) {
+      active[s] = err[s] >= tol[s] && st_out[s] == TFS_OK;
+      if (active[s] && updates[s] >= cap) {
+        st_out[s] = TFS_NOT_CONVERGED;
+        active[s] = 0;
+      }
+      nact += active[s];
+    }
+    if (!nact) break;
+    // each active system's recurrence continues from its true residual
+    gather(n, D.r, to_s, F.r);
+    int ndone = 0;
+    for (int s = 0; s < ns; ++s) {
+      KState &h = hi[s];
+      if (!started[s]) {
+        std::memset(&h, 0, sizeof(KState));
+        h.cap = kSegmentCap;
+      } else {
+        h.rho_old = h.rho;  // the last iteration's rho (kept on its exit)
+        h.cap = h.iter + kSegmentCap;
+      }
+      h.done = active[s] ? 0 : 1;
+      h.status = 0;
+      h.tol = std::max(delta * err[s], 0.5 * tol[s]);
+      h.err = err[s];
+      if (active[s]) started[s] = 1;
+      ndone += !active[s];
+    }
+    KState &ha = hi[2 * ns];
+    std::memset(&ha, 0, sizeof(KState));
+    ha.cap = ns;
+    ha.ctr[0] = (unsigned)ndone;
+    TFS_TRY(batch_write(I));
+    TFS_TRY(batch_iterate_fn(I, 2, prec, si, alli, 1 << 30,
+                             [&] { batch_reliable_iteration(I, prec, F, c, si, alli); }));
+    // x += c for every system (c is zero for the idle ones); true residuals
+    hipLaunchKernelGGL(k_fold_correction, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, D.x, c,
+                       to_s);
+    TFS_TRY(true_residual());
+    for (int s = 0; s < ns; ++s) {
+      if (!active[s]) continue;
+      ++updates[s];
+      const double e = ho[s].bnrm2;
+      err[s] = e;
+      if (hi[s].status == kStBreakdown || !std::isfinite(e)) {
+        st_out[s] = TFS_ERR_BREAKDOWN;
+      } else if (hi[s].status == kStNotConverged && e >= tol[s]) {
+        st_out[s] = TFS_NOT_CONVERGED;  // a segment ran out of iterations
+      } else {
+        stalls[s] = e >= best[s] ? stalls[s] + 1 : 0;
+        best[s] = std::min(best[s], e);
+        if (stalls[s] >= 3 && e >= tol[s]) st_out[s] = TFS_NOT_CONVERGED;
+      }
+    }
+  }
+  TFS_TRY(batch_out(B, D.x, A->stage_x, d_x));
+  TFS_TRY(batch_out(B, D.r, A->stage_r, d_r1));
+  TFS_HIP(hipGetLastError());
+  TFS_HIP(hipStreamSynchronize(cur_stream()));
+  TFS_TRY(check_sweep(A));
+  TFS_TRY(check_sweep(I->A));
+  int worst = TFS_OK;
+  for (int s = 0; s < ns; ++s) {
+    ini_res[s] = err0[s];
+    fin_res[s] = err[s];
+    status[s] = st_out[s];
+    steps[s] = st_out[s] == TFS_NOT_CONVERGED ? cap + 1 : updates[s];
+    inner_iters[s] = started[s] ? hi[s].iter : 0;
+    if (status[s] == TFS_ERR_BREAKDOWN || (status[s] == TFS_NOT_CONVERGED && worst == TFS_OK))
+      worst = status[s];
+  }
+  return worst;
+}
+
+}  // namespace
+
 extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *const *d_x,
                                             double *const *d_r1, int cap, const double *tol,
                                             double inner_rtol, int *steps, int *inner_iters,
@@ -2523,6 +2700,10 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
   if (!B || !d_x || !d_r1 || !tol || !steps || !inner_iters || !ini_res || !fin_res || !status)
     return TFS_ERR_INVALID;
   if (prec < 0 || prec > 2) prec = 0;
+  if (B->refine == TFS_REFINE_RELIABLE && B->inner_order == TFS_INNER_COLOURED)
+    return batch_solve_reliable(B, prec, d_x, d_r1, cap, tol, inner_rtol, steps, inner_iters,
+                                ini_res, fin_res, status);
+  if (!(inner_rtol > 0.0)) inner_rtol = 1e-2;
   inner_rtol = std::min(0.5, std::max(1e-6, inner_rtol));
   tfs_matrix *A = B->A;
   const int n = A->n, ns = B->nsys;
@@ -2765,7 +2946,8 @@ extern "C" int tfs_batch_set_inner_block(tfs_batch *B, int rows) {
 }
 
 extern "C" int tfs_batch_set_refinement(tfs_batch *B, int outer) {
-  if (!B || (outer != TFS_REFINE_FCG && outer != TFS_REFINE_DEFECT)) return TFS_ERR_INVALID;
+  if (!B || (outer != TFS_REFINE_FCG && outer != TFS_REFINE_DEFECT && outer != TFS_REFINE_RELIABLE))
+    return TFS_ERR_INVALID;
   B->refine = outer;
   return TFS_OK;
 }

@@ -530,7 +530,7 @@ class Batch:
                "tfs_batch_solve_device")
         return niter, ini, fin, st
 
-    def solve_mixed(self, xs, rs, prec, cap, tols, inner_rtol=1e-2):
+    def solve_mixed(self, xs, rs, prec, cap, tols, inner_rtol=0.0):
         """FP32-inner refinement per system; arrays (steps, inner, ini, fin, status)."""
         px, pr = self._vecs(xs, rs)
         tols = np.ascontiguousarray(tols, np.float64)

@@ -73,18 +73,22 @@ def test_batch_equals_single_solves(torch, S, oracle, prec):
     assert B.info()["graph_launches"] > 0
 
 
-@pytest.mark.parametrize("outer", ["fcg", "defect"])
+REFINE = {"reliable": "TFS_REFINE_RELIABLE", "fcg": "TFS_REFINE_FCG", "defect": "TFS_REFINE_DEFECT"}
+
+
+@pytest.mark.parametrize("outer", ["reliable", "fcg", "defect"])
 @pytest.mark.parametrize("order", ["reference", "coloured"])
 def test_batch_mixed_refinement(torch, S, oracle, order, outer):
     """Every system reaches its FP64 tolerance (true residual), with the error
     against the exact solution bounded as for the single refinement solve
     (stopping on the residual: within 1e-6, or within 10x of the FP64
     oracle's own error where that is larger).  The colour-ordered inner
-    copy is the same batch of systems, so its results stay per system."""
+    copy is the same batch of systems, so its results stay per system.  (With
+    the reference order the reliable-update setting runs flexible CG.)"""
     sysl = small_batch()
     B = S.Batch([(c.row, c.col, c.dia, c.val) for c in sysl])
     B.set_inner_order(S.TFS_INNER_COLOURED if order == "coloured" else S.TFS_INNER_REFERENCE)
-    B.set_refinement(S.TFS_REFINE_FCG if outer == "fcg" else S.TFS_REFINE_DEFECT)
+    B.set_refinement(getattr(S, REFINE[outer]))
     tols = [1e-10 * float(np.linalg.norm(c.b)) for c in sysl]
     xs = [torch.zeros(c.n, dtype=torch.float64, device="cuda") for c in sysl]
     rs = [torch.from_numpy(c.b.copy()).cuda() for c in sysl]
@@ -102,7 +106,7 @@ def test_batch_mixed_refinement(torch, S, oracle, order, outer):
     if order == "coloured":
         # the same systems alone in a batch give the same results, bitwise
         B1 = S.Batch([(sysl[0].row, sysl[0].col, sysl[0].dia, sysl[0].val)])
-        B1.set_refinement(S.TFS_REFINE_FCG if outer == "fcg" else S.TFS_REFINE_DEFECT)
+        B1.set_refinement(getattr(S, REFINE[outer]))
         x1 = [torch.zeros(sysl[0].n, dtype=torch.float64, device="cuda")]
         r1 = [torch.from_numpy(sysl[0].b.copy()).cuda()]
         s1, i1, _, f1, st1 = B1.solve_mixed(x1, r1, 2, 50, tols[:1])
