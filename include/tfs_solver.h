@@ -304,7 +304,13 @@ int tfs_topology_build_device(int n_cells, long long n_faces, const int *d_faces
  * Device arrays: phi has n_cells + n_bnd entries (cells, then boundary cells);
  * px, py, pz n_cells each.  tfs_grad_p_device rewrites phi's boundary part,
  * as GradP does.  Synchronous, on the calling thread's default stream. */
-enum { TFS_BC_WALL = 0, TFS_BC_SYMMETRY = 1, TFS_BC_PRESSURE = 2, TFS_BC_BUFFER = 3 };
+enum {
+  TFS_BC_WALL = 0, TFS_BC_SYMMETRY = 1, TFS_BC_PRESSURE = 2, TFS_BC_BUFFER = 3,
+  /* as walls for the gradients and the velocity correction (which single out
+     symmetry, pressure and buffer cells only); told apart by the SIMPLE face
+     loop */
+  TFS_BC_INFLOW = 4, TFS_BC_OUTFLOW = 5, TFS_BC_CONVECT = 6
+};
 typedef struct tfs_gradient tfs_gradient;
 int tfs_gradient_create(int n_cells, int n_bnd_cells, long long n_faces, const int *faces_c,
                         const double *d, const double *xc, const unsigned char *bc,
@@ -354,6 +360,31 @@ int tfs_pressure_fractional_device(tfs_gradient *S, const double *d_f, const dou
                                    const double *d_py, const double *d_pz, const double *d_p,
                                    const double *d_den, double *d_flux, double *d_a12,
                                    double *d_b);
+
+/* The face loop of Compute_Pressure_Simple
+ * (Sources/Process/Compute_Pressure_Simple.f90:45-180) on the mesh of a
+ * tfs_gradient object, every material a fluid and no BUFFER boundary
+ * (TFS_ERR_INVALID otherwise): per interior face the density f den(c1) +
+ * (1-f) den(c2), SMDPN = |S|^2 / (S . d) (d: the object's face vectors), the
+ * interpolated velocity and pressure gradient, flux = DENs (u_f . S) +
+ * A12 (p1 - p2) + A12 (grad p_f . d), A12 = 0.5 DENs SMDPN (vol1/sav1 +
+ * vol2/sav2); per boundary face by the kind of its boundary cell (TFS_BC_*):
+ * inflow / outflow / convective take the boundary cell's velocity, pressure
+ * the inside cell's, walls and symmetry planes have no flux; outflow,
+ * convective and pressure faces add A12 = DENs SMDPN vol1/sav1 to their
+ * diagonal (bnd_on = 1, else 0; a12 = 0 where a face adds nothing); b per
+ * cell = inflow - outflow in face order.  Bitwise the reference's.  The
+ * matrix then comes from tfs_assemble_device(a12, NULL, bnd_on, NULL, 1, NULL)
+ * (the reference sets the off-diagonals to -A12, which 0 - A12 equals for
+ * every A12 != 0).  Device arrays: f, s (sx, sy, sz blocks), flux, a12,
+ * bnd_on per face; vol, sav, px, py, pz, b per cell; u, v, w, p, den per cell
+ * then boundary cell. */
+int tfs_pressure_simple_device(tfs_gradient *S, const double *d_f, const double *d_s,
+                               const double *d_vol, const double *d_sav, const double *d_u,
+                               const double *d_v, const double *d_w, const double *d_px,
+                               const double *d_py, const double *d_pz, const double *d_p,
+                               const double *d_den, double *d_flux, double *d_a12,
+                               unsigned char *d_bnd_on, double *d_b);
 
 /* ---- serving: continuous batching of solves on one GPU ------------------------
  * Not in the reference.  A case (one system) is registered with its structure

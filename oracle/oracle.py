@@ -71,7 +71,25 @@ class Oracle:
         lib.or_correct_velocity.argtypes = [ctypes.c_int, ctypes.c_int, _I32P] + [_VP] * 16 + \
             [ctypes.c_double, ctypes.c_double, _VP, _VP, _VP, _VP]
         lib.or_pressure_fractional.argtypes = [ctypes.c_int, ctypes.c_int, _I32P] + [_VP] * 15
+        lib.or_pressure_simple.argtypes = [ctypes.c_int, ctypes.c_int, _I32P] + [_VP] * 14 + \
+            [_I32P, _I32P, ctypes.c_int] + [_VP] * 5
         self.lib = lib
+
+    def pressure_simple(self, n_cells, faces_c, f, s, d, vol, sav, u, v, w, p, px, py, pz, den,
+                        bc, pos, dia, nnz):
+        """Compute_Pressure_Simple's face loop: dict(flux, a12, bnd_on, val, b);
+        faces_c, pos, dia 1-based; bc = kind per boundary cell (or None: walls)."""
+        faces_c, pos, dia = _c32(faces_c), _c32(pos), _c32(dia)
+        nf = faces_c.size // 2
+        keep = [_c64(a).ravel() for a in (f, s, d, vol, sav, u, v, w, p, px, py, pz, den)]
+        bck = None if bc is None else np.ascontiguousarray(bc, np.uint8)
+        flux, a12, b = np.zeros(nf), np.zeros(nf), np.zeros(n_cells)
+        bnd_on, val = np.zeros(nf, np.uint8), np.zeros(nnz)
+        self.lib.or_pressure_simple(n_cells, nf, faces_c, *(a.ctypes.data for a in keep),
+                                    None if bck is None else bck.ctypes.data, pos, dia, int(nnz),
+                                    flux.ctypes.data, a12.ctypes.data, bnd_on.ctypes.data,
+                                    val.ctypes.data, b.ctypes.data)
+        return dict(flux=flux, a12=a12, bnd_on=bnd_on, val=val, b=b)
 
     def pressure_fractional(self, n_cells, faces_c, f, s, vol, sav, u, v, w, px, py, pz, p, den):
         """Compute_Pressure_Fractional's face loop: (flux, a12, b)."""
@@ -246,6 +264,10 @@ class Reference:
             lib.tfs_ref_mesh_press_frac.argtypes = [ctypes.c_void_p] * 8 + \
                 [ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double] + \
                 [ctypes.c_void_p] * 4
+            lib.tfs_ref_mesh_press_simple.restype = ctypes.c_int
+            lib.tfs_ref_mesh_press_simple.argtypes = [ctypes.c_void_p] * 8 + \
+                [ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                 ctypes.c_double] + [ctypes.c_void_p] * 4
             lib.tfs_ref_mesh_get.restype = ctypes.c_int
             lib.tfs_ref_mesh_get.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                              ctypes.c_void_p]
@@ -378,6 +400,23 @@ class Reference:
         if rc != 0:
             raise RuntimeError(f"tfs_ref_mesh_correct returned {rc}")
         return dict(u=u, v=v, w=w, flux=flux, b=b, errmax=res[0], cfl=res[1], pe=res[2])
+
+    def mesh_press_simple(self, n_cells, n_faces, nnz, u, v, w, p, px, py, pz, sav, den, bc,
+                          prec, stol, urf):
+        """Compute_Pressure_Simple on the grid and matrix of the last mesh()
+        call with boundary-cell kinds bc (0 wall, 1 symmetry, 2 pressure,
+        4 inflow, 5 outflow, 6 convective): dict(flux, val, pp, p)."""
+        keep = [_c64(a) for a in (u, v, w, p, px, py, pz, sav)]
+        bck = np.ascontiguousarray(bc, np.int32)
+        flux, val = np.zeros(n_faces), np.zeros(nnz)
+        pp, pn = np.zeros(n_cells), np.zeros(n_cells)
+        rc = self.lib.tfs_ref_mesh_press_simple(*(a.ctypes.data for a in keep), float(den),
+                                                bck.ctypes.data, int(prec), float(stol),
+                                                float(urf), flux.ctypes.data, val.ctypes.data,
+                                                pp.ctypes.data, pn.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"tfs_ref_mesh_press_simple returned {rc}")
+        return dict(flux=flux, val=val, pp=pp, p=pn)
 
     def mesh_press_frac(self, n_cells, n_faces, nnz, u, v, w, p, px, py, pz, sav, den, prec,
                         stol, urf):

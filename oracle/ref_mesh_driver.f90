@@ -25,7 +25,7 @@
   use Grid_Mod
   use Matrix_Mod
   use Info_Mod
-  use Parameters_Mod, only: FRACT
+  use Parameters_Mod, only: FRACT, SIMPLE
   use Work_Mod
   use Solvers_Mod, only: D
   implicit none
@@ -323,6 +323,76 @@
   cpp(1:nc)      = PP % n(1:nc)
   cpn(1:nc)      = P % n(1:nc)
   tfs_ref_mesh_press_frac = 0
+  end function
+
+  !---------------------------------------------------------------------------!
+  !   Compute_Pressure_Simple on the grid and matrix last loaded (every        !
+  !   material a fluid of density den): boundary cell -b of kind bc(b) (0 WALL, !
+  !   1 SYMMETRY, 2 PRESSURE, 4 INFLOW, 5 OUTFLOW, 6 CONVECT), the face fluxes, !
+  !   the pressure-correction matrix, its Bicg solve (prec, tolerance stol,    !
+  !   cap 40 as the routine sets it) and the pressure update by urf.           !
+  !   cu, cv, cw, cp: cells then boundary cells; gx, gy, gz, csav: cells.     !
+  !   Out: flux(nf), val(nnz) = A % val, pp(nc) = PP % n, pn(nc) = P % n.     !
+  !---------------------------------------------------------------------------!
+  integer(c_int) function tfs_ref_mesh_press_simple(cu, cv, cw, cp, gx, gy, gz, csav, den, &
+                          bc, cprec, stol, urf, cflux, cval, cpp, cpn)                     &
+                 bind(C, name='tfs_ref_mesh_press_simple')
+  real(c_double), intent(in)  :: cu(*), cv(*), cw(*), cp(*), gx(*), gy(*), gz(*), csav(*)
+  real(c_double), value       :: den, stol, urf
+  integer(c_int), intent(in)  :: bc(*)
+  integer(c_int), value       :: cprec
+  real(c_double), intent(out) :: cflux(*), cval(*), cpp(*), cpn(*)
+  integer, parameter :: kinds(0:6) = (/WALL, SYMMETRY, PRESSURE, BUFFER, INFLOW, OUTFLOW,  &
+                                       CONVECT/)
+  integer :: c, nc, nb, nf
+
+  tfs_ref_mesh_press_simple = -1
+  if(.not. allocated(mg)) return
+  nc = m_nc;  nb = m_nb;  nf = m_nf
+  do c = 1, nb
+    TypeBC(-c) = kinds(bc(c))
+  end do
+  material = 1
+  StateMat = FLUID
+  DENc(1) = den
+  ALGOR   = SIMPLE
+  PREC    = cprec
+  PP % STol = stol
+  P % URF   = urf
+  if(allocated(U % n)) deallocate(U % n, V % n, W % n, PP % n, P % x, P % y, P % z)
+  if(allocated(P % n)) deallocate(P % n)
+  if(allocated(Flux))  deallocate(Flux)
+  if(allocated(b))     deallocate(b)
+  allocate(U % n(-nb:nc), V % n(-nb:nc), W % n(-nb:nc), PP % n(-nb:nc), P % n(-nb:nc))
+  allocate(P % x(-nb:nc), P % y(-nb:nc), P % z(-nb:nc), Flux(nf), b(nc))
+  do c = 1, nc
+    U % n(c) = cu(c);  V % n(c) = cv(c);  W % n(c) = cw(c);  P % n(c) = cp(c)
+    P % x(c) = gx(c);  P % y(c) = gy(c);  P % z(c) = gz(c)
+  end do
+  do c = 1, nb
+    U % n(-c) = cu(nc+c);  V % n(-c) = cv(nc+c);  W % n(-c) = cw(nc+c);  P % n(-c) = cp(nc+c)
+    P % x(-c) = 0.0;  P % y(-c) = 0.0;  P % z(-c) = 0.0
+  end do
+  PP % n = 0.0
+  A % sav(1:nc) = csav(1:nc)
+  if(allocated(r_cell_01)) deallocate(r_cell_01, r_cell_02, r_cell_03, r_cell_04,      &
+                                      r_cell_05, r_cell_06, r_cell_07, r_cell_08, r_cell_09)
+  call Work_Mod_Allocate_Real_Cells(mg, 9)
+  if(allocated(D % row)) deallocate(D % row, D % col, D % dia, D % val)
+  allocate(D % row(size(A % row)), D % col(size(A % col)), D % dia(size(A % dia)),      &
+           D % val(size(A % val)))
+  D % row = A % row;  D % col = A % col;  D % dia = A % dia;  D % val = 0.0
+  D % nonzeros = A % nonzeros
+  D % pnt_grid => mg
+
+  call Compute_Pressure_Simple(mg)
+
+  cflux(1:nf)    = Flux(1:nf)
+  cval(1:m_nnz)  = A % val(1:m_nnz)
+  cpp(1:nc)      = PP % n(1:nc)
+  cpn(1:nc)      = P % n(1:nc)
+  TypeBC(-nb:-1) = WALL
+  tfs_ref_mesh_press_simple = 0
   end function
 
   !---------------------------------------------------------------------------!

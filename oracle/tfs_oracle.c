@@ -611,3 +611,63 @@ void or_pressure_fractional(int n_cells, int n_faces, const int *faces_c, const 
     }
   }
 }
+
+/* Sources/Process/Compute_Pressure_Simple.f90:45-180 (boundary-cell kinds as
+ * in or_correct_velocity's bc; no BUFFER faces). */
+void or_pressure_simple(int n_cells, int n_faces, const int *faces_c, const double *f,
+                        const double *s, const double *d, const double *vol, const double *sav,
+                        const double *u, const double *v, const double *w, const double *p,
+                        const double *px, const double *py, const double *pz, const double *den,
+                        const unsigned char *bc, const int *pos, const int *dia, int nnz,
+                        double *flux, double *a12, unsigned char *bnd_on, double *val, double *b) {
+  const double *sx = s, *sy = s + n_faces, *sz = s + 2 * (size_t)n_faces;
+  const double *dx = d, *dy = d + n_faces, *dz = d + 2 * (size_t)n_faces;
+  for (int k = 0; k < nnz; ++k) val[k] = 0.0; /* :46 */
+  for (int c = 0; c < n_cells; ++c) b[c] = 0.0; /* :47 */
+  for (int k = 0; k < n_faces; ++k) {
+    const int c1 = faces_c[2 * k], c2 = faces_c[2 * k + 1];
+    const size_t k1 = (size_t)(c1 - 1);
+    const size_t k2 = c2 > 0 ? (size_t)(c2 - 1) : (size_t)n_cells + (size_t)(-c2 - 1);
+    const double fs = f[k];
+    const double dens = fs * den[k1] + (1.0 - fs) * den[k2]; /* :61-62 */
+    a12[k] = 0.0;
+    bnd_on[k] = 0;
+    if (c2 > 0) { /* :82-127 */
+      const double smdpn = (sx[k] * sx[k] + sy[k] * sy[k] + sz[k] * sz[k]) /
+                           (sx[k] * dx[k] + sy[k] * dy[k] + sz[k] * dz[k]);
+      const double us = fs * u[k1] + (1.0 - fs) * u[k2];
+      const double vs = fs * v[k1] + (1.0 - fs) * v[k2];
+      const double ws = fs * w[k1] + (1.0 - fs) * w[k2];
+      const double A12 = 0.5 * dens * smdpn * (vol[k1] / sav[k1] + vol[k2] / sav[k2]);
+      val[pos[2 * k] - 1] = -A12;
+      val[pos[2 * k + 1] - 1] = -A12;
+      val[dia[c1 - 1] - 1] = val[dia[c1 - 1] - 1] + A12;
+      val[dia[c2 - 1] - 1] = val[dia[c2 - 1] - 1] + A12;
+      const double dpxi = .5 * (px[k1] + px[k2]) * dx[k];
+      const double dpyi = .5 * (py[k1] + py[k2]) * dy[k];
+      const double dpzi = .5 * (pz[k1] + pz[k2]) * dz[k];
+      flux[k] = dens * (us * sx[k] + vs * sy[k] + ws * sz[k]) + A12 * (p[k1] - p[k2]) +
+                A12 * (dpxi + dpyi + dpzi);
+      b[k1] = b[k1] - flux[k];
+      b[k2] = b[k2] + flux[k];
+      a12[k] = A12;
+    } else { /* :130-177 */
+      const int kind = bc ? bc[-c2 - 1] : 0;
+      if (kind == 4 || kind == 5 || kind == 6 || kind == 2) {
+        const size_t kv = kind == 2 ? k1 : k2; /* pressure: the inside velocity */
+        flux[k] = dens * (u[kv] * sx[k] + v[kv] * sy[k] + w[kv] * sz[k]);
+        b[k1] = b[k1] - flux[k];
+        if (kind != 4) {
+          const double smdpn = (sx[k] * sx[k] + sy[k] * sy[k] + sz[k] * sz[k]) /
+                               (sx[k] * dx[k] + sy[k] * dy[k] + sz[k] * dz[k]);
+          const double A12 = dens * smdpn * vol[k1] / sav[k1];
+          val[dia[c1 - 1] - 1] = val[dia[c1 - 1] - 1] + A12;
+          a12[k] = A12;
+          bnd_on[k] = 1;
+        }
+      } else { /* walls and symmetry planes */
+        flux[k] = 0.0;
+      }
+    }
+  }
+}

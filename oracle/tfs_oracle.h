@@ -120,6 +120,30 @@ void or_pressure_fractional(int n_cells, int n_faces, const int *faces_c, const 
                             const double *py, const double *pz, const double *p,
                             const double *den, double *flux, double *a12, double *b);
 
+/* The face loop of Compute_Pressure_Simple (Sources/Process/
+ * Compute_Pressure_Simple.f90:45-180), every material a fluid, no BUFFER
+ * faces: A % val and b cleared; per interior face the f-weighted density,
+ * SMDPN = |S|^2 / (S . d), the interpolated velocity, A12 = 0.5 DENs SMDPN
+ * (vol1/sav1 + vol2/sav2) with the off-diagonals set to -A12 and both
+ * diagonals increased, the interpolated pressure gradient and the face flux;
+ * per boundary face by the kind of its boundary cell (bc: 0 wall,
+ * 1 symmetry, 2 pressure, 4 inflow, 5 outflow, 6 convective outflow): the
+ * flux from the boundary cell's velocity (inflow, outflow, convective) or the
+ * inside cell's (pressure), with A12 = DENs SMDPN vol1/sav1 on the diagonal
+ * for outflow, convective and pressure boundaries, and no flux on walls and
+ * symmetry planes; b(c1) -= flux, b(c2) += flux in face order.  f, s and d
+ * (3 blocks each) per face; vol, sav, px/py/pz per cell; u/v/w, p, den on
+ * cells then boundary cells; pos (2 per face), dia (1-based), nnz.  Outputs
+ * flux, a12 (the coefficient each face adds to the matrix, 0 for the faces
+ * that add none), bnd_on (1 for a boundary face that adds to its diagonal),
+ * val (nnz), b (per cell). */
+void or_pressure_simple(int n_cells, int n_faces, const int *faces_c, const double *f,
+                        const double *s, const double *d, const double *vol, const double *sav,
+                        const double *u, const double *v, const double *w, const double *p,
+                        const double *px, const double *py, const double *pz, const double *den,
+                        const unsigned char *bc, const int *pos, const int *dia, int nnz,
+                        double *flux, double *a12, unsigned char *bnd_on, double *val, double *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -320,6 +320,64 @@ k_pf_faces(int n, long long nf, const int2 *__restrict__ faces, const double *__
   }
 }
 
+// Compute_Pressure_Simple.f90:50-177 for fluid materials without BUFFER
+// faces: per interior face SMDPN = |S|^2 / (S . d), A12 = 0.5 DENs SMDPN
+// (vol1/sav1 + vol2/sav2), the interpolated velocity and pressure gradient and
+// the face flux; per boundary face by the kind of its boundary cell: the flux
+// from the boundary cell's velocity (inflow, outflow, convective) or the
+// inside cell's (pressure), A12 = DENs SMDPN vol1/sav1 added to the diagonal
+// (bnd_on) for outflow, convective and pressure boundaries, no flux on walls
+// and symmetry planes.  Every expression in the reference's order.
+__global__ void __launch_bounds__(kThreads)
+k_ps_faces(int n, long long nf, const int2 *__restrict__ faces,
+           const unsigned char *__restrict__ kind, const double *__restrict__ f,
+           const double *__restrict__ sv, const double *__restrict__ dv,
+           const double *__restrict__ vol, const double *__restrict__ sav,
+           const double *__restrict__ u, const double *__restrict__ v,
+           const double *__restrict__ w, const double *__restrict__ p,
+           const double *__restrict__ px, const double *__restrict__ py,
+           const double *__restrict__ pz, const double *__restrict__ den,
+           double *__restrict__ flux, double *__restrict__ a12,
+           unsigned char *__restrict__ bnd_on) {
+  for (long long s = blockIdx.x * (long long)kThreads + threadIdx.x; s < nf;
+       s += (long long)gridDim.x * kThreads) {
+    const int2 fc = faces[s];
+    const long long k1 = fc.x - 1;
+    const long long k2 = fc.y > 0 ? (long long)(fc.y - 1) : (long long)n + (-fc.y - 1);
+    const double fs = f[s], sx = sv[s], sy = sv[nf + s], sz = sv[2 * nf + s];
+    const double dx = dv[s], dy = dv[nf + s], dz = dv[2 * nf + s];
+    const double dens = fs * den[k1] + (1.0 - fs) * den[k2];
+    double fl = 0.0, a = 0.0;
+    unsigned char on = 0;
+    if (fc.y > 0) {
+      const double smdpn = (sx * sx + sy * sy + sz * sz) / (sx * dx + sy * dy + sz * dz);
+      const double us = fs * u[k1] + (1.0 - fs) * u[k2];
+      const double vs = fs * v[k1] + (1.0 - fs) * v[k2];
+      const double ws = fs * w[k1] + (1.0 - fs) * w[k2];
+      a = 0.5 * dens * smdpn * (vol[k1] / sav[k1] + vol[k2] / sav[k2]);
+      const double dpxi = .5 * (px[k1] + px[k2]) * dx;
+      const double dpyi = .5 * (py[k1] + py[k2]) * dy;
+      const double dpzi = .5 * (pz[k1] + pz[k2]) * dz;
+      fl = dens * (us * sx + vs * sy + ws * sz) + a * (p[k1] - p[k2]) + a * (dpxi + dpyi + dpzi);
+    } else {
+      const int kd = kind[-fc.y - 1];
+      if (kd == TFS_BC_INFLOW || kd == TFS_BC_OUTFLOW || kd == TFS_BC_CONVECT ||
+          kd == TFS_BC_PRESSURE) {
+        const long long kv = kd == TFS_BC_PRESSURE ? k1 : k2;
+        fl = dens * (u[kv] * sx + v[kv] * sy + w[kv] * sz);
+        if (kd != TFS_BC_INFLOW) {
+          const double smdpn = (sx * sx + sy * sy + sz * sz) / (sx * dx + sy * dy + sz * dz);
+          a = dens * smdpn * vol[k1] / sav[k1];
+          on = 1;
+        }
+      }
+    }
+    flux[s] = fl;
+    a12[s] = a;
+    bnd_on[s] = on;
+  }
+}
+
 // b(c) = inflow - outflow over the cell's faces in face order
 // (Compute_Pressure_Fractional.f90:112-113, :126)
 __global__ void __launch_bounds__(kThreads)
@@ -387,7 +445,7 @@ extern "C" int tfs_gradient_create(int n_cells, int n_bnd_cells, long long n_fac
       // a boundary cell belongs to one face: GradP writes it from that face
       if (seen[(size_t)(-c2 - 1)]++) return TFS_ERR_INVALID;
       const int kind = bc ? bc[-c2 - 1] : TFS_BC_WALL;
-      if (kind > TFS_BC_BUFFER) return TFS_ERR_INVALID;
+      if (kind > TFS_BC_CONVECT) return TFS_ERR_INVALID;
       kinds[(size_t)(-c2 - 1)] = (unsigned char)kind;
       any_buffer = any_buffer || kind == TFS_BC_BUFFER;
       if (kind == TFS_BC_SYMMETRY) sym[(size_t)(-c2 - 1)] = 1;
@@ -559,6 +617,29 @@ extern "C" int tfs_pressure_fractional_device(tfs_gradient *S, const double *d_f
   if (S->nf > 0)
     hipLaunchKernelGGL(k_pf_faces, dim3(grid_of(S->nf)), dim3(kThreads), 0, nullptr, S->n, S->nf,
                        S->faces, d_f, d_s, S->rec, d_flux, d_a12);
+  hipLaunchKernelGGL(k_pf_rhs, dim3(grid_of(S->n)), dim3(kThreads), 0, nullptr, S->n, S->ptr,
+                     S->inc, d_flux, d_b);
+  TFS_G_HIP(hipGetLastError());
+  TFS_G_HIP(hipStreamSynchronize(nullptr));
+  return TFS_OK;
+}
+
+extern "C" int tfs_pressure_simple_device(tfs_gradient *S, const double *d_f, const double *d_s,
+                                          const double *d_vol, const double *d_sav,
+                                          const double *d_u, const double *d_v,
+                                          const double *d_w, const double *d_px,
+                                          const double *d_py, const double *d_pz,
+                                          const double *d_p, const double *d_den,
+                                          double *d_flux, double *d_a12,
+                                          unsigned char *d_bnd_on, double *d_b) {
+  if (!S || S->has_buffer || !d_vol || !d_sav || !d_u || !d_v || !d_w || !d_px || !d_py ||
+      !d_pz || !d_p || !d_den || !d_b ||
+      (S->nf > 0 && (!d_f || !d_s || !d_flux || !d_a12 || !d_bnd_on)))
+    return TFS_ERR_INVALID;
+  if (S->nf > 0)
+    hipLaunchKernelGGL(k_ps_faces, dim3(grid_of(S->nf)), dim3(kThreads), 0, nullptr, S->n, S->nf,
+                       S->faces, S->kind, d_f, d_s, S->d, d_vol, d_sav, d_u, d_v, d_w, d_p, d_px,
+                       d_py, d_pz, d_den, d_flux, d_a12, d_bnd_on);
   hipLaunchKernelGGL(k_pf_rhs, dim3(grid_of(S->n)), dim3(kThreads), 0, nullptr, S->n, S->ptr,
                      S->inc, d_flux, d_b);
   TFS_G_HIP(hipGetLastError());

@@ -52,6 +52,13 @@ enum Epi {
   EPI_NAUX0, EPI_NAUX1, EPI_NAUX2, EPI_NAUX3, EPI_NAUX4, EPI_NAUX5
 };
 
+// SpMV modes (k_spmv, k_ell64): plain y = A x; residual y = y - A x (term by
+// term, Residual.f90:22-27); dot: y = A x and partial sum of w.y
+// (Cg.f90:148-153, Cgs.f90:157-162); cgs tail: q2 = A p1, xs += alpha p1,
+// rs -= alpha q2 and partial ||rs||^2 (Cgs.f90:182-221, row-local once alpha
+// is known).
+enum SpmvMode { SPMV_PLAIN = 0, SPMV_RESID, SPMV_DOT, SPMV_CGS_TAIL };
+
 // Reductions that run whatever the solve's state (the rest return at once
 // once the solve is done).
 __device__ __forceinline__ bool epi_always(int epi) { return epi == EPI_BNRM || epi >= EPI_AUX0; }
@@ -376,6 +383,30 @@ k_add_correction(int n, double *__restrict__ x, const float *__restrict__ e,
 // entries carry the row's own column (or 0 past the last row) and value 0.
 // Arithmetic is FP64 (free in a bandwidth-bound kernel); only storage is FP32.
 constexpr int kEllWaves = 4;  // slices per workgroup (one wavefront each)
+
+// Slice walk of the ELL kernels.  With `xcd` set, the 8 XCDs (workgroups are
+// dispatched to them round-robin: XCD = blockIdx % 8) each walk one
+// contiguous eighth of the slices, so the x gathers of one XCD's L2 fall in
+// one window of the vector; otherwise workgroups take slices interleaved.
+struct SliceWalk {
+  int first, step, end;
+  __device__ SliceWalk(int nslice, bool xcd) {
+    const int wv = (int)threadIdx.x / kWave;
+    if (xcd && gridDim.x >= 8) {
+      const int x = (int)blockIdx.x % 8, k = (int)blockIdx.x / 8, g8 = (int)gridDim.x / 8;
+      const int per = (nslice + 7) / 8;
+      const int lo = x * per, hi = min(nslice, lo + per);
+      // the last gridDim % 8 workgroups join XCDs 0.. (same interleave)
+      first = lo + k * kEllWaves + wv;
+      step = g8 * kEllWaves;
+      end = k < g8 ? hi : lo;  // an extra workgroup has nothing of its own
+    } else {
+      first = (int)blockIdx.x * kEllWaves + wv;
+      step = (int)gridDim.x * kEllWaves;
+      end = nslice;
+    }
+  }
+};
 constexpr int kEllBatch = 8;  // entries per lane loaded before their gathers
 
 template <bool DOT>
@@ -383,11 +414,12 @@ __global__ void __launch_bounds__(kEllWaves * kWave)
 k_ell_diff(int n, int nslice, const long long *__restrict__ off, const int *__restrict__ col,
            const float *__restrict__ val, const float *__restrict__ rsum,
            const float *__restrict__ x, float *__restrict__ y, const KState *st,
-           double *partials) {
+           double *partials, int xcd) {
   if (st && solve_done(st)) return;
-  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
   double red = 0.0;
-  for (int s = blockIdx.x * kEllWaves + wv; s < nslice; s += gridDim.x * kEllWaves) {
+  const SliceWalk sw(nslice, xcd != 0);
+  for (int s = sw.first; s < sw.end; s += sw.step) {
     const int su = __builtin_amdgcn_readfirstlane(s);
     const long long e0 = off[su];
     const int w = (int)((off[su + 1] - e0) / kWave);
@@ -494,16 +526,77 @@ k_colour_sweep(int s0, int s1, const int *__restrict__ row0, const long long *__
   if (BWD || LAST) store_partial<kEllWaves * kWave>(rho, partials);
 }
 
-// The sliced-ELL values (FP32) from the FP64 caller values: src[e] is the
-// caller entry of slot e, or -1 for padding.
+// Sliced-ELL values (FP32 or FP64) from the FP64 caller values: src[e] is
+// the caller entry of slot e, or -1 for padding.
+template <typename T>
 __global__ void __launch_bounds__(kRedThreads)
 k_ell_values(long long len, const int *__restrict__ src, const double *__restrict__ val,
-             float *__restrict__ eval) {
+             T *__restrict__ eval) {
   for (long long e = (long long)blockIdx.x * kRedThreads + threadIdx.x; e < len;
        e += (long long)gridDim.x * kRedThreads) {
     const int k = src[e];
-    eval[e] = k >= 0 ? (float)val[k] : 0.0f;
+    eval[e] = k >= 0 ? (T)val[k] : T(0);
   }
+}
+
+// FP64 SpMV over a sliced ELL of all the entries (diagonal included), each
+// row's entries in the caller's order (padding: column -1, skipped, so no
+// term is added that the row does not have): one lane per row sums its
+// products left to right, as k_spmv and the reference do (Cg.f90:128-134,
+// Residual.f90:22-27) -- bitwise the same results -- but with every col/val
+// load of a wave one contiguous 256-byte run and no row pointers or LDS
+// staging.  Modes as k_spmv; the dot products are summed per workgroup.
+template <int MODE>
+__global__ void __launch_bounds__(kEllWaves * kWave)
+k_ell64(int n, int nslice, const long long *__restrict__ off, const int *__restrict__ col,
+        const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y,
+        const double *__restrict__ w, double *__restrict__ xs, double *__restrict__ rs,
+        const KState *st, double *partials, int xcd) {
+  if (st && solve_done(st)) return;
+  const int lane = threadIdx.x % kWave;
+  const double alpha = MODE == SPMV_CGS_TAIL ? st->alpha : 0.0;
+  double red = 0.0;
+  const SliceWalk sw(nslice, xcd != 0);
+  for (int s = sw.first; s < sw.end; s += sw.step) {
+    const int su = __builtin_amdgcn_readfirstlane(s);
+    const long long e0 = off[su];
+    const int wd = (int)((off[su + 1] - e0) / kWave);
+    const int i = su * kWave + lane;
+    const bool live = i < n;
+    double acc = MODE == SPMV_RESID && live ? y[i] : 0.0;
+    const long long base = e0 + lane;
+    for (int j0 = 0; j0 < wd; j0 += kEllBatch) {
+      int c[kEllBatch];
+      double a[kEllBatch], xv[kEllBatch];
+#pragma unroll
+      for (int k = 0; k < kEllBatch; ++k)
+        if (j0 + k < wd) {
+          c[k] = col[base + (long long)(j0 + k) * kWave];
+          a[k] = val[base + (long long)(j0 + k) * kWave];
+        }
+#pragma unroll
+      for (int k = 0; k < kEllBatch; ++k)
+        if (j0 + k < wd) xv[k] = c[k] >= 0 ? x[c[k]] : 0.0;
+#pragma unroll
+      for (int k = 0; k < kEllBatch; ++k)
+        if (j0 + k < wd && c[k] >= 0) {
+          const double pr = a[k] * xv[k];
+          acc = MODE == SPMV_RESID ? acc - pr : acc + pr;
+        }
+    }
+    if (live) {
+      y[i] = acc;
+      if (MODE == SPMV_DOT) red += w[i] * acc;
+      if (MODE == SPMV_CGS_TAIL) {
+        xs[i] = xs[i] + alpha * x[i];
+        const double ri = rs[i] - alpha * acc;
+        rs[i] = ri;
+        red += ri * ri;
+      }
+    }
+  }
+  if (MODE == SPMV_DOT || MODE == SPMV_CGS_TAIL)
+    store_partial<kEllWaves * kWave>(red, partials);
 }
 
 // Row sums of the operator, summed in FP64 over the caller's entries of each
@@ -703,7 +796,6 @@ k_finish_seg(const int *__restrict__ tile_first, const double *__restrict__ part
 // Residual.f90:22-27); dot: y = A x and partial sum of w.y (Cg.f90:148-153,
 // Cgs.f90:157-162); cgs tail: q2 = A p1, xs += alpha p1, rs -= alpha q2 and
 // partial ||rs||^2 (Cgs.f90:182-221, all row-local once alpha is known).
-enum SpmvMode { SPMV_PLAIN = 0, SPMV_RESID, SPMV_DOT, SPMV_CGS_TAIL };
 
 // WAVES: wavefronts per workgroup (2, 4 or 8; the row blocks, at most 64 rows
 // each, are a runtime layout) -- the two shapes the SpMV tuner picks.

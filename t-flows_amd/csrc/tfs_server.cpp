@@ -62,13 +62,15 @@ struct Request {
 };
 
 // Device bytes of a resident case before its first solve (then replaced by
-// the exact figure): per entry, col + source index in the S layout, the
-// caller-order and S-order FP64 values, and col + value in the L or U
-// triangle (off-diagonal entries); per row, row pointers, maps, diagonal
-// copies, the factor in two orders, two staging vectors and up to 12 work
-// vectors (Cgs); plus the fixed per-matrix buffers.
+// the exact figure), an upper bound: per entry, col + source index in the S
+// layout, the caller-order and S-order FP64 values, the FP64 sliced ELL (col,
+// source index, value; up to 1.25 nnz slots with padding), and col + value in
+// the L or U triangle (off-diagonal entries); per row, row pointers, maps,
+// diagonal copies, the factor in two orders, two staging vectors and up to 12
+// work vectors (Cgs); plus the fixed per-matrix buffers (state, the partial
+// sums of the reductions, flags).
 long long estimate_bytes(int n, long long nnz) {
-  return 24 * nnz + 12 * (nnz - n) + 200 * (long long)n + (32 << 10);
+  return 24 * nnz + 20 * nnz + 12 * (nnz - n) + 200 * (long long)n + (320 << 10);
 }
 
 }  // namespace

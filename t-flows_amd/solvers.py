@@ -735,6 +735,7 @@ class Gradient:
             lib.tfs_correct_velocity_device.argtypes = [_VP] + [_VP] * 14 + \
                 [ctypes.c_double, ctypes.c_double, _VP, _VP, _DP, _DP, _DP]
             lib.tfs_pressure_fractional_device.argtypes = [_VP] * 16
+            lib.tfs_pressure_simple_device.argtypes = [_VP] * 17
             lib._tfs_grad_typed = True
         self._lib = lib
         self.n, self.nb = int(n_cells), int(n_bnd_cells)
@@ -826,6 +827,30 @@ class Gradient:
             _dptr(px, n, "px"), _dptr(py, n, "py"), _dptr(pz, n, "pz"), _dptr(p, na, "p"),
             _dptr(den, na, "den"), _dptr(flux, nf, "flux"), _dptr(a12, nf, "a12"),
             _dptr(b, n, "b")), "tfs_pressure_fractional_device")
+
+    def pressure_simple(self, f, s, vol, sav, u, v, w, px, py, pz, p, den, flux, a12, bnd_on, b):
+        """Compute_Pressure_Simple's face loop on the GPU: flux, a12 and bnd_on
+        (uint8) per face, b per cell (GPU tensors; s holds sx, sy, sz blocks);
+        the matrix then comes from Assembly.assemble(a12, bnd_on=bnd_on)."""
+        import torch
+        n, na, nf = self.n, self.n + self.nb, self.nf
+        for t, what in ((f, "f"), (flux, "flux"), (a12, "a12"), (bnd_on, "bnd_on")):
+            _check_count(t, nf, what)
+        _check_count(s, 3 * nf, "s (sx, sy, sz blocks)")
+        for t, m, what in ((u, na, "u"), (v, na, "v"), (w, na, "w"), (p, na, "p"),
+                           (den, na, "den"), (vol, n, "vol"), (sav, n, "sav"), (px, n, "px"),
+                           (py, n, "py"), (pz, n, "pz"), (b, n, "b")):
+            _check_count(t, m, what, exact=False)
+        if not (_is_device_tensor(bnd_on) and bnd_on.dtype == torch.uint8
+                and bnd_on.is_contiguous()):
+            raise ValueError("bnd_on: contiguous uint8 GPU tensor expected")
+        _sync()
+        _check(self._lib.tfs_pressure_simple_device(
+            self._h, _dptr(f, nf, "f"), _dptr(s, 3 * nf, "s"), _dptr(vol, n, "vol"),
+            _dptr(sav, n, "sav"), _dptr(u, na, "u"), _dptr(v, na, "v"), _dptr(w, na, "w"),
+            _dptr(px, n, "px"), _dptr(py, n, "py"), _dptr(pz, n, "pz"), _dptr(p, na, "p"),
+            _dptr(den, na, "den"), _dptr(flux, nf, "flux"), _dptr(a12, nf, "a12"),
+            bnd_on.data_ptr(), _dptr(b, n, "b")), "tfs_pressure_simple_device")
 
     def close(self):
         if getattr(self, "_h", None):

@@ -25,6 +25,10 @@ non-orthogonal backstep).  This script then keeps, per mesh:
   * in mesh_reference.json, "pressure_fractional": the SHA-256 of the
     reference's Compute_Pressure_Fractional outputs for the same inputs (pp
     as the old pressure): face fluxes, A % val, PP % n and P % n;
+  * in mesh_reference.json, "pressure_simple": the same for the reference's
+    Compute_Pressure_Simple with the boundary kinds of
+    tests/mesh_fixtures.simple_kinds (walls, symmetry, pressure, inflow,
+    outflow and convective boundaries; Bicg prec 2 to 1e-9, cap 40);
   * in <name>_cg_p2.npz: the reference's Cg (prec 2) on that pressure matrix
     with cell 1 pinned (a_11 doubled) and the seeded right-hand side of
     tflows_amd.mesh.mesh_case, to tol 1e-10 ||b||: x, niter, ini_res, fin_res.
@@ -50,7 +54,7 @@ from conftest import load_package  # noqa: E402
 load_package()
 from tflows_amd import mesh  # noqa: E402
 from oracle.oracle import Reference  # noqa: E402
-from mesh_fixtures import CV_DEN, CV_VISC, SEED, cv_inputs, grad_inputs  # noqa: E402
+from mesh_fixtures import CV_DEN, CV_VISC, SEED, cv_inputs, grad_inputs, simple_kinds  # noqa: E402,E501
 
 OUT = HERE / "mesh"
 GEN = ROOT / "oracle" / "_ref" / "gen"
@@ -98,6 +102,11 @@ def main():
                                  cv["pp"], cv["px"], cv["py"], cv["pz"], cv["sav"], CV_DEN, 2,
                                  1e-9, 0.8)
         entry["pressure_fractional"] = {k: sha(pf[k]) for k in ("flux", "val", "pp", "p")}
+        # Compute_Pressure_Simple (the same fields, mixed boundary kinds)
+        ps = ref.mesh_press_simple(nc, r["n_faces"], r["nnz"], cv["u"], cv["v"], cv["w"],
+                                   cv["pp"], cv["px"], cv["py"], cv["pz"], cv["sav"], CV_DEN,
+                                   simple_kinds(nb), 2, 1e-9, 0.8)
+        entry["pressure_simple"] = {k: sha(ps[k]) for k in ("flux", "val", "pp", "p")}
         meta["meshes"][name] = entry
 
         # the pinned pressure system, solved by the reference

@@ -61,3 +61,14 @@ def cv_inputs(n_cells, n_bnd, n_faces):
     flux = stream(0x43560030, n_faces)
     _, bc = grad_inputs(n_cells, n_bnd, True)
     return dict(u=u, v=v, w=w, pp=pp, px=px, py=py, pz=pz, sav=sav, flux=flux, bc=bc)
+
+
+SIMPLE_KINDS = (0, 1, 2, 4, 5, 6)  # wall, symmetry, pressure, inflow, outflow, convective
+
+
+def simple_kinds(n_bnd):
+    """Seeded boundary kinds of the SIMPLE face-loop checks, drawn from
+    SIMPLE_KINDS (every kind but buffer)."""
+    lib = mesh._lib()
+    return np.array([SIMPLE_KINDS[int(6.0 * lib.tfs_rng_u01(SEED, 0x53494D50, i))]
+                     for i in range(n_bnd)], np.int8)

@@ -182,3 +182,43 @@ def test_pressure_fractional_bitwise(torch, S, oracle, meshes, name):
     assert np.array_equal(val, oracle.assemble(nc, cns["faces_c"], pos, dia, len(col), a12_or))
     want = META["meshes"][name]["pressure_fractional"]
     assert sha(flux.cpu().numpy()) == want["flux"] and sha(val) == want["val"]
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_pressure_simple_bitwise(torch, S, oracle, meshes, name):
+    """Compute_Pressure_Simple's face loop on the GPU (fluxes, A12, which
+    boundary faces carry a coefficient, the right-hand side), with mixed
+    boundary kinds, and the matrix assembled from its A12 on the GPU, against
+    the oracle bitwise and the reference's digests."""
+    from mesh_fixtures import CV_DEN, cv_inputs, simple_kinds
+    cns, geo, d = meshes[name]
+    nc, nb, nf = cns["n_cells"], cns["n_bnd_cells"], cns["n_faces"]
+    print(f"pressure_simple {name} n={nc}", flush=True)
+    cv = cv_inputs(nc, nb, nf)
+    bc = simple_kinds(nb)
+    den = np.full(nc + nb, CV_DEN)
+    row, col, dia, pos = mesh.topology(nc, cns["faces_c"], base=1, with_pos=True)
+    want = oracle.pressure_simple(nc, cns["faces_c"], geo["f"], geo["s"], d, geo["vol"],
+                                  cv["sav"], cv["u"], cv["v"], cv["w"], cv["pp"], cv["px"],
+                                  cv["py"], cv["pz"], den, bc, pos, dia, len(col))
+    G = S.Gradient(nc, nb, cns["faces_c"], d, bc=bc)
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a, np.float64).ravel()).cuda()
+    flux = torch.full((nf,), np.nan, dtype=torch.float64, device="cuda")
+    a12 = torch.full((nf,), np.nan, dtype=torch.float64, device="cuda")
+    bnd_on = torch.full((nf,), 7, dtype=torch.uint8, device="cuda")
+    b = torch.full((nc,), np.nan, dtype=torch.float64, device="cuda")
+    G.pressure_simple(dev(geo["f"]), dev(geo["s"]), dev(geo["vol"]), dev(cv["sav"]),
+                      dev(cv["u"]), dev(cv["v"]), dev(cv["w"]), dev(cv["px"]), dev(cv["py"]),
+                      dev(cv["pz"]), dev(cv["pp"]), dev(den), flux, a12, bnd_on, b)
+    assert np.array_equal(flux.cpu().numpy(), want["flux"])
+    assert np.array_equal(a12.cpu().numpy(), want["a12"])
+    assert np.array_equal(bnd_on.cpu().numpy(), want["bnd_on"])
+    assert np.array_equal(b.cpu().numpy(), want["b"])
+    A = S.Matrix(row - 1, col - 1, dia - 1)
+    S.Assembly(A, cns["faces_c"], pos, dia, base=1).assemble(a12, bnd_on=bnd_on)
+    val = A.get_values()
+    assert np.array_equal(val, want["val"])
+    ref = META["meshes"][name]["pressure_simple"]
+    assert sha(flux.cpu().numpy()) == ref["flux"] and sha(val) == ref["val"]
