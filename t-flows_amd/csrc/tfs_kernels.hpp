@@ -545,19 +545,13 @@ k_ell_values(long long len, const int *__restrict__ src, const double *__restric
 // products left to right, as k_spmv and the reference do (Cg.f90:128-134,
 // Residual.f90:22-27) -- bitwise the same results -- but with every col/val
 // load of a wave one contiguous 256-byte run and no row pointers or LDS
-// staging.  Stencil slices (sten[slice] >= 0) carry no column indices: slot
-// j of row i is column i + stab[sten + j] (the slice's distinct offsets,
-// ascending, so a row's entries keep their order) where bit j of mask[i] is
-// set, padding where it is not; their x loads are contiguous too.  Modes as
-// k_spmv; the dot products are summed per workgroup.
+// staging.  Modes as k_spmv; the dot products are summed per workgroup.
 template <int MODE>
 __global__ void __launch_bounds__(kEllWaves * kWave)
 k_ell64(int n, int nslice, const long long *__restrict__ off, const int *__restrict__ col,
-        const int *__restrict__ sten, const int *__restrict__ stab,
-        const unsigned char *__restrict__ mask, const double *__restrict__ val,
-        const double *__restrict__ x, double *__restrict__ y, const double *__restrict__ w,
-        double *__restrict__ xs, double *__restrict__ rs, const KState *st, double *partials,
-        int xcd) {
+        const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y,
+        const double *__restrict__ w, double *__restrict__ xs, double *__restrict__ rs,
+        const KState *st, double *partials, int xcd) {
   if (st && solve_done(st)) return;
   const int lane = threadIdx.x % kWave;
   const double alpha = MODE == SPMV_CGS_TAIL ? st->alpha : 0.0;
@@ -567,30 +561,19 @@ k_ell64(int n, int nslice, const long long *__restrict__ off, const int *__restr
     const int su = __builtin_amdgcn_readfirstlane(s);
     const long long e0 = off[su];
     const int wd = (int)((off[su + 1] - e0) / kWave);
-    const int sb = sten ? sten[su] : -1;  // uniform: the slice's stencil, or -1
     const int i = su * kWave + lane;
     const bool live = i < n;
-    const unsigned int m = sb >= 0 && live ? mask[i] : 0u;
     double acc = MODE == SPMV_RESID && live ? y[i] : 0.0;
     const long long base = e0 + lane;
     for (int j0 = 0; j0 < wd; j0 += kEllBatch) {
       int c[kEllBatch];
       double a[kEllBatch], xv[kEllBatch];
-      if (sb >= 0) {
 #pragma unroll
-        for (int k = 0; k < kEllBatch; ++k)
-          if (j0 + k < wd) {
-            c[k] = (m >> (j0 + k)) & 1u ? i + stab[sb + j0 + k] : -1;
-            a[k] = val[base + (long long)(j0 + k) * kWave];
-          }
-      } else {
-#pragma unroll
-        for (int k = 0; k < kEllBatch; ++k)
-          if (j0 + k < wd) {
-            c[k] = col[base + (long long)(j0 + k) * kWave];
-            a[k] = val[base + (long long)(j0 + k) * kWave];
-          }
-      }
+      for (int k = 0; k < kEllBatch; ++k)
+        if (j0 + k < wd) {
+          c[k] = col[base + (long long)(j0 + k) * kWave];
+          a[k] = val[base + (long long)(j0 + k) * kWave];
+        }
 #pragma unroll
       for (int k = 0; k < kEllBatch; ++k)
         if (j0 + k < wd) xv[k] = c[k] >= 0 ? x[c[k]] : 0.0;

@@ -46,11 +46,7 @@ int cm_bfs(int start, int stamp, const std::vector<int> &row, const std::vector<
 
 // Reverse Cuthill-McKee (Cuthill & McKee 1969; George 1971), starting each
 // connected component from a pseudo-peripheral node (George & Liu 1979).
-// With keep_stencil, a component whose rows are a contiguous caller range in
-// stencil shape (stencil_rows) keeps its own order instead; the decision is
-// per component, so a system keeps the same S order inside a block-diagonal
-// batch as alone.
-std::vector<int> rcm(int n, const int *row, const int *col, bool keep_stencil) {
+std::vector<int> rcm(int n, const int *row, const int *col) {
   // symmetric adjacency without the diagonal
   std::vector<int> cnt(n + 1, 0);
   for (int i = 0; i < n; ++i)
@@ -82,12 +78,11 @@ std::vector<int> rcm(int n, const int *row, const int *col, bool keep_stencil) {
   for (int i = 0; i < n; ++i) deg[i] = arow[i + 1] - arow[i];
 
   // mark: 0 = unvisited; 1 = placed in `order`; >= 2 = probe stamps
-  std::vector<int> mark(n, 0), order, tmp, last, comp_start;
+  std::vector<int> mark(n, 0), order, tmp, last;
   order.reserve(n);
   int stamp = 2;
   for (int s0 = 0; s0 < n; ++s0) {
     if (mark[s0] == 1) continue;
-    comp_start.push_back((int)order.size());
     int r = s0, ecc = -1;
     for (int it = 0; it < 4; ++it) {
       tmp.clear();
@@ -103,26 +98,8 @@ std::vector<int> rcm(int n, const int *row, const int *col, bool keep_stencil) {
     }
     cm_bfs(r, 1, arow, a2, deg, mark, order, nullptr);
   }
-  comp_start.push_back(n);
-  // components last-found first, each reversed (or kept in stencil shape):
-  // the reverse of the whole Cuthill-McKee order where nothing is kept
-  std::vector<int> out;
-  out.reserve(n);
-  for (int c = (int)comp_start.size() - 2; c >= 0; --c) {
-    const int a = comp_start[c], b = comp_start[c + 1];
-    int lo = n, hi = -1;
-    for (int k = a; k < b; ++k) {
-      lo = std::min(lo, order[k]);
-      hi = std::max(hi, order[k]);
-    }
-    if (keep_stencil && hi - lo + 1 == b - a &&
-        stencil_rows(lo, b - a, row, col, kStencilRun, kStencilWidth, kStencilShare)) {
-      for (int i = lo; i <= hi; ++i) out.push_back(i);
-    } else {
-      for (int k = b - 1; k >= a; --k) out.push_back(order[k]);
-    }
-  }
-  return out;  // out[k] = caller row placed at position k
+  std::reverse(order.begin(), order.end());
+  return order;  // order[k] = caller row placed at position k
 }
 
 // With `mirror` (the lower layout, for the upper triangle) the levels are
@@ -335,45 +312,16 @@ int block_colour_order(int n, const int *row, const int *col, int block, std::ve
   return ncol;
 }
 
-bool stencil_rows(int first, int count, const int *row, const int *col, int rows, int width,
-                  double share) {
-  if (count <= 0 || rows <= 0) return false;
-  const int runs = (count + rows - 1) / rows, end = first + count;
-  int good = 0;
-  std::vector<int> offs;
-  for (int r = 0; r < runs; ++r) {
-    offs.clear();
-    bool ok = true;
-    for (int i = first + r * rows; i < std::min(end, first + (r + 1) * rows) && ok; ++i)
-      for (int e = row[i]; e < row[i + 1]; ++e) {
-        const int o = col[e] - i;
-        if (std::find(offs.begin(), offs.end(), o) == offs.end()) {
-          offs.push_back(o);
-          if ((int)offs.size() > width) {
-            ok = false;
-            break;
-          }
-        }
-      }
-    good += ok;
-  }
-  return good >= share * runs;
-}
-
 void build_layout(int n, const int *row, const int *col, const int *dia, int spmv_rows,
                   int spmv_cap, int small_level, Layout &out, bool identity_s) {
   out.n = n;
   out.nnz = n > 0 ? row[n] : 0;
-  // components in stencil shape keep the caller's order (their SpMV slices
-  // need no column indices, and their gathers are as local as RCM's)
   if (identity_s) {
     out.s_orig.resize(n);
     for (int i = 0; i < n; ++i) out.s_orig[i] = i;
   } else {
-    out.s_orig = rcm(n, row, col, true);
+    out.s_orig = rcm(n, row, col);
   }
-  out.s_is_caller = true;
-  for (int i = 0; i < n && out.s_is_caller; ++i) out.s_is_caller = out.s_orig[i] == i;
   out.s_of_orig.assign(n, 0);
   for (int k = 0; k < n; ++k) out.s_of_orig[out.s_orig[k]] = k;
   out.s_row.assign(n + 1, 0);

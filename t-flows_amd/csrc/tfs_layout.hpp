@@ -51,7 +51,6 @@ struct Layout {
   std::vector<int> u_to_l;  // L position of the row at U position q
   std::vector<int> s_to_u;  // U position of the row at S position k
   int bandwidth = 0;        // max |S(i) - S(j)| over entries (diagnostic)
-  bool s_is_caller = false; // S is the caller's order (kept: it is stencil-shaped)
 };
 
 // Greedy colouring of the (symmetrised) pattern, vertices visited in index
@@ -83,18 +82,6 @@ int block_colour_order(int n, const int *row, const int *col, int block, std::ve
 // row/col/dia are 0-based, validated (diagonal present, columns ascending).
 // identity_s keeps the caller's order as the S order (for a matrix whose rows
 // are already in the order its kernels want, e.g. a block multi-colour copy).
-// Whether rows [first, first + count) of the caller's order are
-// stencil-shaped: in at least `share` of their runs of `rows` consecutive
-// rows (from `first`), the distinct column offsets (col - row) number at most
-// `width` (a lexicographically numbered structured grid: every run of a
-// 7-point stencil has at most 7).  A connected component in such an order
-// keeps it as its S order: its SpMV needs no per-entry column index.
-bool stencil_rows(int first, int count, const int *row, const int *col, int rows, int width,
-                  double share);
-constexpr int kStencilRun = 64;       // rows per SpMV slice (a wavefront)
-constexpr int kStencilWidth = 8;      // distinct offsets of a stencil slice (mask bits)
-constexpr double kStencilShare = 0.9; // share of stencil slices for keeping an order
-
 void build_layout(int n, const int *row, const int *col, const int *dia, int spmv_rows,
                   int spmv_cap, int small_level, Layout &out, bool identity_s = false);
 

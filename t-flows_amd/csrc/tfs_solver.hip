@@ -16,7 +16,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <map>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -206,12 +205,6 @@ struct tfs_matrix {
   long long *e64_off = nullptr;
   int *e64_col = nullptr, *e64_src = nullptr;
   double *e64_val = nullptr;
-  // stencil slices: per slice the start of its offsets in e64_stab (-1: the
-  // slice has column indices), per row the mask of the slots it has
-  int *e64_sten = nullptr, *e64_stab = nullptr;
-  unsigned char *e64_mask = nullptr;
-  int e64_stencil_slices = 0;
-  int e64_xcd = 0;  // walk slices XCD by XCD (stencil-shaped S order)
   // Point-coloured incomplete-Cholesky sweeps of the inner copy: per colour,
   // sliced ELL of the strictly-lower (L) and strictly-upper (U) entries.
   int csw_colours = 0;
@@ -471,9 +464,8 @@ void launch_spmv(tfs_matrix *A, const T *x, T *y, const T *w, T *xs, T *rs, cons
   if constexpr (sizeof(T) == 8) {
     if (A->e64_len > 0) {
       hipLaunchKernelGGL(k_ell64<MODE>, dim3(ell64_grid(A)), dim3(kEllWaves * kWave), 0,
-                         cur_stream(), A->n, A->e64_nslice, A->e64_off, A->e64_col,
-                         A->e64_sten, A->e64_stab, A->e64_mask, A->e64_val, x, y, w, xs, rs, st,
-                         A->partials, A->e64_xcd);
+                         cur_stream(), A->n, A->e64_nslice, A->e64_off, A->e64_col, A->e64_val,
+                         x, y, w, xs, rs, st, A->partials, kEll64Xcd);
       tm_end(TFS_KC_SPMV);
       return;
     }
@@ -1460,66 +1452,24 @@ int upload_sweep(tfs_matrix *A, const SweepLayout &S, DevSweep &D) {
 
 // The FP64 sliced ELL of all entries (k_ell64) from the host layout: 64 S
 // rows per slice, each row's entries in the caller's order, column -1 as
-// padding.  A slice whose rows use at most kStencilWidth distinct column
-// offsets (col - row; every slice of a lexicographically numbered grid) and
-// list their entries in ascending S position is stored as a stencil slice:
-// one slot per distinct offset, ascending (so each row's entry order), a bit
-// per slot in the row's mask, no column indices.  Not built when the padding would exceed a quarter of nnz (rows
+// padding.  Not built when the padding would exceed a quarter of nnz (rows
 // of very uneven length); the CSR kernel serves then.
 int build_ell64(tfs_matrix *A, const Layout &Ly) {
   const int n = A->n, ns = (n + kWave - 1) / kWave;
   std::vector<long long> off((size_t)ns + 1, 0);
-  std::vector<int> sten((size_t)std::max(ns, 1), -1), stab;
-  std::vector<std::vector<int>> slice_offs((size_t)ns);
-  std::map<std::vector<int>, int> seen;  // stencils already in stab
-  int nsten = 0;
   for (int sl = 0; sl < ns; ++sl) {
     int w = 0;
-    std::vector<int> offs;
-    bool stencil = true;
-    for (int i = sl * kWave; i < std::min(n, (sl + 1) * kWave); ++i) {
+    for (int i = sl * kWave; i < std::min(n, (sl + 1) * kWave); ++i)
       w = std::max(w, Ly.s_row[i + 1] - Ly.s_row[i]);
-      for (int e = Ly.s_row[i]; e < Ly.s_row[i + 1] && stencil; ++e) {
-        // the slots ascend by offset: a row's entries must ascend in S
-        // position too, or the sum would change order
-        if (e > Ly.s_row[i] && Ly.s_col[e] <= Ly.s_col[e - 1]) {
-          stencil = false;
-          break;
-        }
-        const int o = Ly.s_col[e] - i;
-        if (std::find(offs.begin(), offs.end(), o) == offs.end()) {
-          offs.push_back(o);
-          stencil = (int)offs.size() <= kStencilWidth;
-        }
-      }
-    }
-    if (stencil) {
-      std::sort(offs.begin(), offs.end());
-      auto it = seen.find(offs);
-      if (it == seen.end()) {
-        it = seen.emplace(offs, (int)stab.size()).first;
-        stab.insert(stab.end(), offs.begin(), offs.end());
-      }
-      sten[(size_t)sl] = it->second;
-      w = (int)offs.size();
-      slice_offs[(size_t)sl] = std::move(offs);
-      ++nsten;
-    }
     off[(size_t)sl + 1] = off[(size_t)sl] + (long long)w * kWave;
   }
   const long long len = off[(size_t)ns];
   if (n == 0 || (double)len > 1.25 * (double)A->nnz) return 0;
   std::vector<int> ecol((size_t)len, -1), esrc((size_t)len, -1);
-  std::vector<unsigned char> mask((size_t)n, 0);
   for (int i = 0; i < n; ++i) {
     const int sl = i / kWave, lane = i % kWave;
-    const std::vector<int> &offs = slice_offs[(size_t)sl];
     int j = 0;
     for (int e = Ly.s_row[i]; e < Ly.s_row[i + 1]; ++e, ++j) {
-      if (sten[(size_t)sl] >= 0) {  // the slot of this offset
-        j = (int)(std::find(offs.begin(), offs.end(), Ly.s_col[e] - i) - offs.begin());
-        mask[(size_t)i] |= (unsigned char)(1u << j);
-      }
       const size_t q = (size_t)(off[(size_t)sl] + (long long)j * kWave + lane);
       ecol[q] = Ly.s_col[e];
       esrc[q] = Ly.s_src[e];
@@ -1529,18 +1479,8 @@ int build_ell64(tfs_matrix *A, const Layout &Ly) {
   TFS_TRY(upload(A, &A->e64_col, ecol));
   TFS_TRY(upload(A, &A->e64_src, esrc));
   TFS_TRY(dev_alloc(A, (void **)&A->e64_val, sizeof(double) * (size_t)len));
-  if (nsten > 0) {
-    TFS_TRY(upload(A, &A->e64_sten, sten));
-    TFS_TRY(upload(A, &A->e64_stab, stab));
-    TFS_TRY(dev_alloc(A, (void **)&A->e64_mask, (size_t)n));
-    TFS_HIP(hipMemcpy(A->e64_mask, mask.data(), (size_t)n, hipMemcpyHostToDevice));
-  }
   A->e64_nslice = ns;
   A->e64_len = len;
-  A->e64_stencil_slices = nsten;
-  // a stencil-shaped (caller) order is walked XCD by XCD: each XCD's L2 then
-  // holds the planes its slices gather from
-  A->e64_xcd = Ly.s_is_caller ? 1 : kEll64Xcd;
   return 0;
 }
 
