@@ -177,6 +177,13 @@ def measured_traffic(name):
     return None
 
 
+def spmv_kernel_name(A):
+    """The FP64 SpMV kernel the matrix runs: the sliced ELL or the CSR kernel."""
+    if A.info()["ell64_slices"] > 0:
+        return "k_ell64 (FP64 sliced-ELL SpMV, reference row order)"
+    return "k_spmv (FP64 CSR SpMV)"
+
+
 def spmv_roofline(A, x, y, n, nnz, reps=50, inloop_ms=None, traffic_profile=None):
     """The SpMV against HBM.  With inloop_ms (the average launch timed with
     HIP events inside the timed solves) that is the figure; otherwise each
@@ -189,7 +196,7 @@ def spmv_roofline(A, x, y, n, nnz, reps=50, inloop_ms=None, traffic_profile=None
     ms = inloop_ms if inloop_ms is not None else cold
     achieved = algo / (ms * 1e-3) / 1e9
     return {
-        "kernel": "k_spmv (FP64 CSR SpMV)", "bound": "hbm", "achieved": achieved,
+        "kernel": spmv_kernel_name(A), "bound": "hbm", "achieved": achieved,
         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
         "traffic": measured_traffic(traffic_profile) if traffic_profile else None,
         "algorithmic_bytes": algo, "launch_us": ms * 1e3,
@@ -436,8 +443,14 @@ def mixed_vs_fp64(args):
             "algorithmic_bytes_formula": per, "launch_us": 1e3 * dom_launch_ms,
             "launches_timed": dom_n.value, "timing": "HIP events around every launch of the "
             "kernel class in the timed solves, on the matrix's stream"}
+    # the FP64 drop-in's SpMV and sweeps, each launch timed with HIP events
+    # inside its solves (the breakdown above)
     y = torch.empty_like(b)
-    roof64 = spmv_roofline(A, x, y, c.n, c.nnz)
+    spmv64_ms = bd64["spmv"]["ms_per_solve"] / max(bd64["spmv"]["launches_per_solve"], 1)
+    roof64 = spmv_roofline(A, x, y, c.n, c.nnz, inloop_ms=spmv64_ms,
+                           traffic_profile="k_ell64_poisson256_counters")
+    sweep64_us = 1e3 * bd64["sweeps"]["ms_per_solve"] / max(bd64["sweeps"]["launches_per_solve"], 1)
+    sroof64 = sweep_roofline(A, c, sweep64_us, traffic_profile="k_sweep_poisson256_counters")
     cpu = None
     if not args.no_cpu:
         log(f"timing the CPU baseline ({args.cpu_reps} iteration samples after a warm-up)...")
@@ -447,12 +460,14 @@ def mixed_vs_fp64(args):
     out = line(args, args.steps / e32, e32,
                f"Poisson {kw['nx']}^3 stretched to aspect 100 (tanh beta "
                f"{kw['beta']:.4f}), zero-mean RHS, cell 1 pinned: FP32-inner PCG (DIC, "
-               f"block multi-colour inner order) with reliable FP64 updates to rel 1e-10; "
+               f"{'point-coloured' if A.info()['inner_block'] <= 1 else 'block multi-colour'}"
+               f" inner order) with reliable FP64 updates to rel 1e-10; "
                f"plain FP64 Cg + DIC beside it", {
                    "latency_ms": {"p50": float(np.percentile(lat, 50)),
                                   "p99": float(np.percentile(lat, 99))},
                    "roofline": roof,
                    "roofline_fp64_spmv": roof64,
+                   "roofline_fp64_sweeps": sroof64,
                    "speedup_vs_fp64": speedup, "rel_l2_vs_fp64": err,
                    "fp64": {"solves_per_s": 1e3 / ms64, "ms_per_solve": ms64,
                             "repetitions": args.fp64_reps, "niter": res64[0],

@@ -117,6 +117,7 @@ typedef struct {
   int spmv_waves;       /*             wavefronts per workgroup              */
   int inner_block;      /* rows per block of the inner order (1 = points)    */
   int inner_levels;     /* depth of the inner forward sweep (0 until built)  */
+  int ell64_slices;     /* FP64 SpMV: sliced-ELL slices (0: CSR kernel)      */
 } tfs_matrix_info;
 int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info);
 

@@ -1725,6 +1725,7 @@ extern "C" int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info) {
   info->spmv_waves = A->spmv_waves;
   info->inner_block = A->inner_block;
   info->inner_levels = A->inner ? A->inner->L.nlev : 0;
+  info->ell64_slices = A->e64_len > 0 ? A->e64_nslice : 0;
   return TFS_OK;
 }
 

@@ -62,7 +62,7 @@ class _Info(ctypes.Structure):
                 ("row_blocks", ctypes.c_int), ("device_bytes", ctypes.c_longlong),
                 ("inner_colours", ctypes.c_int), ("spmv_rows", ctypes.c_int),
                 ("spmv_waves", ctypes.c_int), ("inner_block", ctypes.c_int),
-                ("inner_levels", ctypes.c_int)]
+                ("inner_levels", ctypes.c_int), ("ell64_slices", ctypes.c_int)]
 
 
 _LIVE = []  # weak references to the library objects, in creation order
@@ -224,7 +224,8 @@ class Matrix:
                     levels_upper=i.levels_upper, row_blocks=i.row_blocks,
                     device_bytes=i.device_bytes, inner_colours=i.inner_colours,
                     spmv_rows=i.spmv_rows, spmv_waves=i.spmv_waves,
-                    inner_block=i.inner_block, inner_levels=i.inner_levels)
+                    inner_block=i.inner_block, inner_levels=i.inner_levels,
+                    ell64_slices=i.ell64_slices)
 
     def tune_spmv(self):
         """Times the SpMV shapes and keeps the fastest; returns (rows, waves)."""
