@@ -443,6 +443,26 @@ def mixed_vs_fp64(args):
             "algorithmic_bytes_formula": per, "launch_us": 1e3 * dom_launch_ms,
             "launches_timed": dom_n.value, "timing": "HIP events around every launch of the "
             "kernel class in the timed solves, on the matrix's stream"}
+    # what a Fortran caller of Solvers_Mod sees for the same FP64 solve: the
+    # values uploaded from host memory before the call (tfs_matrix_set_values,
+    # as Solvers_Mod does on every call), then Cg on host arrays (x and r1
+    # copied in and out by tfs_solve)
+    xh = np.zeros(c.n)
+    rh = np.ascontiguousarray(c.b, np.float64).copy()
+    vh = np.ascontiguousarray(c.val, np.float64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    A.set_values(vh)
+    t1 = time.perf_counter()
+    resh = solvers.Cg(A, xh, rh, solve["prec"], solve["cap"], tol)
+    t2 = time.perf_counter()
+    fortran_call = {"ms": 1e3 * (t2 - t0), "values_upload_ms": 1e3 * (t1 - t0),
+                    "solve_on_host_arrays_ms": 1e3 * (t2 - t1), "niter": resh[0],
+                    "what": "tfs_matrix_set_values from host memory + tfs_solve on host x/r1 "
+                            "(the Solvers_Mod drop-in's calls), FP64 Cg + DIC"}
+    log(f"  Fortran-call view: {fortran_call['ms']:.1f} ms (values upload "
+        f"{fortran_call['values_upload_ms']:.1f} ms, solve on host arrays "
+        f"{fortran_call['solve_on_host_arrays_ms']:.1f} ms, niter {resh[0]})")
     # the FP64 drop-in's SpMV and sweeps, each launch timed with HIP events
     # inside its solves (the breakdown above)
     y = torch.empty_like(b)
@@ -469,6 +489,7 @@ def mixed_vs_fp64(args):
                    "roofline_fp64_spmv": roof64,
                    "roofline_fp64_sweeps": sroof64,
                    "speedup_vs_fp64": speedup, "rel_l2_vs_fp64": err,
+                   "fortran_call_fp64": fortran_call,
                    "fp64": {"solves_per_s": 1e3 / ms64, "ms_per_solve": ms64,
                             "repetitions": args.fp64_reps, "niter": res64[0],
                             "fin_res": res64[2],
