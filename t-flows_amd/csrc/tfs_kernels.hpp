@@ -614,10 +614,14 @@ k_row_sums(int n, const int *__restrict__ s_row, const int *__restrict__ s_src,
 // Reliable-update iteration tail: c += alpha p (the FP32 direction folded into
 // an FP64 correction every iteration -- an FP32 running correction would
 // carry a rounding error that A amplifies by its largest eigenvalue),
-// r -= alpha q (FP32 recurrence residual), partial ||r||^2.
+// r -= alpha q (FP32 recurrence residual), partial ||r||^2.  With n0 > 0 it
+// also does the next sweep's first step for the rows of colour 0 (the first
+// n0 rows of a point-coloured order, which have no lower entries):
+// y = r d, the same operations as that sweep launch, which is then skipped.
 __global__ void __launch_bounds__(kRedThreads)
 k_axpy_mixed(int n, double *__restrict__ c, float *__restrict__ r, const float *__restrict__ p,
-             const float *__restrict__ q, const KState *st, double *partials) {
+             const float *__restrict__ q, const KState *st, double *partials, int n0,
+             const float *__restrict__ d, float *__restrict__ y) {
   if (solve_done(st)) return;
   const double alpha = st->alpha;
   double s = 0.0;
@@ -625,6 +629,7 @@ k_axpy_mixed(int n, double *__restrict__ c, float *__restrict__ r, const float *
     c[i] = c[i] + alpha * (double)p[i];
     const float ri = (float)((double)r[i] - alpha * (double)q[i]);
     r[i] = ri;
+    if (i < n0) y[i] = (float)((double)ri * (double)d[i]);
     s += (double)ri * (double)ri;
   }
   store_partial<kRedThreads>(s, partials);

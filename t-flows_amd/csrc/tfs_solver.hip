@@ -843,7 +843,8 @@ int prepare_colour_sweeps(tfs_matrix *A) {
 // (FP32), factor d (its L order is its S order).  Each launch that writes z
 // leaves per-workgroup partial sums of (r, z) at its own offset; returns
 // their total count, for the rho reduction.
-int colour_sweeps(tfs_matrix *A, const float *r, float *y, float *z, const KState *st) {
+int colour_sweeps(tfs_matrix *A, const float *r, float *y, float *z, const KState *st,
+                  bool y0_ready = false) {
   const float *d = A->v32.dL;
   const int K = A->csw_colours;
   // every launch that writes z keeps its own partial slots: at most
@@ -866,7 +867,9 @@ int colour_sweeps(tfs_matrix *A, const float *r, float *y, float *z, const KStat
     tm_end(TFS_KC_SWEEP);
     if (bwd || last) parts += g;
   };
-  for (int c = 0; c < K; ++c) launch(c, false, c == K - 1);
+  // y0_ready: colour 0's forward step (y = r d, no lower entries) was done by
+  // the kernel that last wrote r (k_axpy_mixed)
+  for (int c = y0_ready && K >= 2 ? 1 : 0; c < K; ++c) launch(c, false, c == K - 1);
   for (int c = K - 2; c >= 0; --c) launch(c, true, false);
   return parts;
 }
@@ -988,13 +991,14 @@ int refresh_inner(tfs_matrix *A) {
 // I (Cg.f90:97-178 with FP32 vectors): z = M^-1 r (FP32 factor and sweeps),
 // rho = (r, z), p = z + beta p, q = A32 p in difference form with (p, q),
 // c += alpha p into the FP64 correction, r -= alpha q with ||r||.
-void reliable_iteration(tfs_matrix *I, int prec, const Vecs<float> &F, double *c, KState *st) {
+void reliable_iteration(tfs_matrix *I, int prec, const Vecs<float> &F, double *c, KState *st,
+                        bool y0_ready) {
   const int n = I->n, g = vgrid(n);
   const bool csw = prec == 2 && I->csw_colours > 0;
   float *z = prec == 0 ? F.r : F.z;
   if (csw) {
     // z in S order (forward result in slot r2), with the rho partial sums
-    finish(colour_sweeps(I, F.r, F.r2, F.z, st), I->partials, EPI_RHO, st);
+    finish(colour_sweeps(I, F.r, F.r2, F.z, st, y0_ready), I->partials, EPI_RHO, st);
   } else {
     if (prec == 2) dic_apply<float>(I, F.r, F.yL, F.zU, st);
     const int mode = prec == 2 ? 2 : prec == 1 ? 1 : 0;
@@ -1005,8 +1009,11 @@ void reliable_iteration(tfs_matrix *I, int prec, const Vecs<float> &F, double *c
   hipLaunchKernelGGL(k_pupdate<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, z, F.p, st);
   launch_ell<true>(I, F.p, F.q, st);
   finish(ell_grid(I), I->partials, EPI_ALPHA, st);
+  // with point-coloured sweeps the tail also does the next sweep's colour-0
+  // step (y0_ready in the next iteration)
+  const int n0 = csw && I->csw_colours >= 2 ? I->csw_cstart[1] : 0;
   hipLaunchKernelGGL(k_axpy_mixed, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, c, F.r, F.p,
-                     F.q, st, I->partials);
+                     F.q, st, I->partials, n0, I->v32.dL, F.r2);
   finish(g, I->partials, EPI_NORM, st);
 }
 
@@ -1100,7 +1107,13 @@ int solve_reliable_device(tfs_matrix *A, int prec, double *x, double *r1, int *n
     }
     TFS_TRY(push_state(I, 0));
     started = true;
-    TFS_TRY(iterate(I, hi->cap, [&] { reliable_iteration(I, prec, F, c, si); }));
+    // the first iteration of a segment starts from a replaced residual: its
+    // sweep does colour 0's step itself; later ones find it done by the tail
+    bool seg_first = true;
+    TFS_TRY(iterate(I, hi->cap, [&] {
+      reliable_iteration(I, prec, F, c, si, !seg_first);
+      seg_first = false;
+    }));
     if (hi->status == kStBreakdown) {
       status = TFS_ERR_BREAKDOWN;
       break;
