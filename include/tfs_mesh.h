@@ -104,6 +104,15 @@ int tfs_layout_block_colour(int n, const int *row, const int *col, int block, in
  * per row.  Returns the number of colours. */
 int tfs_layout_block_colour_bfs(int n, const int *row, const int *col, int block, int *perm,
                                 int *colour, int *block_of);
+/* The brick order of the FP32 inner solves (TFS_INNER_BRICKS; 0-based CRS in
+ * the caller's order, columns ascending, dia the diagonal's entry): rows
+ * grouped by their monotone potentials divided by `edge` (on a
+ * lexicographically numbered grid, edge^3 bricks), groups over `cap` rows
+ * split, blocks coloured greedily; perm = rows by colour, then block, then
+ * caller order; colour[i] and block_of[i] per row (blocks numbered in perm
+ * order).  Returns the number of colours. */
+int tfs_layout_brick_colour(int n, const int *row, const int *col, const int *dia, int edge,
+                            int cap, int *perm, int *colour, int *block_of);
 /* The layouts the solver builds for a matrix (0-based CRS): s_orig = caller
  * row at each S (reverse Cuthill-McKee) position; level of each caller row in
  * the lower dependency DAG of the caller's numbering (forward sweep) and in

@@ -2,6 +2,7 @@
 #include "tfs_layout.hpp"
 
 #include <algorithm>
+#include <unordered_map>
 #include <cstdlib>
 
 namespace tfs {
@@ -309,6 +310,141 @@ int block_colour_order(int n, const int *row, const int *col, int block, std::ve
     for (int i = 0; i < n; ++i) (*colour_out)[i] = bcol[blk[i]];
   }
   if (block_out) *block_out = std::move(blk);
+  return ncol;
+}
+
+int brick_colour_order(int n, const int *row, const int *col, const int *dia, int edge, int cap,
+                       std::vector<int> &perm, std::vector<int> &bstart,
+                       std::vector<int> &bcolour) {
+  perm.clear();
+  bstart.assign(1, 0);
+  bcolour.clear();
+  if (n <= 0) return 0;
+  edge = std::max(1, edge);
+  cap = std::max(1, cap);
+  // classes of the dependencies j -> i by the binary order of i - j
+  constexpr int kClasses = 3;
+  std::vector<long long> hist(32, 0);
+  auto order_of = [](unsigned d) { return 31 - __builtin_clz(d); };
+  long long ntri = 0;
+  for (int i = 0; i < n; ++i)
+    for (int e = row[i]; e < dia[i]; ++e) {
+      hist[order_of((unsigned)(i - col[e]))]++;
+      ++ntri;
+    }
+  std::vector<int> cls(32, 0);
+  {
+    int c = 0;
+    long long acc = 0;
+    bool any = false;
+    for (int o = 0; o < 32; ++o) {
+      if (!hist[o]) continue;
+      if (any && c < kClasses - 1 && acc * kClasses >= (long long)(c + 1) * ntri) ++c;
+      cls[o] = c;
+      acc += hist[o];
+      any = true;
+    }
+  }
+  // potentials, row by row in caller order (dependencies come first)
+  std::vector<int> phi((size_t)n * kClasses, 0);
+  for (int i = 0; i < n; ++i) {
+    int *f = &phi[(size_t)i * kClasses];
+    for (int e = row[i]; e < dia[i]; ++e) {
+      const int j = col[e], c = cls[order_of((unsigned)(i - j))];
+      const int *g = &phi[(size_t)j * kClasses];
+      for (int k = 0; k < kClasses; ++k) f[k] = std::max(f[k], g[k] + (k == c ? 1 : 0));
+    }
+  }
+  // groups: one id per distinct brick key, in order of their first row; a
+  // group's rows in caller order, split into blocks of at most cap rows
+  long long span = 1;
+  for (size_t i = 0; i < phi.size(); ++i) span = std::max<long long>(span, phi[i] / edge + 1);
+  std::unordered_map<long long, int> gid;
+  gid.reserve((size_t)n / (size_t)std::max(1, edge * edge * edge) + 16);
+  std::vector<int> grp(n);
+  std::vector<int> gcount;
+  for (int i = 0; i < n; ++i) {
+    const int *f = &phi[(size_t)i * kClasses];
+    const long long key = (f[0] / edge) + span * ((f[1] / edge) + span * (long long)(f[2] / edge));
+    auto it = gid.find(key);
+    if (it == gid.end()) {
+      it = gid.emplace(key, (int)gcount.size()).first;
+      gcount.push_back(0);
+    }
+    grp[i] = it->second;
+    gcount[(size_t)it->second]++;
+  }
+  std::vector<int>().swap(phi);
+  const int ng = (int)gcount.size();
+  std::vector<int> gfirst((size_t)ng + 1, 0);  // blocks of each group
+  for (int g = 0; g < ng; ++g) gfirst[(size_t)g + 1] = gfirst[(size_t)g] + (gcount[(size_t)g] + cap - 1) / cap;
+  const int nb = gfirst[(size_t)ng];
+  std::vector<int> blk(n), seen_in_group((size_t)ng, 0);
+  for (int i = 0; i < n; ++i) {
+    const int g = grp[i];
+    blk[i] = gfirst[(size_t)g] + seen_in_group[(size_t)g]++ / cap;
+  }
+  // greedy colouring of the block graph (symmetrised pattern), blocks in id order
+  std::vector<int> bstart_c((size_t)nb + 1, 0);
+  for (int i = 0; i < n; ++i) bstart_c[(size_t)blk[i] + 1]++;
+  for (int b = 0; b < nb; ++b) bstart_c[(size_t)b + 1] += bstart_c[(size_t)b];
+  std::vector<int> rows_of((size_t)n), fill(bstart_c.begin(), bstart_c.end() - 1);
+  for (int i = 0; i < n; ++i) rows_of[(size_t)fill[(size_t)blk[i]]++] = i;  // caller order within
+  // upper neighbours too: a block must differ from every adjacent block
+  std::vector<int> deg(n + 1, 0);
+  for (int i = 0; i < n; ++i)
+    for (int e = row[i]; e < row[i + 1]; ++e)
+      if (col[e] != i) {
+        deg[i + 1]++;
+        deg[col[e] + 1]++;
+      }
+  for (int i = 0; i < n; ++i) deg[i + 1] += deg[i];
+  std::vector<int> adj((size_t)deg[n]);
+  {
+    std::vector<int> f2(deg.begin(), deg.end() - 1);
+    for (int i = 0; i < n; ++i)
+      for (int e = row[i]; e < row[i + 1]; ++e)
+        if (col[e] != i) {
+          adj[(size_t)f2[i]++] = col[e];
+          adj[(size_t)f2[col[e]]++] = i;
+        }
+  }
+  std::vector<int> bcol((size_t)nb, -1), taken;
+  int ncol = 0;
+  for (int b = 0; b < nb; ++b) {
+    for (int k = bstart_c[(size_t)b]; k < bstart_c[(size_t)b + 1]; ++k) {
+      const int v = rows_of[(size_t)k];
+      for (int e = deg[v]; e < deg[v + 1]; ++e) {
+        const int bb = blk[(size_t)adj[(size_t)e]];
+        if (bb != b && bcol[(size_t)bb] >= 0) taken[(size_t)bcol[(size_t)bb]] = b;
+      }
+    }
+    int c = 0;
+    while (c < ncol && taken[(size_t)c] == b) ++c;
+    if (c == ncol) {
+      ++ncol;
+      taken.push_back(-1);
+    }
+    bcol[(size_t)b] = c;
+  }
+  // order: colour, block, caller order
+  std::vector<int> cstart((size_t)ncol + 1, 0);
+  for (int b = 0; b < nb; ++b) cstart[(size_t)bcol[(size_t)b] + 1]++;
+  for (int c = 0; c < ncol; ++c) cstart[(size_t)c + 1] += cstart[(size_t)c];
+  std::vector<int> border((size_t)nb);
+  {
+    std::vector<int> f3(cstart.begin(), cstart.end() - 1);
+    for (int b = 0; b < nb; ++b) border[(size_t)f3[(size_t)bcol[(size_t)b]]++] = b;
+  }
+  perm.reserve(n);
+  bstart.reserve((size_t)nb + 1);
+  bcolour.reserve(nb);
+  for (int k = 0; k < nb; ++k) {
+    const int b = border[(size_t)k];
+    for (int q = bstart_c[(size_t)b]; q < bstart_c[(size_t)b + 1]; ++q) perm.push_back(rows_of[(size_t)q]);
+    bstart.push_back((int)perm.size());
+    bcolour.push_back(bcol[(size_t)b]);
+  }
   return ncol;
 }
 

@@ -82,6 +82,23 @@ int block_colour_order(int n, const int *row, const int *col, int block, std::ve
 // row/col/dia are 0-based, validated (diagonal present, columns ascending).
 // identity_s keeps the caller's order as the S order (for a matrix whose rows
 // are already in the order its kernels want, e.g. a block multi-colour copy).
+// Block multi-colour order for the FP32 inner solves, from the caller's
+// numbering: rows grouped by (phi_0 / edge, phi_1 / edge, phi_2 / edge) of the
+// monotone potentials of the lower triangle's dependency graph -- phi_c(i) is
+// the longest path to i counting the dependencies j -> i (j < i) of class c,
+// the classes being the binary orders of the index step i - j merged into
+// three groups of about equal entry counts; on a lexicographically numbered
+// grid (steps 1, nx, nx ny) they are the cell's coordinates, so the groups are
+// edge^3 bricks -- groups larger than `cap` rows split in caller order, and the
+// blocks coloured greedily (in order of their first row) so that no two
+// adjacent blocks share a colour.  perm lists the rows by (colour, block,
+// caller order); bstart (nblocks + 1) gives each block's first position in
+// that order and bcolour its colour.  Returns the number of colours.
+// row/col/dia: 0-based CRS in the caller's order, columns ascending.
+int brick_colour_order(int n, const int *row, const int *col, const int *dia, int edge, int cap,
+                       std::vector<int> &perm, std::vector<int> &bstart,
+                       std::vector<int> &bcolour);
+
 void build_layout(int n, const int *row, const int *col, const int *dia, int spmv_rows,
                   int spmv_cap, int small_level, Layout &out, bool identity_s = false);
 

@@ -1,6 +1,7 @@
 // tfs_layout_capi.cpp -- the solver's host-side layout builders behind a C
 // ABI in the CPU library (libtfs_mesh.so), so that the permutations and level
 // structures the device code relies on can be checked without a GPU.
+#include <cstddef>
 #include <vector>
 
 #include "tfs_layout.hpp"
@@ -57,5 +58,22 @@ extern "C" int tfs_layout_block_colour_bfs(int n, const int *row, const int *col
     colour[i] = c[i];
     block_of[i] = b[i];
   }
+  return ncol;
+}
+
+extern "C" int tfs_layout_brick_colour(int n, const int *row, const int *col, const int *dia,
+                                       int edge, int cap, int *perm, int *colour,
+                                       int *block_of) {
+  if (n < 0 || edge < 1 || cap < 1 ||
+      (n > 0 && (!row || !col || !dia || !perm || !colour || !block_of)))
+    return -1;
+  std::vector<int> p, bstart, bcolour;
+  const int ncol = tfs::brick_colour_order(n, row, col, dia, edge, cap, p, bstart, bcolour);
+  for (std::size_t b = 0; b + 1 < bstart.size(); ++b)
+    for (int k = bstart[b]; k < bstart[b + 1]; ++k) {
+      perm[k] = p[(size_t)k];
+      colour[p[(size_t)k]] = bcolour[b];
+      block_of[p[(size_t)k]] = (int)b;
+    }
   return ncol;
 }

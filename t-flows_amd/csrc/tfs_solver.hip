@@ -274,6 +274,8 @@ inline int ell64_grid(const tfs_matrix *A) { return grid_for(A->e64_nslice, kEll
 // (1022 against 1039 ms of SpMV per solve; 60 against 65) and slower for the
 // FP64 one in RCM order (719 against 662; 42 against 41).
 constexpr int kEll64Xcd = 0, kEllDiffXcd = 1;
+// brick inner order: 4^3 bricks (at most 64 rows a block)
+constexpr int kBrickEdge = 4, kBrickCap = 64;
 // Partial sums a fused-dot SpMV of precision T leaves (one per workgroup).
 template <typename T>
 inline int spmv_parts(const tfs_matrix *A) {
@@ -922,12 +924,34 @@ int build_inner(tfs_matrix *A) {
   TFS_HIP(hipMemcpyAsync(src.data(), A->s_src, sizeof(int) * nnz, hipMemcpyDeviceToHost,
                          cur_stream()));
   TFS_HIP(hipStreamSynchronize(cur_stream()));
-  std::vector<int> perm;
+  std::vector<int> perm, bstart, bcolour;
   int ncol = 0;
+  bool bricks = false;
   if (A->inner_order == TFS_INNER_REFERENCE) {
     // the caller's order (the reference factor, in FP32)
     perm.resize(n);
     TFS_HIP(hipMemcpy(perm.data(), A->s_of_orig, sizeof(int) * n, hipMemcpyDeviceToHost));
+  } else if (A->inner_order == TFS_INNER_BRICKS && n > 0) {
+    // bricks from the caller's numbering: its CRS rebuilt from the S layout
+    std::vector<int> s_orig(n), s_of(n);
+    TFS_HIP(hipMemcpy(s_orig.data(), A->s_orig, sizeof(int) * n, hipMemcpyDeviceToHost));
+    TFS_HIP(hipMemcpy(s_of.data(), A->s_of_orig, sizeof(int) * n, hipMemcpyDeviceToHost));
+    std::vector<int> crow(n + 1, 0), ccol((size_t)nnz), cdia(n, 0);
+    for (int i = 0; i < n; ++i) {
+      const int k = s_of[i];
+      crow[i + 1] = crow[i] + row[k + 1] - row[k];
+      int q = crow[i];
+      for (int e = row[k]; e < row[k + 1]; ++e, ++q) {
+        ccol[(size_t)q] = s_orig[col[e]];  // the S row keeps the caller's entry order
+        if (ccol[(size_t)q] == i) cdia[i] = q;
+      }
+    }
+    std::vector<int> pc;
+    ncol = brick_colour_order(n, crow.data(), ccol.data(), cdia.data(), kBrickEdge, kBrickCap, pc,
+                              bstart, bcolour);
+    perm.resize(n);
+    for (int i = 0; i < n; ++i) perm[i] = s_of[pc[i]];
+    bricks = true;
   } else {
     ncol = A->inner_block > 1
                ? block_colour_order(n, row.data(), col.data(), A->inner_block, perm)
@@ -935,6 +959,7 @@ int build_inner(tfs_matrix *A) {
   }
   std::vector<int> pos(n);
   for (int i = 0; i < n; ++i) pos[perm[i]] = i;
+
   std::vector<int> prow(n + 1, 0), pcol((size_t)nnz), pdia(n, -1), emap((size_t)nnz);
   std::vector<std::pair<int, int>> ent;
   long long q = 0;
@@ -959,7 +984,10 @@ int build_inner(tfs_matrix *A) {
   int rc = upload(A, &A->inner_emap, emap);
   if (rc == 0) rc = upload(A, &A->inner_from_s, from_s);
   if (rc == 0) rc = build_ell(C, Ly);
-  if (rc == 0 && A->inner_order == TFS_INNER_COLOURED && A->inner_block == 1)
+  // the point-coloured copy sweeps with k_colour_sweep; the brick-ordered one
+  // with the synchronisation-free sweeps (its blocks' own DAGs are a few levels
+  // deep: 14 for 4^3 bricks in two colours)
+  if (rc == 0 && !bricks && A->inner_order != TFS_INNER_REFERENCE && A->inner_block == 1)
     rc = build_colour_sweeps(C, Ly);
   if (rc < 0) {
     tfs_matrix_destroy(C);
@@ -1208,7 +1236,7 @@ int solve_mixed_device(tfs_matrix *A, int prec, double *x, double *r1, int *nite
   // The inner solves run on I: this matrix, or its colour-ordered copy when
   // the factor's order matters (prec 2).  The diagonal preconditioners do not
   // depend on the order.
-  const bool coloured = A->inner_order == TFS_INNER_COLOURED && prec == 2;
+  const bool coloured = A->inner_order != TFS_INNER_REFERENCE && prec == 2;
   const bool krylov = A->refine == TFS_REFINE_FCG;
   tfs_matrix *I = A;
   if (coloured) {
@@ -1794,7 +1822,9 @@ extern "C" int tfs_matrix_set_refinement(tfs_matrix *A, int outer) {
 }
 
 extern "C" int tfs_matrix_set_inner_order(tfs_matrix *A, int order) {
-  if (!A || (order != TFS_INNER_REFERENCE && order != TFS_INNER_COLOURED)) return TFS_ERR_INVALID;
+  if (!A || (order != TFS_INNER_REFERENCE && order != TFS_INNER_COLOURED &&
+             order != TFS_INNER_BRICKS))
+    return TFS_ERR_INVALID;
   if (order == A->inner_order) return TFS_OK;
   A->inner_order = order;
   return drop_inner(A);

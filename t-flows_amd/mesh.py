@@ -80,6 +80,9 @@ def _lib():
         lib.tfs_layout_block_colour.restype = ctypes.c_int
         lib.tfs_layout_block_colour.argtypes = [ctypes.c_int, _I32P, _I32P, ctypes.c_int, _I32P,
                                                 _I32P]
+        lib.tfs_layout_brick_colour.restype = ctypes.c_int
+        lib.tfs_layout_brick_colour.argtypes = [ctypes.c_int, _I32P, _I32P, _I32P, ctypes.c_int,
+                                                ctypes.c_int, _I32P, _I32P, _I32P]
         lib.tfs_layout_block_colour_bfs.restype = ctypes.c_int
         lib.tfs_layout_block_colour_bfs.argtypes = [ctypes.c_int, _I32P, _I32P, ctypes.c_int,
                                                     _I32P, _I32P, _I32P]
@@ -223,6 +226,21 @@ def layout_block_colour_bfs(row, col, block):
     k = _lib().tfs_layout_block_colour_bfs(n, row, col, int(block), perm, colour, block_of)
     if k < 0:
         raise ValueError("tfs_layout_block_colour_bfs failed")
+    return k, perm, colour, block_of
+
+
+def layout_brick_colour(row, col, dia, edge=4, cap=64):
+    """Brick order of the FP32 inner solves (TFS_INNER_BRICKS): blocks from
+    the monotone potentials of the caller's lower DAG, coloured greedily:
+    (ncolours, perm, colour, block_of)."""
+    row = np.ascontiguousarray(row, np.int32)
+    n = len(row) - 1
+    perm, colour, block_of = (np.empty(n, np.int32) for _ in range(3))
+    k = _lib().tfs_layout_brick_colour(n, row, np.ascontiguousarray(col, np.int32),
+                                       np.ascontiguousarray(dia, np.int32), int(edge), int(cap),
+                                       perm, colour, block_of)
+    if k < 0:
+        raise ValueError("tfs_layout_brick_colour failed")
     return k, perm, colour, block_of
 
 

@@ -38,7 +38,7 @@ TFS_ERR_DEVICE = -4
 TFS_ERR_NO_DEVICE = -5
 
 CG, BICG, CGS = 0, 1, 2
-TFS_INNER_REFERENCE, TFS_INNER_COLOURED = 0, 1
+TFS_INNER_REFERENCE, TFS_INNER_COLOURED, TFS_INNER_BRICKS = 0, 1, 2
 TFS_REFINE_FCG, TFS_REFINE_DEFECT, TFS_REFINE_RELIABLE = 0, 1, 2
 TFS_SWEEP_LEVELS, TFS_SWEEP_STATIC, TFS_SWEEP_DYNAMIC, TFS_SWEEP_COOPERATIVE = 0, 1, 2, 3
 _METHODS = {"cg": CG, "bicg": BICG, "cgs": CGS}

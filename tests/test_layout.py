@@ -100,3 +100,48 @@ def test_block_colouring(name, block):
         assert bcol[b] == min(set(range(k + 1)) - earlier)
     key = colour[perm] * c.n + perm
     assert np.all(np.diff(key) > 0)
+
+
+@pytest.mark.parametrize("edge", [2, 4])
+def test_brick_order_on_a_lexicographic_box(edge):
+    """On a lexicographically numbered box the potentials are the cells'
+    coordinates: the blocks are edge^3 bricks (clipped at the faces), two
+    colours in a checkerboard of bricks, rows ordered by colour, then brick,
+    then caller order."""
+    nx, ny, nz = 9, 6, 5
+    c = mesh.make_case(kind=mesh.POISSON, nx=nx, ny=ny, nz=nz)
+    k, perm, colour, blk = mesh.layout_brick_colour(c.row, c.col, c.dia, edge=edge)
+    idx = np.arange(c.n)
+    x, y, z = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    key = (x // edge) + 100 * ((y // edge) + 100 * (z // edge))
+    assert k == 2
+    assert sorted(perm.tolist()) == list(range(c.n))
+    # a block is exactly a brick
+    for b in np.unique(blk):
+        assert len(np.unique(key[blk == b])) == 1
+        assert (blk == b).sum() == (key == key[blk == b][0]).sum()
+    # checkerboard of bricks
+    assert np.array_equal(colour, ((x // edge) + (y // edge) + (z // edge) + colour[0]) % 2)
+    # perm: colour, then block (contiguous), then caller order
+    assert np.all(np.diff(colour[perm]) >= 0)
+    starts = np.flatnonzero(np.r_[True, np.diff(blk[perm]) != 0])
+    assert len(starts) == len(np.unique(blk))
+    for s0, s1 in zip(starts, np.r_[starts[1:], c.n]):
+        assert np.all(np.diff(perm[s0:s1]) > 0)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_brick_order_is_a_proper_block_colouring(name):
+    """Any matrix: every row in one block of at most `cap` rows, no two
+    adjacent blocks (rows coupled by an entry) of one colour."""
+    c = mesh.make_case(**CASES[name])
+    k, perm, colour, blk = mesh.layout_brick_colour(c.row, c.col, c.dia, edge=3, cap=20)
+    assert sorted(perm.tolist()) == list(range(c.n))
+    assert np.bincount(blk).max() <= 20
+    for i in range(c.n):
+        for j in neighbours(c, i):
+            if blk[j] != blk[i]:
+                assert colour[j] != colour[i]
+            else:
+                assert colour[j] == colour[i]
+    assert k == colour.max() + 1
