@@ -353,6 +353,9 @@ def fp32_sweep_bytes(n, nnz):
     return 0.5 * (8 * (nnz - n) + 32 * n)
 
 
+INNER_NAMES = {0: "reference", 1: "point-coloured", 2: "4^3-brick two-colour"}
+
+
 def mixed_vs_fp64(args):
     """Workload 3, the headline: the FP32-inner refinement timed over `steps`
     solves of the Poisson system (the value), the plain FP64 Cg drop-in timed
@@ -429,6 +432,13 @@ def mixed_vs_fp64(args):
         kname = "k_ell_diff (FP32 sliced-ELL SpMV, difference form)"
         prof = "k_ell_diff_poisson256_counters"
         per = "8 nnz + 12 N + 4 per SpMV (FP32 values and vectors), one SpMV per FP32 iteration"
+    elif A.info()["inner_order"] == solvers.TFS_INNER_BRICKS:
+        units, per_unit = 2 * inner, fp32_sweep_bytes(c.n, c.nnz)
+        kname = ("k_level / k_sweep_syncfree (FP32 DIC sweeps of the brick-ordered copy: "
+                 f"4^3 bricks in two colours, {A.info()['inner_levels']} levels)")
+        prof = "k_brick_sweep_poisson256_counters"
+        per = ("(8 (nnz - N) + 32 N) / 2 per sweep (FP32 DIC forward + backward pair), two "
+               "sweeps per FP32 iteration, spread over the launches of the sweeps")
     else:
         units, per_unit = 2 * inner, fp32_sweep_bytes(c.n, c.nnz)
         kname = "k_colour_sweep (FP32 point-coloured DIC sweeps, one launch per colour)"
@@ -480,7 +490,7 @@ def mixed_vs_fp64(args):
     out = line(args, args.steps / e32, e32,
                f"Poisson {kw['nx']}^3 stretched to aspect 100 (tanh beta "
                f"{kw['beta']:.4f}), zero-mean RHS, cell 1 pinned: FP32-inner PCG (DIC, "
-               f"{'point-coloured' if A.info()['inner_block'] <= 1 else 'block multi-colour'}"
+               f"{INNER_NAMES.get(A.info()['inner_order'], 'block multi-colour')}"
                f" inner order) with reliable FP64 updates to rel 1e-10; "
                f"plain FP64 Cg + DIC beside it", {
                    "latency_ms": {"p50": float(np.percentile(lat, 50)),
@@ -496,6 +506,7 @@ def mixed_vs_fp64(args):
                             "breakdown_ms_per_solve": {k2: v["ms_per_solve"]
                                                        for k2, v in bd64.items()}},
                    "mixed": {"residual_replacements": steps_out, "fp32_iterations": inner,
+                             "inner_order": INNER_NAMES.get(A.info()["inner_order"]),
                              "fin_res": fin32, "outer": args.refine,
                              "inner_rtol": args.inner_rtol,
                              "inner_block": A.info()["inner_block"],

@@ -118,6 +118,7 @@ typedef struct {
   int inner_block;      /* rows per block of the inner order (1 = points)    */
   int inner_levels;     /* depth of the inner forward sweep (0 until built)  */
   int ell64_slices;     /* FP64 SpMV: sliced-ELL slices (0: CSR kernel)      */
+  int inner_order;      /* the inner order built (TFS_INNER_*; -1 until then) */
 } tfs_matrix_info;
 int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info);
 
@@ -184,8 +185,18 @@ int tfs_matrix_set_spmv_shape(tfs_matrix *A, int rows_per_block, int waves_per_g
  *                        units, each block in the caller's order inside: a
  *                        stronger factor than point colouring (fewer inner
  *                        iterations) whose sweeps are a few levels deep (14
- *                        for 4^3 bricks in two colours). */
-enum { TFS_INNER_REFERENCE = 0, TFS_INNER_COLOURED = 1, TFS_INNER_BRICKS = 2 };
+ *                        for 4^3 bricks in two colours);
+ *   TFS_INNER_AUTO       (default) the bricks for systems of at least 2^23
+ *                        rows whose bricks take two colours (measured: at
+ *                        256^3 2.64 against 3.01 s per solve, at 128^3 233
+ *                        against 223 ms), point colouring otherwise.
+ * tfs_matrix_info.inner_order reports the order built (0, 1 or 2). */
+enum {
+  TFS_INNER_REFERENCE = 0,
+  TFS_INNER_COLOURED = 1,
+  TFS_INNER_BRICKS = 2,
+  TFS_INNER_AUTO = 3
+};
 int tfs_matrix_set_inner_order(tfs_matrix *A, int order);
 /* Block multi-colouring of the inner order: blocks of at most `rows` rows
  * grown breadth-first over the pattern are coloured as units, keeping their
@@ -465,7 +476,8 @@ int tfs_sweep_time(tfs_matrix *A, int reps, double *ms);
 /* Kernel classes for event timing during real solves. */
 enum {
   TFS_KC_SPMV = 0,    /* k_spmv in every mode                                  */
-  TFS_KC_SWEEP = 1,   /* forward / backward preconditioner sweeps (per level) */
+  TFS_KC_SWEEP = 1,   /* forward / backward preconditioner sweeps (one bracket
+                         per sweep, also where a sweep is a launch per level) */
   TFS_KC_FACTOR = 2,  /* Prec_Form level kernels                               */
   TFS_KC_VECTOR = 3,  /* reserved: vector update / reduction kernels           */
   TFS_KC_SPMV32 = 4   /* k_ell_diff, the refinement's FP32 operator            */

@@ -182,7 +182,8 @@ struct tfs_matrix {
   std::vector<void *> allocs;
   // FP32 inner solves of the refinement on a colour-ordered copy P A P^T,
   // P a greedy colouring of the S-order graph (built on first use)
-  int inner_order = TFS_INNER_COLOURED;
+  int inner_order = TFS_INNER_AUTO;
+  int inner_built = -1;           // the order the inner copy was built in
   int inner_block = kInnerBlock;  // rows coloured as one unit in the inner order
   int refine = TFS_REFINE_RELIABLE;  // outer loop of the refinement
   tfs_matrix *inner = nullptr;
@@ -276,6 +277,7 @@ inline int ell64_grid(const tfs_matrix *A) { return grid_for(A->e64_nslice, kEll
 constexpr int kEll64Xcd = 0, kEllDiffXcd = 1;
 // brick inner order: 4^3 bricks (at most 64 rows a block)
 constexpr int kBrickEdge = 4, kBrickCap = 64;
+constexpr int kBrickAutoRows = 1 << 23;  // TFS_INNER_AUTO: bricks from this size
 // Partial sums a fused-dot SpMV of precision T leaves (one per workgroup).
 template <typename T>
 inline int spmv_parts(const tfs_matrix *A) {
@@ -357,8 +359,10 @@ void launch_sweep(tfs_matrix *A, const DevSweep &S, const Tri<T> &Tr, T *d, cons
                  hipGetErrorString(e));
     A->persistent = false;
   }
+  // one launch per level (or run of small levels): the timing brackets the
+  // whole sweep, so that a measured "launch" is one sweep in every schedule
+  tm_begin(kc);
   for (const Segment &g : S.segs) {
-    tm_begin(kc);
     if (g.small) {
       hipLaunchKernelGGL((k_levels_small<T, OP>), dim3(1), dim3(kSmallThreads), 0, cur_stream(),
                          g.l0, g.l1, S.lev, Tr, d, b, x, st);
@@ -367,8 +371,8 @@ void launch_sweep(tfs_matrix *A, const DevSweep &S, const Tri<T> &Tr, T *d, cons
       hipLaunchKernelGGL((k_level<T, OP>), dim3((p1 - p0 + kLevelThreads - 1) / kLevelThreads),
                          dim3(kLevelThreads), 0, cur_stream(), p0, p1, Tr, d, b, x, st);
     }
-    tm_end(kc);
   }
+  tm_end(kc);
 }
 
 template <typename T>
@@ -931,7 +935,9 @@ int build_inner(tfs_matrix *A) {
     // the caller's order (the reference factor, in FP32)
     perm.resize(n);
     TFS_HIP(hipMemcpy(perm.data(), A->s_of_orig, sizeof(int) * n, hipMemcpyDeviceToHost));
-  } else if (A->inner_order == TFS_INNER_BRICKS && n > 0) {
+  } else if ((A->inner_order == TFS_INNER_BRICKS ||
+              (A->inner_order == TFS_INNER_AUTO && A->inner_block == 1 && n >= kBrickAutoRows)) &&
+             n > 0) {
     // bricks from the caller's numbering: its CRS rebuilt from the S layout
     std::vector<int> s_orig(n), s_of(n);
     TFS_HIP(hipMemcpy(s_orig.data(), A->s_orig, sizeof(int) * n, hipMemcpyDeviceToHost));
@@ -952,6 +958,10 @@ int build_inner(tfs_matrix *A) {
     perm.resize(n);
     for (int i = 0; i < n; ++i) perm[i] = s_of[pc[i]];
     bricks = true;
+    if (A->inner_order == TFS_INNER_AUTO && ncol != 2) {  // irregular blocks: points
+      bricks = false;
+      ncol = colour_order(n, row.data(), col.data(), perm, nullptr, 1);
+    }
   } else {
     ncol = A->inner_block > 1
                ? block_colour_order(n, row.data(), col.data(), A->inner_block, perm)
@@ -998,6 +1008,9 @@ int build_inner(tfs_matrix *A) {
   C->coop = A->coop;
   A->inner = C;
   A->inner_colours = ncol;
+  A->inner_built = A->inner_order == TFS_INNER_REFERENCE ? TFS_INNER_REFERENCE
+                   : bricks                              ? TFS_INNER_BRICKS
+                                                         : TFS_INNER_COLOURED;
   A->inner_ready = false;
   A->bytes += C->bytes;
   return 0;
@@ -1767,6 +1780,7 @@ extern "C" int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info) {
   info->inner_block = A->inner_block;
   info->inner_levels = A->inner ? A->inner->L.nlev : 0;
   info->ell64_slices = A->e64_len > 0 ? A->e64_nslice : 0;
+  info->inner_order = A->inner ? A->inner_built : -1;
   return TFS_OK;
 }
 
@@ -1822,9 +1836,7 @@ extern "C" int tfs_matrix_set_refinement(tfs_matrix *A, int outer) {
 }
 
 extern "C" int tfs_matrix_set_inner_order(tfs_matrix *A, int order) {
-  if (!A || (order != TFS_INNER_REFERENCE && order != TFS_INNER_COLOURED &&
-             order != TFS_INNER_BRICKS))
-    return TFS_ERR_INVALID;
+  if (!A || order < TFS_INNER_REFERENCE || order > TFS_INNER_AUTO) return TFS_ERR_INVALID;
   if (order == A->inner_order) return TFS_OK;
   A->inner_order = order;
   return drop_inner(A);

@@ -38,7 +38,7 @@ TFS_ERR_DEVICE = -4
 TFS_ERR_NO_DEVICE = -5
 
 CG, BICG, CGS = 0, 1, 2
-TFS_INNER_REFERENCE, TFS_INNER_COLOURED, TFS_INNER_BRICKS = 0, 1, 2
+TFS_INNER_REFERENCE, TFS_INNER_COLOURED, TFS_INNER_BRICKS, TFS_INNER_AUTO = 0, 1, 2, 3
 TFS_REFINE_FCG, TFS_REFINE_DEFECT, TFS_REFINE_RELIABLE = 0, 1, 2
 TFS_SWEEP_LEVELS, TFS_SWEEP_STATIC, TFS_SWEEP_DYNAMIC, TFS_SWEEP_COOPERATIVE = 0, 1, 2, 3
 _METHODS = {"cg": CG, "bicg": BICG, "cgs": CGS}
@@ -62,7 +62,8 @@ class _Info(ctypes.Structure):
                 ("row_blocks", ctypes.c_int), ("device_bytes", ctypes.c_longlong),
                 ("inner_colours", ctypes.c_int), ("spmv_rows", ctypes.c_int),
                 ("spmv_waves", ctypes.c_int), ("inner_block", ctypes.c_int),
-                ("inner_levels", ctypes.c_int), ("ell64_slices", ctypes.c_int)]
+                ("inner_levels", ctypes.c_int), ("ell64_slices", ctypes.c_int),
+                ("inner_order", ctypes.c_int)]
 
 
 _LIVE = []  # weak references to the library objects, in creation order
@@ -225,7 +226,7 @@ class Matrix:
                     device_bytes=i.device_bytes, inner_colours=i.inner_colours,
                     spmv_rows=i.spmv_rows, spmv_waves=i.spmv_waves,
                     inner_block=i.inner_block, inner_levels=i.inner_levels,
-                    ell64_slices=i.ell64_slices)
+                    ell64_slices=i.ell64_slices, inner_order=i.inner_order)
 
     def tune_spmv(self):
         """Times the SpMV shapes and keeps the fastest; returns (rows, waves)."""
@@ -256,7 +257,8 @@ class Matrix:
                "tfs_matrix_set_inner_block")
 
     def set_inner_order(self, order):
-        """TFS_INNER_REFERENCE or TFS_INNER_COLOURED for solve_mixed."""
+        """TFS_INNER_REFERENCE, TFS_INNER_COLOURED, TFS_INNER_BRICKS or
+        TFS_INNER_AUTO (default) for solve_mixed."""
         _check(self._lib.tfs_matrix_set_inner_order(self._h, int(order)),
                "tfs_matrix_set_inner_order")
 
