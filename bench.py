@@ -354,6 +354,7 @@ def fp32_sweep_bytes(n, nnz):
 
 
 INNER_NAMES = {0: "reference", 1: "point-coloured", 2: "4^3-brick two-colour"}
+INNER_ORDERS = {"reference": 0, "coloured": 1, "bricks": 2, "auto": 3}  # TFS_INNER_* values
 
 
 def mixed_vs_fp64(args):
@@ -377,6 +378,8 @@ def mixed_vs_fp64(args):
                       "defect": solvers.TFS_REFINE_DEFECT}[args.refine])
     if args.inner_block:
         A.set_inner_block(args.inner_block)
+    if args.inner_order:
+        A.set_inner_order(INNER_ORDERS[args.inner_order])
     info = A.info()
     b = torch.from_numpy(c.b).cuda()
     x = torch.zeros(c.n, dtype=torch.float64, device="cuda")
@@ -434,11 +437,15 @@ def mixed_vs_fp64(args):
         per = "8 nnz + 12 N + 4 per SpMV (FP32 values and vectors), one SpMV per FP32 iteration"
     elif A.info()["inner_order"] == solvers.TFS_INNER_BRICKS:
         units, per_unit = 2 * inner, fp32_sweep_bytes(c.n, c.nnz)
-        kname = ("k_level / k_sweep_syncfree (FP32 DIC sweeps of the brick-ordered copy: "
-                 f"4^3 bricks in two colours, {A.info()['inner_levels']} levels)")
+        g = A.info()["inner_sweep_groups"]
+        sched = (f"one persistent launch of {g} workgroups per sweep" if g > 0
+                 else "one launch per level")
+        kname = (("k_sweep_syncfree" if g > 0 else "k_level") +
+                 " (FP32 DIC sweeps of the brick-ordered copy: 4^3 bricks in two colours, "
+                 f"{A.info()['inner_levels']} levels, {sched})")
         prof = "k_brick_sweep_poisson256_counters"
         per = ("(8 (nnz - N) + 32 N) / 2 per sweep (FP32 DIC forward + backward pair), two "
-               "sweeps per FP32 iteration, spread over the launches of the sweeps")
+               "sweeps per FP32 iteration")
     else:
         units, per_unit = 2 * inner, fp32_sweep_bytes(c.n, c.nnz)
         kname = "k_colour_sweep (FP32 point-coloured DIC sweeps, one launch per colour)"
@@ -537,8 +544,7 @@ def batch(args):
     if args.inner_block:
         B.set_inner_block(args.inner_block)
     if args.inner_order:
-        B.set_inner_order(solvers.TFS_INNER_REFERENCE if args.inner_order == "reference"
-                          else solvers.TFS_INNER_COLOURED)
+        B.set_inner_order(INNER_ORDERS[args.inner_order])
     tols = [1e-10 * float(np.linalg.norm(c.b)) for c in sysl]
     bs = [torch.from_numpy(c.b).cuda() for c in sysl]
     xs = [torch.zeros(c.n, dtype=torch.float64, device="cuda") for c in sysl]
@@ -963,8 +969,10 @@ def main():
                     help="workloads 3-4: outer loop of the refinement")
     ap.add_argument("--inner-block", type=int, default=0,
                     help="workloads 3-4: rows per colour block of the inner order (0: default)")
-    ap.add_argument("--inner-order", choices=["reference", "coloured"], default=None,
-                    help="workload 4: the row order of the FP32 inner factor (default coloured)")
+    ap.add_argument("--inner-order", choices=list(INNER_ORDERS), default=None,
+                    help="workloads 3-4: the row order of the FP32 inner factor (default auto: "
+                         "4^3 bricks in two colours from 2^22 rows, point colouring below; the "
+                         "batch of workload 4 takes reference or coloured)")
     ap.add_argument("--cases", type=int, default=24)
     ap.add_argument("--visits", type=int, default=10)
     ap.add_argument("--rate", type=float, default=200.0)

@@ -119,6 +119,9 @@ typedef struct {
   int inner_levels;     /* depth of the inner forward sweep (0 until built)  */
   int ell64_slices;     /* FP64 SpMV: sliced-ELL slices (0: CSR kernel)      */
   int inner_order;      /* the inner order built (TFS_INNER_*; -1 until then) */
+  int sweep_groups;     /* DIC sweeps: workgroups of the persistent launch,
+                           0 for one launch per level                        */
+  int inner_sweep_groups; /* the same for the inner copy (-1 until built)    */
 } tfs_matrix_info;
 int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info);
 
@@ -186,10 +189,11 @@ int tfs_matrix_set_spmv_shape(tfs_matrix *A, int rows_per_block, int waves_per_g
  *                        stronger factor than point colouring (fewer inner
  *                        iterations) whose sweeps are a few levels deep (14
  *                        for 4^3 bricks in two colours);
- *   TFS_INNER_AUTO       (default) the bricks for systems of at least 2^23
- *                        rows whose bricks take two colours (measured: at
- *                        256^3 2.64 against 3.01 s per solve, at 128^3 233
- *                        against 223 ms), point colouring otherwise.
+ *   TFS_INNER_AUTO       (default) the bricks for systems of at least 2^22
+ *                        rows whose bricks take two colours (measured, bricks
+ *                        against point colouring per solve: 256^3 2.48
+ *                        against 3.00 s, 192^3 0.82 against 0.99 s, 128^3 a
+ *                        tie at 0.22 s), point colouring otherwise.
  * tfs_matrix_info.inner_order reports the order built (0, 1 or 2). */
 enum {
   TFS_INNER_REFERENCE = 0,

@@ -42,6 +42,7 @@ constexpr int kChunk = 256;              // threads per persistent-sweep workgro
 constexpr int kPrefetch = 8;             // row entries held in registers
 constexpr unsigned kSpinLimit = 1u << 20;
 constexpr int kSweepGroups = 768;        // cap on a persistent sweep's workgroups
+constexpr int kSweepGroupsWide = 1536;   // the wider grid the schedule tuner also times
 constexpr int kInnerBlock = 1;           // rows per colour block of the FP32 inner order
 
 // EPI_AUX0..5 store the plain sum into aux[k] (dot products), EPI_NAUX0..5

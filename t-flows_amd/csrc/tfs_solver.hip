@@ -176,6 +176,7 @@ struct tfs_matrix {
   bool sweep_tuned = false;    // schedule picked by timing (first factor)
   unsigned int *sweep_ctr = nullptr;  // the dynamic schedule's counters
   int grid_persistent = 0;
+  bool grid_fixed = false;     // grid limited by the caller or the environment
   int sweep_capacity = 1;      // co-resident workgroups of the heaviest sweep kernel
   long long bytes = 0;
   int bandwidth = 0;
@@ -277,7 +278,10 @@ inline int ell64_grid(const tfs_matrix *A) { return grid_for(A->e64_nslice, kEll
 constexpr int kEll64Xcd = 0, kEllDiffXcd = 1;
 // brick inner order: 4^3 bricks (at most 64 rows a block)
 constexpr int kBrickEdge = 4, kBrickCap = 64;
-constexpr int kBrickAutoRows = 1 << 23;  // TFS_INNER_AUTO: bricks from this size
+// TFS_INNER_AUTO: bricks from this size (measured on MI355X, Poisson box of
+// aspect 100, point colouring against bricks per solve: 128^3 217 / 215 ms,
+// 192^3 985 / 818 ms, 256^3 2997 / 2484 ms)
+constexpr int kBrickAutoRows = 1 << 22;
 // Partial sums a fused-dot SpMV of precision T leaves (one per workgroup).
 template <typename T>
 inline int spmv_parts(const tfs_matrix *A) {
@@ -529,11 +533,16 @@ template <typename T> int *vec_count(tfs_matrix *A);
 template <> int *vec_count<double>(tfs_matrix *A) { return &A->n_vec64; }
 template <> int *vec_count<float>(tfs_matrix *A) { return &A->n_vec32; }
 
-// Both sweep schedules give the same bits; on the first factor of a matrix
-// the persistent synchronisation-free sweeps and the per-level launches are
-// timed on it (two forward/backward pairs each) and the faster is kept.  Deep
-// DAGs of small matrices favour the levels (whose small levels run in one
-// workgroup), wide ones the persistent kernel.
+// Every sweep schedule gives the same bits; on the first factor of a matrix
+// the persistent synchronisation-free sweeps (at the default grid and, unless
+// the caller limited it, at a wider one) and the per-level launches are timed
+// on it (two forward/backward pairs each) and the fastest is kept.  Deep DAGs
+// of small matrices favour the levels (whose small levels run in one
+// workgroup); deep wide ones the persistent kernel at the default grid, whose
+// pollers then delay the stores they wait for the least; shallow DAGs with
+// levels of a million rows (the 4^3 brick order of a 256^3 box: 14 levels)
+// the persistent kernel at the wider grid (on MI355X 227 us a sweep with 1536
+// workgroups, 332 with 768, 249 by per-level launches).
 template <typename T>
 int tune_sweeps(tfs_matrix *A) {
   A->sweep_tuned = true;
@@ -541,9 +550,20 @@ int tune_sweeps(tfs_matrix *A) {
   const Vecs<T> V = vecs_of(*vec_slots<T>(A), A->n);
   hipLaunchKernelGGL(k_fill<T>, dim3(vgrid(A->n)), dim3(kRedThreads), 0, cur_stream(), A->n, V.z,
                      T(1));
-  float t[2] = {0.f, 0.f};
-  for (int mode = 0; mode < 2; ++mode) {  // 0 persistent (current), 1 levels
-    A->persistent = mode == 0;
+  const int chunks = (A->n + kChunk - 1) / kChunk;
+  const int grid[3] = {
+      A->grid_persistent,
+      A->grid_fixed ? A->grid_persistent
+                    : std::max(1, std::min({chunks, A->sweep_capacity, kSweepGroupsWide})),
+      A->grid_persistent};
+  float t[3] = {0.f, 0.f, 0.f};
+  for (int mode = 0; mode < 3; ++mode) {  // 0 persistent (default grid), 1 wider, 2 levels
+    if (mode == 1 && grid[1] <= grid[0]) {
+      t[1] = t[0];
+      continue;
+    }
+    A->persistent = mode < 2;
+    A->grid_persistent = grid[mode];
     dic_apply<T>(A, V.z, V.yL, V.zU, nullptr);  // warm
     TFS_HIP(hipEventRecord(A->ev0, cur_stream()));
     for (int k = 0; k < 2; ++k) dic_apply<T>(A, V.z, V.yL, V.zU, nullptr);
@@ -552,7 +572,9 @@ int tune_sweeps(tfs_matrix *A) {
     TFS_HIP(hipEventSynchronize(A->ev1));
     TFS_HIP(hipEventElapsedTime(&t[mode], A->ev0, A->ev1));
   }
-  A->persistent = t[0] <= t[1];
+  const int best = t[0] <= t[1] && t[0] <= t[2] ? 0 : t[1] <= t[2] ? 1 : 2;
+  A->persistent = best < 2;
+  A->grid_persistent = grid[best];
   if (A->persistent) {
     // the level launches left values where the persistent sweeps expect
     // sentinels: re-arm the forward sweep's output in both precisions
@@ -1006,6 +1028,10 @@ int build_inner(tfs_matrix *A) {
   C->persistent = A->persistent;
   C->dynamic = A->dynamic;
   C->coop = A->coop;
+  if (A->grid_fixed) {  // a caller's limit holds for the copy too
+    C->grid_persistent = std::min(C->grid_persistent, A->grid_persistent);
+    C->grid_fixed = true;
+  }
   A->inner = C;
   A->inner_colours = ncol;
   A->inner_built = A->inner_order == TFS_INNER_REFERENCE ? TFS_INNER_REFERENCE
@@ -1616,7 +1642,10 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
   }
   if (const char *e = std::getenv("TFS_SWEEP_GROUPS")) {  // the cap, up to the occupancy
     const int g = std::atoi(e);
-    if (g > 0) A->grid_persistent = std::max(1, std::min({chunks, n_cu * std::max(1, per_cu), g}));
+    if (g > 0) {
+      A->grid_persistent = std::max(1, std::min({chunks, n_cu * std::max(1, per_cu), g}));
+      A->grid_fixed = true;
+    }
   }
   A->bandwidth = Ly.bandwidth;
   // the uploads and fills above ran on the null stream; the matrix stream is
@@ -1763,6 +1792,7 @@ int tfs::sweep_capacity(const tfs_matrix *A) { return A ? A->sweep_capacity : 1;
 void tfs::set_sweep_grid(tfs_matrix *A, int groups) {
   if (!A) return;
   A->grid_persistent = std::max(1, std::min(A->grid_persistent, groups));
+  A->grid_fixed = true;
   if (A->inner) set_sweep_grid(A->inner, groups);
 }
 
@@ -1781,6 +1811,9 @@ extern "C" int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info) {
   info->inner_levels = A->inner ? A->inner->L.nlev : 0;
   info->ell64_slices = A->e64_len > 0 ? A->e64_nslice : 0;
   info->inner_order = A->inner ? A->inner_built : -1;
+  info->sweep_groups = A->persistent ? A->grid_persistent : 0;
+  info->inner_sweep_groups =
+      A->inner ? (A->inner->persistent ? A->inner->grid_persistent : 0) : -1;
   return TFS_OK;
 }
 

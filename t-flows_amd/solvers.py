@@ -63,7 +63,8 @@ class _Info(ctypes.Structure):
                 ("inner_colours", ctypes.c_int), ("spmv_rows", ctypes.c_int),
                 ("spmv_waves", ctypes.c_int), ("inner_block", ctypes.c_int),
                 ("inner_levels", ctypes.c_int), ("ell64_slices", ctypes.c_int),
-                ("inner_order", ctypes.c_int)]
+                ("inner_order", ctypes.c_int), ("sweep_groups", ctypes.c_int),
+                ("inner_sweep_groups", ctypes.c_int)]
 
 
 _LIVE = []  # weak references to the library objects, in creation order
@@ -226,7 +227,8 @@ class Matrix:
                     device_bytes=i.device_bytes, inner_colours=i.inner_colours,
                     spmv_rows=i.spmv_rows, spmv_waves=i.spmv_waves,
                     inner_block=i.inner_block, inner_levels=i.inner_levels,
-                    ell64_slices=i.ell64_slices, inner_order=i.inner_order)
+                    ell64_slices=i.ell64_slices, inner_order=i.inner_order,
+                    sweep_groups=i.sweep_groups, inner_sweep_groups=i.inner_sweep_groups)
 
     def tune_spmv(self):
         """Times the SpMV shapes and keeps the fastest; returns (rows, waves)."""

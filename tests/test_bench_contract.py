@@ -24,3 +24,15 @@ def test_help_lists_the_workloads():
     assert r.returncode == 0
     for flag in ("--workload", "--ingest", "--steps", "--warmup", "--inner-order", "--inner-rtol"):
         assert flag in r.stdout
+
+
+def test_inner_order_names_match_the_library_constants():
+    import importlib.util
+    from tflows_amd import solvers
+    spec = importlib.util.spec_from_file_location("tfs_bench", ROOT / "bench.py")
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    assert bench.INNER_ORDERS == {"reference": solvers.TFS_INNER_REFERENCE,
+                                  "coloured": solvers.TFS_INNER_COLOURED,
+                                  "bricks": solvers.TFS_INNER_BRICKS,
+                                  "auto": solvers.TFS_INNER_AUTO}
