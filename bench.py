@@ -353,7 +353,15 @@ def fp32_sweep_bytes(n, nnz):
     return 0.5 * (8 * (nnz - n) + 32 * n)
 
 
-INNER_NAMES = {0: "reference", 1: "point-coloured", 2: "4^3-brick two-colour"}
+INNER_NAMES = {0: "reference", 1: "point-coloured", 2: "brick two-colour"}
+
+
+def inner_name(info):
+    """The inner order a matrix built, with its brick edge for the brick order."""
+    if info["inner_order"] == 2:
+        e = info["inner_brick_edge"]
+        return f"{e}^3-brick two-colour"
+    return INNER_NAMES.get(info["inner_order"], "block multi-colour")
 INNER_ORDERS = {"reference": 0, "coloured": 1, "bricks": 2, "auto": 3}  # TFS_INNER_* values
 
 
@@ -441,7 +449,8 @@ def mixed_vs_fp64(args):
         sched = (f"one persistent launch of {g} workgroups per sweep" if g > 0
                  else "one launch per level")
         kname = (("k_sweep_syncfree" if g > 0 else "k_level") +
-                 " (FP32 DIC sweeps of the brick-ordered copy: 4^3 bricks in two colours, "
+                 f" (FP32 DIC sweeps of the brick-ordered copy: {A.info()['inner_brick_edge']}^3 "
+                 "bricks in two colours, "
                  f"{A.info()['inner_levels']} levels, {sched})")
         prof = "k_brick_sweep_poisson256_counters"
         per = ("(8 (nnz - N) + 32 N) / 2 per sweep (FP32 DIC forward + backward pair), two "
@@ -497,7 +506,7 @@ def mixed_vs_fp64(args):
     out = line(args, args.steps / e32, e32,
                f"Poisson {kw['nx']}^3 stretched to aspect 100 (tanh beta "
                f"{kw['beta']:.4f}), zero-mean RHS, cell 1 pinned: FP32-inner PCG (DIC, "
-               f"{INNER_NAMES.get(A.info()['inner_order'], 'block multi-colour')}"
+               f"{inner_name(A.info())}"
                f" inner order) with reliable FP64 updates to rel 1e-10; "
                f"plain FP64 Cg + DIC beside it", {
                    "latency_ms": {"p50": float(np.percentile(lat, 50)),
@@ -513,7 +522,7 @@ def mixed_vs_fp64(args):
                             "breakdown_ms_per_solve": {k2: v["ms_per_solve"]
                                                        for k2, v in bd64.items()}},
                    "mixed": {"residual_replacements": steps_out, "fp32_iterations": inner,
-                             "inner_order": INNER_NAMES.get(A.info()["inner_order"]),
+                             "inner_order": inner_name(A.info()),
                              "fin_res": fin32, "outer": args.refine,
                              "inner_rtol": args.inner_rtol,
                              "inner_block": A.info()["inner_block"],
@@ -971,7 +980,8 @@ def main():
                     help="workloads 3-4: rows per colour block of the inner order (0: default)")
     ap.add_argument("--inner-order", choices=list(INNER_ORDERS), default=None,
                     help="workloads 3-4: the row order of the FP32 inner factor (default auto: "
-                         "4^3 bricks in two colours from 2^22 rows, point colouring below; the "
+                         "bricks in two colours from 2^22 rows (4^3, 8^3 from 3*2^21, 16^3 from "
+                         "3*2^22), point colouring below; the "
                          "batch of workload 4 takes reference or coloured)")
     ap.add_argument("--cases", type=int, default=24)
     ap.add_argument("--visits", type=int, default=10)

@@ -122,6 +122,7 @@ typedef struct {
   int sweep_groups;     /* DIC sweeps: workgroups of the persistent launch,
                            0 for one launch per level                        */
   int inner_sweep_groups; /* the same for the inner copy (-1 until built)    */
+  int inner_brick_edge; /* TFS_INNER_BRICKS built: the brick edge, else 0    */
 } tfs_matrix_info;
 int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info);
 
@@ -182,13 +183,14 @@ int tfs_matrix_set_spmv_shape(tfs_matrix *A, int rows_per_block, int waves_per_g
  *                        order).  The inner preconditioner is then the same
  *                        incomplete Cholesky formula on the permuted matrix;
  *                        the refinement still converges to the FP64 tol;
- *   TFS_INNER_BRICKS     blocks of at most 64 rows (4 x 4 x 4 bricks on a
- *                        lexicographically numbered grid, from the monotone
- *                        potentials of the caller's lower DAG) coloured as
- *                        units, each block in the caller's order inside: a
- *                        stronger factor than point colouring (fewer inner
- *                        iterations) whose sweeps are a few levels deep (14
- *                        for 4^3 bricks in two colours);
+ *   TFS_INNER_BRICKS     bricks of E^3 rows (on a lexicographically numbered
+ *                        grid; in general blocks from the monotone potentials
+ *                        of the caller's lower DAG) coloured as units, each
+ *                        block in the caller's order inside: a stronger factor
+ *                        than point colouring (fewer inner iterations) whose
+ *                        sweeps are a few levels deep (4E - 2 for bricks in
+ *                        two colours).  E = 16 from 3 * 2^22 rows, 8 from
+ *                        3 * 2^21, 4 below;
  *   TFS_INNER_AUTO       (default) the bricks for systems of at least 2^22
  *                        rows whose bricks take two colours (measured, bricks
  *                        against point colouring per solve: 256^3 2.48

@@ -185,6 +185,7 @@ struct tfs_matrix {
   // P a greedy colouring of the S-order graph (built on first use)
   int inner_order = TFS_INNER_AUTO;
   int inner_built = -1;           // the order the inner copy was built in
+  int inner_brick = 0;            // its brick edge (bricks only)
   int inner_block = kInnerBlock;  // rows coloured as one unit in the inner order
   int refine = TFS_REFINE_RELIABLE;  // outer loop of the refinement
   tfs_matrix *inner = nullptr;
@@ -276,12 +277,19 @@ inline int ell64_grid(const tfs_matrix *A) { return grid_for(A->e64_nslice, kEll
 // (1022 against 1039 ms of SpMV per solve; 60 against 65) and slower for the
 // FP64 one in RCM order (719 against 662; 42 against 41).
 constexpr int kEll64Xcd = 0, kEllDiffXcd = 1;
-// brick inner order: 4^3 bricks (at most 64 rows a block)
-constexpr int kBrickEdge = 4, kBrickCap = 64;
 // TFS_INNER_AUTO: bricks from this size (measured on MI355X, Poisson box of
-// aspect 100, point colouring against bricks per solve: 128^3 217 / 215 ms,
-// 192^3 985 / 818 ms, 256^3 2997 / 2484 ms)
+// aspect 100, point colouring against 4^3 bricks per solve: 128^3 217 / 215
+// ms, 192^3 985 / 818 ms, 256^3 2997 / 2484 ms)
 constexpr int kBrickAutoRows = 1 << 22;
+// The brick edge grows with the system: larger bricks are a stronger factor
+// (fewer FP32 iterations) whose sweeps have more, narrower levels, and a level
+// costs a dependency round trip however narrow it is; the edge is the largest
+// whose levels keep about 2e5 rows or more (on a lexicographic box a two-colour
+// brick order of edge E has 4E - 2 levels).  Measured per solve, bricks of
+// 4^3 / 8^3 / 16^3: 192^3 818 / 733 / 803 ms, 256^3 2484 / 2167 / 2097 ms
+// (and 6^3 2365, 12^3 2211, 32^3 2428 ms at 256^3).
+constexpr int kBrick8Rows = 3 << 21, kBrick16Rows = 3 << 22;
+inline int brick_edge(int n) { return n >= kBrick16Rows ? 16 : n >= kBrick8Rows ? 8 : 4; }
 // Partial sums a fused-dot SpMV of precision T leaves (one per workgroup).
 template <typename T>
 inline int spmv_parts(const tfs_matrix *A) {
@@ -975,7 +983,9 @@ int build_inner(tfs_matrix *A) {
       }
     }
     std::vector<int> pc;
-    ncol = brick_colour_order(n, crow.data(), ccol.data(), cdia.data(), kBrickEdge, kBrickCap, pc,
+    const int edge = brick_edge(n);
+    A->inner_brick = edge;
+    ncol = brick_colour_order(n, crow.data(), ccol.data(), cdia.data(), edge, edge * edge * edge, pc,
                               bstart, bcolour);
     perm.resize(n);
     for (int i = 0; i < n; ++i) perm[i] = s_of[pc[i]];
@@ -991,6 +1001,36 @@ int build_inner(tfs_matrix *A) {
   }
   std::vector<int> pos(n);
   for (int i = 0; i < n; ++i) pos[perm[i]] = i;
+  if (bricks || (A->inner_order != TFS_INNER_REFERENCE && A->inner_block > 1)) {
+    // Rows of a block order (swept by the generic sweeps) sorted by their
+    // forward level, ties in block order.  Every coupling keeps its direction
+    // (a row's earlier neighbours sit on lower levels), so the factor is the
+    // same; the copy's S order is then its L order and the U order its
+    // reverse, so the forward sweep reads its right-hand side and k_rho the
+    // backward sweep's output in contiguous runs instead of gathering them
+    // row by row from many blocks.  The FP32 SpMV's gathers lose some
+    // locality in this order.  Measured per solve (MI355X, Poisson box of
+    // aspect 100): 16^3 bricks at 256^3 2094 -> 1972 ms (forward sweep
+    // 304 -> 252 us, k_rho 73 -> 55, SpMV 207 -> 221), 8^3 at 192^3 734 -> 723
+    // ms, 4^3 at 256^3 2510 -> 2526 ms.
+    std::vector<int> lev(n, 0), cnt;
+    for (int i = 0; i < n; ++i) {
+      const int sr = perm[i];
+      int l = 0;
+      for (int k = row[sr]; k < row[sr + 1]; ++k) {
+        const int j = pos[col[k]];
+        if (j < i) l = std::max(l, lev[j] + 1);
+      }
+      lev[i] = l;
+      if (l + 1 > (int)cnt.size()) cnt.resize(l + 1, 0);
+      cnt[l]++;
+    }
+    std::vector<int> start(cnt.size() + 1, 0), sorted(n);
+    for (size_t l = 0; l < cnt.size(); ++l) start[l + 1] = start[l] + cnt[l];
+    for (int i = 0; i < n; ++i) sorted[start[lev[i]]++] = perm[i];
+    perm.swap(sorted);
+    for (int i = 0; i < n; ++i) pos[perm[i]] = i;
+  }
 
   std::vector<int> prow(n + 1, 0), pcol((size_t)nnz), pdia(n, -1), emap((size_t)nnz);
   std::vector<std::pair<int, int>> ent;
@@ -1811,6 +1851,7 @@ extern "C" int tfs_matrix_get_info(const tfs_matrix *A, tfs_matrix_info *info) {
   info->inner_levels = A->inner ? A->inner->L.nlev : 0;
   info->ell64_slices = A->e64_len > 0 ? A->e64_nslice : 0;
   info->inner_order = A->inner ? A->inner_built : -1;
+  info->inner_brick_edge = A->inner && A->inner_built == TFS_INNER_BRICKS ? A->inner_brick : 0;
   info->sweep_groups = A->persistent ? A->grid_persistent : 0;
   info->inner_sweep_groups =
       A->inner ? (A->inner->persistent ? A->inner->grid_persistent : 0) : -1;

@@ -64,7 +64,7 @@ class _Info(ctypes.Structure):
                 ("spmv_waves", ctypes.c_int), ("inner_block", ctypes.c_int),
                 ("inner_levels", ctypes.c_int), ("ell64_slices", ctypes.c_int),
                 ("inner_order", ctypes.c_int), ("sweep_groups", ctypes.c_int),
-                ("inner_sweep_groups", ctypes.c_int)]
+                ("inner_sweep_groups", ctypes.c_int), ("inner_brick_edge", ctypes.c_int)]
 
 
 _LIVE = []  # weak references to the library objects, in creation order
@@ -228,7 +228,8 @@ class Matrix:
                     spmv_rows=i.spmv_rows, spmv_waves=i.spmv_waves,
                     inner_block=i.inner_block, inner_levels=i.inner_levels,
                     ell64_slices=i.ell64_slices, inner_order=i.inner_order,
-                    sweep_groups=i.sweep_groups, inner_sweep_groups=i.inner_sweep_groups)
+                    sweep_groups=i.sweep_groups, inner_sweep_groups=i.inner_sweep_groups,
+                    inner_brick_edge=i.inner_brick_edge)
 
     def tune_spmv(self):
         """Times the SpMV shapes and keeps the fastest; returns (rows, waves)."""

@@ -43,6 +43,10 @@ def test_poisson256_refinement_full_size(torch):
     print(f"  refinement: {steps} replacements, {inner} FP32 iterations, fin {fin / tol:.3f} tol",
           flush=True)
     assert A.status == S.TFS_OK and fin < tol
+    # the default inner order at this size: 16^3 bricks in two colours
+    info = A.info()
+    assert (info["inner_order"], info["inner_brick_edge"], info["inner_colours"]) == (
+        S.TFS_INNER_BRICKS, 16, 2), info
     tr = b.clone()
     A.residual(x, tr)  # the true FP64 residual, independently of the solve
     assert float(torch.linalg.norm(tr)) < tol

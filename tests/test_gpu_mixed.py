@@ -214,6 +214,7 @@ def test_brick_inner_order(torch, S):
         steps, inner[order], ini, fin = S.solve_mixed(A, x, r, 2, 50, tol)
         assert A.status == S.TFS_OK and fin < tol
         assert A.info()["inner_colours"] == 2
+        assert A.info()["inner_brick_edge"] == (4 if order == "bricks" else 0)
         assert rel_l2(x.cpu().numpy(), xs) <= 1e-6
         print(order, "steps", steps, "inner", inner[order], "levels", A.info()["inner_levels"],
               flush=True)
