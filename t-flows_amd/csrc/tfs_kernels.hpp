@@ -1168,7 +1168,8 @@ __device__ __forceinline__ void sweep_row(SweepRow<T> &r, const Tri<T> &Tr, T *o
 // unfinished row always belongs to a running workgroup whose dependencies
 // are complete.  (Issuing the next chunk's loads while the current rows
 // wait -- software pipelining -- measured no faster: 128^3 sweeps 75.7 us
-// against 73.2, 256^3 equal.)
+// against 73.2, 256^3 equal; on the 62-level FP32 brick copy of a 256^3 box
+// 389 us against 249, and the 766-level FP64 sweep 1452 against 1435 us.)
 // true -- dynamic: chunks are taken in order from a counter (ctr[0]), so a
 // chunk is only ever held by a running workgroup and the schedule is
 // deadlock-free whatever else shares the GPU (used when several solves run
