@@ -211,6 +211,9 @@ k_rho(int n, const T *__restrict__ r, T *__restrict__ z, const T *__restrict__ d
     } else if (mode == 2) {
       zi = zU[s_to_u[i]];
       z[i] = zi;
+    } else if (mode == 4) {  // the U order is the S order backwards
+      zi = zU[n - 1 - i];
+      z[i] = zi;
     } else {
       zi = z[i];
     }
@@ -924,8 +927,14 @@ struct Tri {
   const int *__restrict__ col;
   const T *__restrict__ val;
   const T *__restrict__ diag;     // factor only
-  const int *__restrict__ in_idx; // sweeps: gather index of the right-hand side
-};
+  const int *__restrict__ in_idx; // sweeps: gather index of the right-hand side; null
+  int in_rev;                     // for the identity (in_rev 0) or the reversal p -> n-1-p
+};                                // (in_rev n), formed instead of loaded
+
+template <typename T>
+__device__ __forceinline__ int tri_src(const Tri<T> &Tr, int p) {
+  return Tr.in_idx ? Tr.in_idx[p] : Tr.in_rev ? Tr.in_rev - 1 - p : p;
+}
 
 enum SweepOp { OP_FACTOR = 0, OP_FWD, OP_BWD };
 
@@ -957,7 +966,7 @@ __device__ __forceinline__ void sweep_row(const Tri<T> &Tr, T *d, const T *b, T 
     s = Tr.diag[p];
   } else {
     dp = d[p];
-    src = Tr.in_idx[p];
+    src = tri_src(Tr, p);
   }
   int c[kPrefetch];
   T a[kPrefetch], v[kPrefetch];
@@ -1051,7 +1060,7 @@ __device__ __forceinline__ void sweep_load1(SweepRow<T> &r, int p, int n, const 
     r.s = Tr.diag[p];
   } else {
     r.dp = d[p];
-    r.src = Tr.in_idx[p];
+    r.src = tri_src(Tr, p);
   }
 }
 

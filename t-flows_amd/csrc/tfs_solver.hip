@@ -154,6 +154,10 @@ struct tfs_matrix {
   int spmv_waves = kSpmvWaves;  // wavefronts per SpMV workgroup
   DevSweep L, U;
   int *l_to_s = nullptr, *u_to_l = nullptr, *s_to_u = nullptr;
+  // permutations that are the identity or the reversal, formed in the
+  // kernels instead of loaded (the U order is the L order backwards; in a
+  // level-sorted copy the L order is the S order)
+  bool l_ident = false, u_rev = false, s_to_u_rev = false;
   int *s_lpos = nullptr;  // L position of the row at S position k
   double *val = nullptr;       // caller-order FP64 values
   Values<double> v64;
@@ -458,9 +462,10 @@ int prec_form(tfs_matrix *A, int prec) {
 template <typename T>
 void dic_apply(tfs_matrix *A, const T *bS, T *yL, T *zU, const KState *st) {
   Values<T> &V = vals<T>(A);
-  const Tri<T> TL{A->L.ptr, A->L.col, V.l_val, nullptr, A->l_to_s};
+  const Tri<T> TL{A->L.ptr, A->L.col, V.l_val, nullptr, A->l_ident ? nullptr : A->l_to_s, 0};
   launch_sweep<T, OP_FWD>(A, A->L, TL, V.dL, bS, yL, st, zU);
-  const Tri<T> TU{A->U.ptr, A->U.col, V.u_val, nullptr, A->u_to_l};
+  const Tri<T> TU{A->U.ptr, A->U.col, V.u_val, nullptr, A->u_rev ? nullptr : A->u_to_l,
+                  A->u_rev ? A->n : 0};
   launch_sweep<T, OP_BWD>(A, A->U, TU, V.dU, yL, zU, st);
 }
 
@@ -599,7 +604,7 @@ template <typename T>
 void cg_iteration(tfs_matrix *A, int prec, const Vecs<T> &V, KState *st) {
   const int n = A->n, g = vgrid(n);
   if (prec == 2) dic_apply<T>(A, V.r, V.yL, V.zU, st);
-  const int mode = prec == 2 ? 2 : prec == 1 ? 1 : 0;
+  const int mode = prec == 2 ? (A->s_to_u_rev ? 4 : 2) : prec == 1 ? 1 : 0;
   T *z = prec == 0 ? V.r : V.z;  // prec 0: z is a copy of r (Prec_Solve.f90:72-75)
   hipLaunchKernelGGL(k_rho<T>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, V.r, V.z,
                      vals<T>(A).s_diag, V.zU, A->s_to_u, mode, st, A->partials);
@@ -1108,7 +1113,7 @@ void reliable_iteration(tfs_matrix *I, int prec, const Vecs<float> &F, double *c
     finish(colour_sweeps(I, F.r, F.r2, F.z, st, y0_ready), I->partials, EPI_RHO, st);
   } else {
     if (prec == 2) dic_apply<float>(I, F.r, F.yL, F.zU, st);
-    const int mode = prec == 2 ? 2 : prec == 1 ? 1 : 0;
+    const int mode = prec == 2 ? (I->s_to_u_rev ? 4 : 2) : prec == 1 ? 1 : 0;
     hipLaunchKernelGGL(k_rho<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.r, F.z,
                        vals<float>(I).s_diag, F.zU, I->s_to_u, mode, st, I->partials);
     finish(g, I->partials, EPI_RHO, st);
@@ -1633,6 +1638,12 @@ int create_device(tfs_matrix *A, const Layout &Ly) {
   TFS_TRY(upload(A, &A->l_to_s, Ly.l_to_s));
   TFS_TRY(upload(A, &A->u_to_l, Ly.u_to_l));
   TFS_TRY(upload(A, &A->s_to_u, Ly.s_to_u));
+  A->l_ident = A->u_rev = A->s_to_u_rev = true;
+  for (int i = 0; i < n; ++i) {
+    A->l_ident = A->l_ident && Ly.l_to_s[(size_t)i] == i;
+    A->u_rev = A->u_rev && Ly.u_to_l[(size_t)i] == n - 1 - i;
+    A->s_to_u_rev = A->s_to_u_rev && Ly.s_to_u[(size_t)i] == n - 1 - i;
+  }
   {
     std::vector<int> lpos((size_t)n);
     for (int k = 0; k < n; ++k) lpos[(size_t)k] = Ly.L.of_orig[(size_t)Ly.s_orig[(size_t)k]];
