@@ -214,6 +214,8 @@ k_rho(int n, const T *__restrict__ r, T *__restrict__ z, const T *__restrict__ d
     } else if (mode == 4) {  // the U order is the S order backwards
       zi = zU[n - 1 - i];
       z[i] = zi;
+    } else if (mode == 5) {  // the same, z not stored (k_pupdate_rev reads zU backwards)
+      zi = zU[n - 1 - i];
     } else {
       zi = z[i];
     }
@@ -230,6 +232,20 @@ k_pupdate(int n, const T *__restrict__ z, T *__restrict__ p, const KState *st) {
   const bool first = st->iter == 0;
   const double beta = st->beta;
   TFS_GRID_STRIDE(i, n) p[i] = first ? z[i] : (T)((double)z[i] + beta * (double)p[i]);
+}
+
+// The same with z read from the backward sweep's output zU, whose order is
+// the S order backwards: z[i] = zU[n-1-i].
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_pupdate_rev(int n, const T *__restrict__ zU, T *__restrict__ p, const KState *st) {
+  if (solve_done(st)) return;
+  const bool first = st->iter == 0;
+  const double beta = st->beta;
+  TFS_GRID_STRIDE(i, n) {
+    const T zi = zU[n - 1 - i];
+    p[i] = first ? zi : (T)((double)zi + beta * (double)p[i]);
+  }
 }
 
 // x += alpha p; r -= alpha q; partial ||r||^2  (Cg.f90:159-172)

@@ -1111,14 +1111,23 @@ void reliable_iteration(tfs_matrix *I, int prec, const Vecs<float> &F, double *c
   if (csw) {
     // z in S order (forward result in slot r2), with the rho partial sums
     finish(colour_sweeps(I, F.r, F.r2, F.z, st, y0_ready), I->partials, EPI_RHO, st);
-  } else {
+  }
+  // z stays in the backward sweep's (reversed) order when that is the S
+  // order backwards: k_rho and the p update read it from there
+  const bool zrev = prec == 2 && !csw && I->s_to_u_rev;
+  if (!csw) {
     if (prec == 2) dic_apply<float>(I, F.r, F.yL, F.zU, st);
-    const int mode = prec == 2 ? (I->s_to_u_rev ? 4 : 2) : prec == 1 ? 1 : 0;
+    const int mode = prec == 2 ? (zrev ? 5 : 2) : prec == 1 ? 1 : 0;
     hipLaunchKernelGGL(k_rho<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, F.r, F.z,
                        vals<float>(I).s_diag, F.zU, I->s_to_u, mode, st, I->partials);
     finish(g, I->partials, EPI_RHO, st);
   }
-  hipLaunchKernelGGL(k_pupdate<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, z, F.p, st);
+  if (zrev)
+    hipLaunchKernelGGL(k_pupdate_rev<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n,
+                       F.zU, F.p, st);
+  else
+    hipLaunchKernelGGL(k_pupdate<float>, dim3(g), dim3(kRedThreads), 0, cur_stream(), n, z, F.p,
+                       st);
   launch_ell<true>(I, F.p, F.q, st);
   finish(ell_grid(I), I->partials, EPI_ALPHA, st);
   // with point-coloured sweeps the tail also does the next sweep's colour-0
