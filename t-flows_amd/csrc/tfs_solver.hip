@@ -462,10 +462,16 @@ int prec_form(tfs_matrix *A, int prec) {
 template <typename T>
 void dic_apply(tfs_matrix *A, const T *bS, T *yL, T *zU, const KState *st) {
   Values<T> &V = vals<T>(A);
-  const Tri<T> TL{A->L.ptr, A->L.col, V.l_val, nullptr, A->l_ident ? nullptr : A->l_to_s, 0};
+  // The FP32 sweeps form an identity or reversed right-hand-side index
+  // instead of loading it; the FP64 ones (latency-bound) measured faster
+  // loading it (256^3: 1430.5 against 1438.3 us per sweep; 128^3: 633.9
+  // against 640.9).
+  const bool form = sizeof(T) == sizeof(float);
+  const Tri<T> TL{A->L.ptr, A->L.col, V.l_val, nullptr,
+                  form && A->l_ident ? nullptr : A->l_to_s, 0};
   launch_sweep<T, OP_FWD>(A, A->L, TL, V.dL, bS, yL, st, zU);
-  const Tri<T> TU{A->U.ptr, A->U.col, V.u_val, nullptr, A->u_rev ? nullptr : A->u_to_l,
-                  A->u_rev ? A->n : 0};
+  const Tri<T> TU{A->U.ptr, A->U.col, V.u_val, nullptr,
+                  form && A->u_rev ? nullptr : A->u_to_l, form && A->u_rev ? A->n : 0};
   launch_sweep<T, OP_BWD>(A, A->U, TU, V.dU, yL, zU, st);
 }
 
