@@ -145,3 +145,39 @@ def test_brick_order_is_a_proper_block_colouring(name):
             else:
                 assert colour[j] == colour[i]
     assert k == colour.max() + 1
+
+
+def forward_levels(c, perm):
+    """Forward levels of the matrix in the order `perm` (position -> caller row):
+    a row's level is one more than its deepest earlier neighbour's."""
+    pos = np.empty(c.n, dtype=np.int64)
+    pos[perm] = np.arange(c.n)
+    lev = np.zeros(c.n, dtype=np.int64)
+    for i in range(c.n):
+        r = perm[i]
+        earlier = [pos[j] for j in neighbours(c, r) if pos[j] < i]
+        lev[i] = 1 + max(lev[e] for e in earlier) if earlier else 0
+    return lev, pos
+
+
+@pytest.mark.parametrize("edge", [2, 4, 8])
+def test_brick_levels_and_level_sort(edge):
+    """A two-colour brick order of edge E on a lexicographic box has 4E - 2
+    forward levels; sorting its rows by forward level (ties in brick order)
+    leaves every row's set of earlier neighbours unchanged, so the incomplete
+    factor in the sorted order is the same."""
+    n1 = 2 * edge
+    c = mesh.make_case(kind=mesh.POISSON, nx=n1, ny=n1, nz=n1)
+    k, perm, colour, blk = mesh.layout_brick_colour(c.row, c.col, c.dia, edge=edge,
+                                                    cap=edge ** 3)
+    assert k == 2
+    lev, pos = forward_levels(c, perm)
+    assert lev.max() + 1 == 4 * edge - 2
+    order = np.argsort(lev, kind="stable")       # sorted position -> brick position
+    sorted_perm = perm[order]
+    lev2, pos2 = forward_levels(c, sorted_perm)
+    assert np.array_equal(np.sort(lev2), np.sort(lev))
+    for r in range(c.n):
+        before = {j for j in neighbours(c, r) if pos[j] < pos[r]}
+        after = {j for j in neighbours(c, r) if pos2[j] < pos2[r]}
+        assert before == after
