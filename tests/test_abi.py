@@ -80,3 +80,30 @@ def test_server_arguments_rejected_without_gpu():
     assert lib.tfs_server_create(1 << 30, 1, None) == -1
     lib.tfs_server_cancel.argtypes = [ctypes.c_void_p, ctypes.c_longlong]
     assert lib.tfs_server_cancel(None, 1) == -1
+
+
+def test_matrix_info_layout_matches_the_header(tmp_path):
+    """The ctypes mirror of tfs_matrix_info has the header's field names,
+    offsets and size (a field appended on one side only would shift every
+    later read)."""
+    import shutil
+    import subprocess
+    from tflows_amd import solvers
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    info = solvers._Info
+    names = [f[0] for f in info._fields_]
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tfs_solver.h"',
+             'int main(void) {', '  printf("size %zu\\n", sizeof(tfs_matrix_info));']
+    lines += [f'  printf("{n} %zu\\n", offsetof(tfs_matrix_info, {n}));' for n in names]
+    lines += ['  return 0;', '}']
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(line.split() for line in subprocess.run(
+        [str(exe)], check=True, capture_output=True, text=True).stdout.splitlines())
+    assert int(out["size"]) == ctypes.sizeof(info)
+    for n in names:
+        assert int(out[n]) == getattr(info, n).offset, n
