@@ -1068,8 +1068,8 @@ int build_inner(tfs_matrix *A) {
   if (rc == 0) rc = upload(A, &A->inner_from_s, from_s);
   if (rc == 0) rc = build_ell(C, Ly);
   // the point-coloured copy sweeps with k_colour_sweep; the brick-ordered one
-  // with the synchronisation-free sweeps (its blocks' own DAGs are a few levels
-  // deep: 14 for 4^3 bricks in two colours)
+  // with the synchronisation-free sweeps (its DAG is a few levels deep: 4E - 2
+  // for E^3 bricks in two colours, 14 / 30 / 62 for E = 4 / 8 / 16)
   if (rc == 0 && !bricks && A->inner_order != TFS_INNER_REFERENCE && A->inner_block == 1)
     rc = build_colour_sweeps(C, Ly);
   if (rc < 0) {
