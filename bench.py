@@ -168,6 +168,8 @@ def measured_traffic(name):
     profiles/<name>.md (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy
     kernel's reading): the line 'traffic_bytes_per_launch: N'.  None if that
     summary is absent."""
+    if name is None:
+        return None
     f = ROOT / "profiles" / f"{name}.md"
     if not f.exists():
         return None
@@ -455,12 +457,24 @@ def mixed_vs_fp64(args):
         prof = "k_brick_sweep_poisson256_counters"
         per = ("(8 (nnz - N) + 32 N) / 2 per sweep (FP32 DIC forward + backward pair), two "
                "sweeps per FP32 iteration")
-    else:
+    elif (A.info()["inner_order"] == solvers.TFS_INNER_COLOURED
+          and A.info()["inner_block"] == 1):
         units, per_unit = 2 * inner, fp32_sweep_bytes(c.n, c.nnz)
         kname = "k_colour_sweep (FP32 point-coloured DIC sweeps, one launch per colour)"
         prof = "k_colour_sweep_poisson256_counters"
         per = ("(8 (nnz - N) + 32 N) / 2 per sweep (FP32 DIC forward + backward pair), two "
                "sweeps per FP32 iteration, spread over the launches of the colours")
+    else:  # the reference order or BFS blocks: the generic sweeps, no counter summary
+        units, per_unit = 2 * inner, fp32_sweep_bytes(c.n, c.nnz)
+        g = A.info()["inner_sweep_groups"]
+        sched = (f"one persistent launch of {g} workgroups per sweep" if g > 0
+                 else "one launch per level")
+        kname = (("k_sweep_syncfree" if g > 0 else "k_level") +
+                 f" (FP32 DIC sweeps of the {inner_name(A.info())}-ordered copy, "
+                 f"{A.info()['inner_levels']} levels, {sched})")
+        prof = None
+        per = ("(8 (nnz - N) + 32 N) / 2 per sweep (FP32 DIC forward + backward pair), two "
+               "sweeps per FP32 iteration")
     algo = per_unit * units * args.steps / max(dom_n.value, 1)  # bytes per launch
     achieved = algo / (dom_launch_ms * 1e-3) / 1e9
     roof = {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
