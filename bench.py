@@ -363,6 +363,8 @@ def inner_name(info):
     if info["inner_order"] == 2:
         e = info["inner_brick_edge"]
         return f"{e}^3-brick two-colour"
+    if info["inner_order"] == 1 and info["inner_block"] > 1:
+        return f"{info['inner_block']}-row block multi-colour"
     return INNER_NAMES.get(info["inner_order"], "block multi-colour")
 INNER_ORDERS = {"reference": 0, "coloured": 1, "bricks": 2, "auto": 3}  # TFS_INNER_* values
 
