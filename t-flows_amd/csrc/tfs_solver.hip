@@ -2542,20 +2542,48 @@ int build_batch_inner(tfs_batch *B) {
   TFS_HIP(hipMemcpyAsync(orig.data(), A->s_orig, sizeof(int) * n, hipMemcpyDeviceToHost,
                          cur_stream()));
   TFS_HIP(hipStreamSynchronize(cur_stream()));
-  std::vector<int> perm, colour;
-  colour_order(n, row.data(), col.data(), perm, &colour, B->inner_block);
-  // system of each S row (from its caller row), then (system, colour, S) order
-  std::vector<int> sys_of(n);
-  for (int k = 0; k < n; ++k) {
-    const int c = orig[k];
-    sys_of[k] = (int)(std::upper_bound(B->row_off.begin(), B->row_off.end(), c) -
-                      B->row_off.begin()) - 1;
-  }
   std::vector<int> order(n);
-  for (int k = 0; k < n; ++k) order[k] = k;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    return sys_of[a] != sys_of[b] ? sys_of[a] < sys_of[b] : colour[a] < colour[b];
-  });
+  if (B->inner_order == TFS_INNER_BRICKS) {
+    // Bricks per system, from the system's own caller numbering and size
+    // (so that a system's copy is the same alone or in any batch), the
+    // systems in their order: (system, colour, brick, caller order).
+    std::vector<int> s_of(n);
+    for (int k = 0; k < n; ++k) s_of[orig[k]] = k;
+    int at = 0;
+    for (int sy = 0; sy < ns; ++sy) {
+      const int r0 = B->row_off[sy], m = B->n_of[sy];
+      std::vector<int> crow(m + 1, 0), ccol, cdia(m, 0), pc, bstart, bcolour;
+      for (int i = 0; i < m; ++i) {
+        const int k = s_of[r0 + i];
+        for (int e = row[k]; e < row[k + 1]; ++e) {  // the caller's entry order
+          const int cj = orig[col[e]] - r0;
+          if (cj == i) cdia[i] = (int)ccol.size();
+          ccol.push_back(cj);
+        }
+        crow[i + 1] = (int)ccol.size();
+      }
+      if (m > 0) {
+        const int edge = brick_edge(m);
+        brick_colour_order(m, crow.data(), ccol.data(), cdia.data(), edge, edge * edge * edge, pc,
+                           bstart, bcolour);
+      }
+      for (int i = 0; i < m; ++i) order[at++] = s_of[r0 + pc[i]];
+    }
+  } else {
+    std::vector<int> perm, colour;
+    colour_order(n, row.data(), col.data(), perm, &colour, B->inner_block);
+    // system of each S row (from its caller row), then (system, colour, S) order
+    std::vector<int> sys_of(n);
+    for (int k = 0; k < n; ++k) {
+      const int c = orig[k];
+      sys_of[k] = (int)(std::upper_bound(B->row_off.begin(), B->row_off.end(), c) -
+                        B->row_off.begin()) - 1;
+    }
+    for (int k = 0; k < n; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      return sys_of[a] != sys_of[b] ? sys_of[a] < sys_of[b] : colour[a] < colour[b];
+    });
+  }
   std::vector<int> pos(n);
   for (int i = 0; i < n; ++i) pos[order[i]] = i;
   std::vector<int> prow(n + 1, 0), pcol((size_t)nnz), pdia(n, -1), emap((size_t)nnz);
@@ -2922,7 +2950,9 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
   if (!B || !d_x || !d_r1 || !tol || !steps || !inner_iters || !ini_res || !fin_res || !status)
     return TFS_ERR_INVALID;
   if (prec < 0 || prec > 2) prec = 0;
-  if (B->refine == TFS_REFINE_RELIABLE && B->inner_order == TFS_INNER_COLOURED)
+  const bool copy_order =
+      B->inner_order == TFS_INNER_COLOURED || B->inner_order == TFS_INNER_BRICKS;
+  if (B->refine == TFS_REFINE_RELIABLE && copy_order)
     return batch_solve_reliable(B, prec, d_x, d_r1, cap, tol, inner_rtol, steps, inner_iters,
                                 ini_res, fin_res, status);
   if (!(inner_rtol > 0.0)) inner_rtol = 1e-2;
@@ -2930,7 +2960,7 @@ extern "C" int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *cons
   tfs_matrix *A = B->A;
   const int n = A->n, ns = B->nsys;
   // the inner solves run on I: this batch, or its colour-ordered copy (prec 2)
-  const bool coloured = B->inner_order == TFS_INNER_COLOURED && prec == 2;
+  const bool coloured = copy_order && prec == 2;
   tfs_batch *I = B;
   if (coloured) {
     TFS_TRY(build_batch_inner(B));
@@ -3175,8 +3205,19 @@ extern "C" int tfs_batch_set_refinement(tfs_batch *B, int outer) {
 }
 
 extern "C" int tfs_batch_set_inner_order(tfs_batch *B, int order) {
-  if (!B || (order != TFS_INNER_REFERENCE && order != TFS_INNER_COLOURED)) return TFS_ERR_INVALID;
+  if (!B || (order != TFS_INNER_REFERENCE && order != TFS_INNER_COLOURED &&
+             order != TFS_INNER_BRICKS))
+    return TFS_ERR_INVALID;
+  if (order == B->inner_order) return TFS_OK;
   B->inner_order = order;
+  if (B->inner) {  // rebuilt on the next refinement solve
+    StreamScope scope(B->A->stream);
+    tfs_batch_destroy(B->inner);
+    B->inner = nullptr;
+    TFS_TRY(dev_free(B->A, B->inner_emap, sizeof(int) * (size_t)B->A->nnz));
+    TFS_TRY(dev_free(B->A, B->inner_from_s, sizeof(int) * (size_t)B->A->n));
+    B->inner_emap = B->inner_from_s = nullptr;
+  }
   return TFS_OK;
 }
 

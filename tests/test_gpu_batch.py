@@ -77,17 +77,19 @@ REFINE = {"reliable": "TFS_REFINE_RELIABLE", "fcg": "TFS_REFINE_FCG", "defect": 
 
 
 @pytest.mark.parametrize("outer", ["reliable", "fcg", "defect"])
-@pytest.mark.parametrize("order", ["reference", "coloured"])
+@pytest.mark.parametrize("order", ["reference", "coloured", "bricks"])
 def test_batch_mixed_refinement(torch, S, oracle, order, outer):
     """Every system reaches its FP64 tolerance (true residual), with the error
     against the exact solution bounded as for the single refinement solve
     (stopping on the residual: within 1e-6, or within 10x of the FP64
-    oracle's own error where that is larger).  The colour-ordered inner
-    copy is the same batch of systems, so its results stay per system.  (With
-    the reference order the reliable-update setting runs flexible CG.)"""
+    oracle's own error where that is larger).  The colour- or brick-ordered
+    inner copy is the same batch of systems, so its results stay per system.
+    (With the reference order the reliable-update setting runs flexible CG.)"""
     sysl = small_batch()
     B = S.Batch([(c.row, c.col, c.dia, c.val) for c in sysl])
-    B.set_inner_order(S.TFS_INNER_COLOURED if order == "coloured" else S.TFS_INNER_REFERENCE)
+    orders = {"reference": S.TFS_INNER_REFERENCE, "coloured": S.TFS_INNER_COLOURED,
+              "bricks": S.TFS_INNER_BRICKS}
+    B.set_inner_order(orders[order])
     B.set_refinement(getattr(S, REFINE[outer]))
     tols = [1e-10 * float(np.linalg.norm(c.b)) for c in sysl]
     xs = [torch.zeros(c.n, dtype=torch.float64, device="cuda") for c in sysl]
@@ -103,9 +105,10 @@ def test_batch_mixed_refinement(torch, S, oracle, order, outer):
         print(f"  system {k}: rel-L2 vs exact {err:.2e} (oracle {rel_l2(ref['x'], xe):.2e})",
               flush=True)
         assert err <= max(1e-6, refinement_bar(ref["x"], xe))
-    if order == "coloured":
+    if order != "reference":
         # the same systems alone in a batch give the same results, bitwise
         B1 = S.Batch([(sysl[0].row, sysl[0].col, sysl[0].dia, sysl[0].val)])
+        B1.set_inner_order(orders[order])
         B1.set_refinement(getattr(S, REFINE[outer]))
         x1 = [torch.zeros(sysl[0].n, dtype=torch.float64, device="cuda")]
         r1 = [torch.from_numpy(sysl[0].b.copy()).cuda()]
