@@ -998,7 +998,7 @@ def main():
                     help="workloads 3-4: the row order of the FP32 inner factor (default auto: "
                          "bricks in two colours from 2^22 rows (4^3, 8^3 from 3*2^21, 16^3 from "
                          "3*2^22), point colouring below; the "
-                         "batch of workload 4 takes reference or coloured)")
+                         "batch of workload 4: bricks per system by default, coloured or reference)")
     ap.add_argument("--cases", type=int, default=24)
     ap.add_argument("--visits", type=int, default=10)
     ap.add_argument("--rate", type=float, default=200.0)

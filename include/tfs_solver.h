@@ -253,11 +253,11 @@ int tfs_batch_solve_mixed_device(tfs_batch *B, int prec, double *const *d_x,
                                  double inner_rtol, int *steps, int *inner_iters,
                                  double *ini_res, double *fin_res, int *status);
 /* Row order of the batch's FP32 inner solves, as tfs_matrix_set_inner_order:
- * TFS_INNER_REFERENCE, TFS_INNER_COLOURED (the default: every system's rows
- * grouped by colour) or TFS_INNER_BRICKS (each system in bricks from its own
- * numbering, the edge chosen by the system's size); the systems stay in
- * their order, so a system's results do not depend on the others.  Changing
- * it drops the inner copy (rebuilt on the next refinement solve). */
+ * TFS_INNER_BRICKS (the default: each system in bricks from its own
+ * numbering, the edge chosen by the system's size), TFS_INNER_COLOURED
+ * (every system's rows grouped by colour) or TFS_INNER_REFERENCE; the systems
+ * stay in their order, so a system's results do not depend on the others.
+ * Changing it drops the inner copy (rebuilt on the next refinement solve). */
 int tfs_batch_set_inner_order(tfs_batch *B, int order);
 int tfs_batch_set_inner_block(tfs_batch *B, int rows);
 /* Outer loop of the batch's refinement, as tfs_matrix_set_refinement

@@ -2273,7 +2273,7 @@ struct tfs_batch {
   int graph_launches = 0;
   // FP32 inner solves of the refinement on a colour-ordered copy (each
   // system's rows grouped by colour, systems in order), built on first use
-  int inner_order = TFS_INNER_COLOURED;
+  int inner_order = TFS_INNER_BRICKS;  // the refinement's copy: bricks per system
   int inner_block = kInnerBlock;
   int refine = TFS_REFINE_RELIABLE;  // outer loop of the refinement
   tfs_batch *inner = nullptr;
